@@ -1,0 +1,18 @@
+// Small dense host kernels for the reducer's bookkeeping matrices (block
+// Gram matrices of the Lanczos process and its projected eigenproblem).
+// Sizes are at most a few hundred; everything is row-major.
+#pragma once
+#include <vector>
+
+namespace fgpu {
+
+// Cholesky A = R^T R of an n x n SPD matrix; returns R (upper) or false.
+bool chol_upper(int n, const std::vector<double>& A, std::vector<double>& R);
+// Inverse of an upper-triangular n x n matrix.
+void inv_upper(int n, const std::vector<double>& R, std::vector<double>& Ri);
+// Symmetric eigen-decomposition: A = V diag(w) V^T, w ascending.
+// Householder tridiagonalisation followed by implicit QL with Wilkinson-type
+// shifts (Golub & Van Loan, "Matrix Computations", sections 8.3 and 8.3.5).
+void sym_eig(int n, const std::vector<double>& A, std::vector<double>& w, std::vector<double>& V);
+
+}  // namespace fgpu

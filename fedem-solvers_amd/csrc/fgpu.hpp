@@ -1,0 +1,107 @@
+// Internal declarations of the MI355X part reducer (host side).
+//
+// Index conventions used throughout the library:
+//   * node / element / DOF indices are 0-based internally; the C ABI takes and
+//     returns the reference's 1-based SAM arrays (MADOF, MEQN, ...).
+//   * "perm space": the order the device works in.  Positions
+//     0 .. ndof1-1 are the internal (status-1) equations in elimination order
+//     (nested dissection, children before parents); positions
+//     ndof1 .. neq-1 are the external (status-2) equations in MEQN2 order, so
+//     the trailing block of the factorisation is the Schur complement.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fgpu {
+
+// Element type codes are the reference's MELCON values (inadd.f90:108-181).
+enum ElemType : int { BEAM2 = 11, ANDES4 = 24, HEX8 = 44 };
+
+// Per-element property record (doubles), shared by all element types.
+enum PropSlot : int {
+  P_E = 0, P_NU = 1, P_RHO = 2, P_G = 3,
+  P_T1 = 4, P_T2 = 5, P_T3 = 6, P_T4 = 7,        // shell thickness per node
+  P_A = 4, P_IY = 5, P_IZ = 6, P_IT = 7,         // beam section
+  P_DX = 8, P_DY = 9, P_DZ = 10,                 // beam local-z direction
+  P_KY = 11, P_KZ = 12,                          // beam shear reduction factors
+  P_NSM = 13,                                    // non-structural mass
+  NPROP = 16
+};
+
+inline int elem_nodes(int t) { return t == HEX8 ? 8 : t == ANDES4 ? 4 : 2; }
+inline int elem_ndn(int t) { return t == HEX8 ? 3 : 6; }
+// Packed element matrix: node-pair blocks (a >= b) of ndn x ndn, row-major
+// within a block, block index a*(a+1)/2 + b.
+inline int elem_psize(int t) {
+  int n = elem_nodes(t), d = elem_ndn(t);
+  return n * (n + 1) / 2 * d * d;
+}
+
+struct Part {
+  int nnod = 0, nel = 0;
+  std::vector<double> xyz;     // 3*nnod
+  std::vector<int> status;     // per node: 1 external, 0 internal, <0 fixed bitmask
+  std::vector<int> etype;      // nel
+  std::vector<int> eptr;       // nel+1
+  std::vector<int> enodes;     // 0-based node indices
+  std::vector<double> eprop;   // NPROP*nel
+};
+
+// SAM arrays (samModule.f90:27-64), 0-based storage of 1-based values where
+// the reference stores equation numbers (meqn* hold 1-based equation numbers).
+struct Sam {
+  int nnod = 0, ndof = 0, neq = 0, ndof1 = 0, ndof2 = 0;
+  std::vector<int> madof;      // nnod+1, 1-based accumulated DOF pointers
+  std::vector<int> msc;        // ndof, 0 fixed / 1 internal / 2 external
+  std::vector<int> meqn;       // ndof, 0 for fixed, else 1-based equation
+  std::vector<int> meqn1, meqn2, dofPosIn2;
+  std::vector<int> nndof;      // per node
+  int nenod = 0;               // number of external nodes
+  int ndim_ext = 0;            // sum of DOFs of external nodes
+};
+
+int build_sam(const Part& p, Sam& s, std::string& err);
+
+// ---------------------------------------------------------------- symbolic
+struct Symbolic {
+  int neq = 0, ndof1 = 0, ndof2 = 0;
+  std::vector<int> eq_to_perm;        // 0-based equation -> perm index
+  std::vector<int> perm_to_eq;
+  // per element, per local DOF: perm index, -1 if not an equation
+  std::vector<int> edof_ptr;          // nel+1
+  std::vector<int> edof_perm;
+  std::vector<int64_t> elem_off;      // offset of each element's packed matrix
+  int64_t elem_pool = 0;              // total doubles of packed element matrices
+
+  // full symmetric CSR in perm space
+  std::vector<int> rowptr, colind;
+  // pull-assembly contributions of every CSR entry: packed element entries
+  std::vector<int> cptr;              // nnz+1
+  std::vector<int64_t> cidx;          // index into the packed element pool
+  // lower CSR entry -> offset into the front pool (-1 for upper entries)
+  std::vector<int64_t> fmap;
+
+  // supernodes in postorder; the last one (index nsup-1) is the Schur front
+  int nsup = 0;
+  std::vector<int> col0, ncol, nrow, parent, level;
+  std::vector<int> rptr;              // nsup+1 -> rows (perm indices)
+  std::vector<int> rows;
+  std::vector<int64_t> foff;          // front pool offsets (nrow^2 each)
+  int64_t fpool = 0;
+  std::vector<int> chptr, chlist;     // children lists
+  std::vector<int> relptr, rel;       // per supernode: update rows -> parent front row
+  int nlevels = 0;                    // levels of factored supernodes
+  std::vector<int> lvptr, lvsup;      // supernodes of each level
+  int64_t factor_flops = 0, nnzL = 0; // algorithmic factor work and nnz(L)
+  int maxfront = 0;
+};
+
+struct SymbolicOptions {
+  int leaf_nodes = 24;  // stop nested dissection below this many nodes
+};
+
+int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
+                   Symbolic& sym, std::string& err);
+
+}  // namespace fgpu

@@ -1,0 +1,40 @@
+// Shared device-side declarations for the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace fgpu {
+
+// Dense multi-vectors (right-hand sides, B columns, Lanczos blocks) are stored
+// row-major: V[row * nrhs + j], so that one matrix row is contiguous.
+//
+// A factorisation task list entry.  The host builds one list per level and
+// panel step; see reducer_gpu.cpp.
+struct FTask {
+  int sn;      // supernode
+  int a, b;    // meaning depends on the kernel (row chunk, tile indices)
+  int k0;      // first pivot column of the panel
+};
+
+struct SolveTask {
+  int sn;
+  int r0, rn;  // target rows of the front work buffer
+  int k0, kn;  // source rows (K dimension)
+  int j0;      // first right-hand side of this task
+};
+
+struct DevSym {
+  const int* col0;
+  const int* ncol;
+  const int* nrow;
+  const int64_t* foff;
+  const int* rptr;
+  const int* rows;
+  const int* chptr;
+  const int* chlist;
+  const int* relptr;
+  const int* rel;
+  const int64_t* woff;  // per-front offsets of solve work buffers (rows)
+};
+
+}  // namespace fgpu

@@ -1,0 +1,38 @@
+// Host-callable launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+#include "fgpu.hpp"
+#include "kern_common.hpp"
+
+namespace fgpu {
+
+void launch_hex8(hipStream_t st, int nhex, const int* elist, const int* eptr, const int* enodes,
+                 const double* xyz, const double* eprop, const int64_t* eoff, double* Kp, double* Mp,
+                 int* ebad);
+void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* Kp,
+                     const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool);
+void launch_fill(hipStream_t st, double* p, int64_t n, double v);
+void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool);
+void launch_panel(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, int* piv);
+void launch_update(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool);
+void launch_solve_gemm(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
+                       double* W, int nrhs, int backward);
+void launch_solve_diag(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
+                       double* W, int nrhs, int backward);
+void launch_fw_init(hipStream_t st, const int* sns, int n, const DevSym& S, const double* R, double* W,
+                    int nrhs, int ndof1);
+void launch_fw_out(hipStream_t st, const int* sns, int n, const DevSym& S, const double* pool,
+                   const double* W, double* Y, int nrhs, int scale, int ndof1);
+void launch_bw_init(hipStream_t st, const int* sns, int n, const DevSym& S, const double* Y,
+                    const double* X, double* W, int nrhs);
+void launch_bw_out(hipStream_t st, const int* sns, int n, const DevSym& S, const double* W, double* X,
+                   int nrhs);
+void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* colind, const double* val,
+                 const double* X, int ldx, double* Z, int ldz);
+int xty_chunks(int n);
+void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Y, int ldy,
+                double* part, double* C);
+void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
+                       double* Y, int ldy, double alpha, double beta);
+void launch_identity_rows(hipStream_t st, double* X, int64_t nrows, int nrhs, int ndof1);
+
+}  // namespace fgpu

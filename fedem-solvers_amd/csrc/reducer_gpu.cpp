@@ -1,0 +1,805 @@
+// Device orchestration of one part reduction on a single MI355X.
+//
+// Phase map to the reference driver (reducer.f90:22-748):
+//   elements + assembly   INADD / csAddEM                    (inadd.f90:26-464)
+//   factor + Schur        csSolveSupEl SPR branch, SPRFC1 +
+//                         SPREXK                              (solExtensionModule.f90:708-775)
+//   B                     SPRBS1/2 per external DOF           (solExtensionModule.f90:809-843)
+//   component modes       EIGVAL -> csLanczosEigenSolve       (cmstrs.f90:53-263)
+//   mass / stiffness      CMSTRS K11, M11, M12, M22           (cmstrs.f90:728-944)
+//   gravity               GRAVfull + GRAV                     (cmstrs.f90:286-474)
+//   dispfile              vgi = K_ii^{-1} g_i                 (cmstrs.f90:674-712)
+#include "reducer_gpu.hpp"
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include "dense_small.hpp"
+#include "kernels.hpp"
+
+namespace fgpu {
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) {                                                                \
+      err = std::string("HIP error '") + hipGetErrorString(e_) + "' in " + #x;             \
+      return -3;                                                                           \
+    }                                                                                      \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    n = count;
+    return hipMalloc((void**)&p, std::max<size_t>(1, count) * sizeof(T));
+  }
+  hipError_t upload(const std::vector<T>& v) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  }
+};
+
+enum OpKind { EXTADD, PANEL, UPDATE, FW_INIT, FW_DIAG, FW_GEMM, FW_OUT, BW_INIT, BW_DIAG, BW_GEMM, BW_OUT };
+struct Op {
+  int kind, off, n;
+};
+
+struct SolvePlan {
+  int nrhs = 0;
+  std::vector<Op> fw_int, fw_all, bw;
+  DBuf<SolveTask> tasks;
+  DBuf<int> sns;
+};
+
+struct DeviceReducer::Impl {
+  // part
+  DBuf<double> xyz, eprop;
+  DBuf<int> eptr, enodes, hexlist;
+  DBuf<int64_t> eoff;
+  // structure
+  DBuf<int> col0, ncol, nrow, rptr, rows, chptr, chlist, relptr, rel, rowptr, colind, cptr;
+  DBuf<int64_t> foff, woff, cidx, fmap;
+  DevSym S{};
+  DBuf<FTask> ftasks;
+  std::vector<Op> fops;
+  std::map<int, std::unique_ptr<SolvePlan>> plans;
+  // values
+  DBuf<double> Kp, Mp, Kv, Mv, pool, W;
+  DBuf<int> ebad, piv;
+  DBuf<double> X, Z, Y, part, small, acc, grav;
+  // component modes
+  DBuf<double> Q, P, Wl, Zl, Yl, Cc, Phi, Tmp;
+  std::vector<int> hexl;
+  int64_t wrows = 0;
+  hipEvent_t ev[12]{};
+};
+
+DeviceReducer::DeviceReducer() : d_(new Impl) {}
+DeviceReducer::~DeviceReducer() {
+  if (st_) {
+    (void)hipStreamSynchronize(st_);
+    for (auto& e : d_->ev)
+      if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(st_);
+  }
+}
+
+static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, std::vector<Op>& ops,
+                                  double& flops) {
+  flops = 0;
+  for (int l = 0; l < y.nlevels; l++) {
+    const int* sups = &y.lvsup[y.lvptr[l]];
+    const int ns = y.lvptr[l + 1] - y.lvptr[l];
+    int off = (int)tasks.size();
+    for (int q = 0; q < ns; q++) {
+      int s = sups[q];
+      if (y.chptr[s + 1] == y.chptr[s]) continue;
+      for (int j0 = 0; j0 < y.nrow[s]; j0 += 64) tasks.push_back({s, j0, 0, 0});
+    }
+    if ((int)tasks.size() > off) ops.push_back({EXTADD, off, (int)tasks.size() - off});
+    int maxnc = 0;
+    for (int q = 0; q < ns; q++) maxnc = std::max(maxnc, y.ncol[sups[q]]);
+    for (int k0 = 0; k0 < maxnc; k0 += 64) {
+      off = (int)tasks.size();
+      for (int q = 0; q < ns; q++) {
+        int s = sups[q];
+        if (y.ncol[s] <= k0) continue;
+        int kb = std::min(64, y.ncol[s] - k0), rem = y.nrow[s] - k0 - kb;
+        int nch = std::max(1, (rem + 255) / 256);
+        for (int a = 0; a < nch; a++) tasks.push_back({s, a, 0, k0});
+      }
+      if ((int)tasks.size() > off) ops.push_back({PANEL, off, (int)tasks.size() - off});
+      off = (int)tasks.size();
+      for (int q = 0; q < ns; q++) {
+        int s = sups[q];
+        if (y.ncol[s] <= k0) continue;
+        int kb = std::min(64, y.ncol[s] - k0), rem = y.nrow[s] - k0 - kb;
+        if (rem <= 0) continue;
+        int T = (rem + 63) / 64;
+        for (int ti = 0; ti < T; ti++)
+          for (int tj = 0; tj <= ti; tj++) tasks.push_back({s, ti, tj, k0});
+        flops += (double)rem * (rem + 1) * kb;  // lower triangle incl. diagonal, 2 flops per FMA
+      }
+      if ((int)tasks.size() > off) ops.push_back({UPDATE, off, (int)tasks.size() - off});
+    }
+  }
+}
+
+static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<SolveTask>& tasks,
+                             std::vector<int>& sns) {
+  const int nrhs = sp.nrhs, schur = y.nsup - 1;
+  auto jtiles = [&](auto f) {
+    for (int j0 = 0; j0 < nrhs; j0 += 64) f(j0);
+  };
+  auto add_sns = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list) {
+    if (list.empty()) return;
+    int off = (int)sns.size();
+    sns.insert(sns.end(), list.begin(), list.end());
+    ops.push_back({kind, off, (int)list.size()});
+  };
+  auto close = [&](std::vector<Op>& ops, int kind, int off) {
+    if ((int)tasks.size() > off) ops.push_back({kind, off, (int)tasks.size() - off});
+  };
+  // forward: levels bottom-up
+  for (int pass = 0; pass < 2; pass++) {
+    std::vector<Op>& ops = pass == 0 ? sp.fw_int : sp.fw_all;
+    for (int l = 0; l < y.nlevels; l++) {
+      std::vector<int> list;
+      int maxT = 0;
+      for (int q = y.lvptr[l]; q < y.lvptr[l + 1]; q++) {
+        int s = y.lvsup[q];
+        if (s == schur && pass == 0) continue;
+        list.push_back(s);
+        maxT = std::max(maxT, (y.ncol[s] + 63) / 64);
+      }
+      add_sns(ops, FW_INIT, list);
+      for (int p = 0; p < maxT; p++) {
+        int off = (int)tasks.size();
+        for (int s : list) {
+          if (y.ncol[s] <= 64 * p) continue;
+          int r0 = 64 * p, rn = std::min(64, y.ncol[s] - r0);
+          jtiles([&](int j0) { tasks.push_back({s, r0, rn, 0, 0, j0}); });
+        }
+        close(ops, FW_DIAG, off);
+        off = (int)tasks.size();
+        for (int s : list) {
+          if (y.ncol[s] <= 64 * p) continue;
+          int k0 = 64 * p, kn = std::min(64, y.ncol[s] - k0);
+          for (int r0 = k0 + kn; r0 < y.nrow[s]; r0 += 64) {
+            int rn = std::min(64, y.nrow[s] - r0);
+            jtiles([&](int j0) { tasks.push_back({s, r0, rn, k0, kn, j0}); });
+          }
+        }
+        close(ops, FW_GEMM, off);
+      }
+      add_sns(ops, FW_OUT, list);
+    }
+  }
+  // backward: levels top-down, internal fronts only
+  for (int l = y.nlevels - 1; l >= 0; l--) {
+    std::vector<int> list;
+    int maxT = 0;
+    for (int q = y.lvptr[l]; q < y.lvptr[l + 1]; q++) {
+      int s = y.lvsup[q];
+      if (s == schur) continue;
+      list.push_back(s);
+      maxT = std::max(maxT, (y.ncol[s] + 63) / 64);
+    }
+    if (list.empty()) continue;
+    add_sns(sp.bw, BW_INIT, list);
+    int off = (int)tasks.size();
+    for (int s : list) {
+      int nu = y.nrow[s] - y.ncol[s];
+      if (nu <= 0) continue;
+      for (int r0 = 0; r0 < y.ncol[s]; r0 += 64) {
+        int rn = std::min(64, y.ncol[s] - r0);
+        jtiles([&](int j0) { tasks.push_back({s, r0, rn, y.ncol[s], nu, j0}); });
+      }
+    }
+    close(sp.bw, BW_GEMM, off);
+    for (int p = 0; p < maxT; p++) {
+      off = (int)tasks.size();
+      for (int s : list) {
+        int T = (y.ncol[s] + 63) / 64, t = T - 1 - p;
+        if (t < 0) continue;
+        int r0 = 64 * t, rn = std::min(64, y.ncol[s] - r0);
+        jtiles([&](int j0) { tasks.push_back({s, r0, rn, 0, 0, j0}); });
+      }
+      close(sp.bw, BW_DIAG, off);
+      off = (int)tasks.size();
+      for (int s : list) {
+        int T = (y.ncol[s] + 63) / 64, t = T - 1 - p;
+        if (t <= 0) continue;
+        int k0 = 64 * t, kn = std::min(64, y.ncol[s] - k0);
+        for (int r0 = 0; r0 < k0; r0 += 64) jtiles([&](int j0) { tasks.push_back({s, r0, 64, k0, kn, j0}); });
+      }
+      close(sp.bw, BW_GEMM, off);
+    }
+    add_sns(sp.bw, BW_OUT, list);
+  }
+}
+
+int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& err) {
+  part_ = p;
+  opt_ = o;
+  if (build_sam(part_, sam_, err)) return -1;
+  if (build_symbolic(part_, sam_, opt_.sym, sym_, err)) return -1;
+  d_->hexl.clear();
+  for (int e = 0; e < part_.nel; e++) {
+    if (part_.etype[e] == HEX8) d_->hexl.push_back(e);
+    else {
+      err = "element type " + std::to_string(part_.etype[e]) + " not supported on the device path yet";
+      return -2;
+    }
+  }
+  uploaded_ = false;
+  return 0;
+}
+
+// Device upload of the part and of every structure array the kernels read.
+int DeviceReducer::upload(std::string& err) {
+  Impl& d = *d_;
+  if (!st_) {
+    HIPCHK(hipStreamCreate(&st_));
+    for (auto& e : d.ev) HIPCHK(hipEventCreate(&e));
+  }
+  HIPCHK(d.xyz.upload(part_.xyz));
+  HIPCHK(d.eprop.upload(part_.eprop));
+  HIPCHK(d.eptr.upload(part_.eptr));
+  HIPCHK(d.enodes.upload(part_.enodes));
+  HIPCHK(d.hexlist.upload(d.hexl));
+  HIPCHK(d.eoff.upload(sym_.elem_off));
+  const Symbolic& y = sym_;
+  HIPCHK(d.col0.upload(y.col0));
+  HIPCHK(d.ncol.upload(y.ncol));
+  HIPCHK(d.nrow.upload(y.nrow));
+  HIPCHK(d.rptr.upload(y.rptr));
+  HIPCHK(d.rows.upload(y.rows));
+  HIPCHK(d.chptr.upload(y.chptr));
+  HIPCHK(d.chlist.upload(y.chlist));
+  HIPCHK(d.relptr.upload(y.relptr));
+  HIPCHK(d.rel.upload(y.rel));
+  HIPCHK(d.foff.upload(y.foff));
+  std::vector<int64_t> woff(y.nsup + 1, 0);
+  for (int s = 0; s < y.nsup; s++) woff[s + 1] = woff[s] + y.nrow[s];
+  d.wrows = woff[y.nsup];
+  HIPCHK(d.woff.upload(woff));
+  HIPCHK(d.rowptr.upload(y.rowptr));
+  HIPCHK(d.colind.upload(y.colind));
+  HIPCHK(d.cptr.upload(y.cptr));
+  HIPCHK(d.cidx.upload(y.cidx));
+  HIPCHK(d.fmap.upload(y.fmap));
+  d.S = DevSym{d.col0.p, d.ncol.p, d.nrow.p, d.foff.p, d.rptr.p, d.rows.p, d.chptr.p,
+               d.chlist.p, d.relptr.p, d.rel.p, d.woff.p};
+  std::vector<FTask> ft;
+  d.fops.clear();
+  build_factor_schedule(y, ft, d.fops, upd_flops_);
+  HIPCHK(d.ftasks.upload(ft));
+  upd_launches_ = 0;
+  for (auto& op : d.fops)
+    if (op.kind == UPDATE) upd_launches_++;
+  d.plans.clear();
+  // value buffers
+  const int64_t nnz = y.rowptr[y.neq];
+  HIPCHK(d.Kp.alloc(y.elem_pool));
+  HIPCHK(d.Mp.alloc(y.elem_pool));
+  HIPCHK(d.Kv.alloc(nnz));
+  HIPCHK(d.Mv.alloc(nnz));
+  HIPCHK(d.pool.alloc(y.fpool));
+  HIPCHK(d.ebad.alloc(part_.nel));
+  HIPCHK(d.piv.alloc(2));
+  // gravity directions: unit accelerations on translational equation DOFs
+  std::vector<double> a((size_t)y.neq * 3, 0.0);
+  for (int i = 0; i < sam_.nnod; i++)
+    for (int j = 0; j < 3 && j < sam_.nndof[i]; j++) {
+      int dof = sam_.madof[i] - 1 + j;
+      if (sam_.meqn[dof] > 0) a[(size_t)y.eq_to_perm[sam_.meqn[dof] - 1] * 3 + j] = 1.0;
+    }
+  HIPCHK(d.grav.upload(a));
+  uploaded_ = true;
+  return 0;
+}
+
+static int get_plan(const Symbolic& y, DeviceReducer::Impl& d, int nrhs, SolvePlan*& out, std::string& err) {
+  auto it = d.plans.find(nrhs);
+  if (it != d.plans.end()) { out = it->second.get(); return 0; }
+  auto sp = std::make_unique<SolvePlan>();
+  sp->nrhs = nrhs;
+  std::vector<SolveTask> tasks;
+  std::vector<int> sns;
+  build_solve_plan(y, *sp, tasks, sns);
+  HIPCHK(sp->tasks.upload(tasks));
+  HIPCHK(sp->sns.upload(sns));
+  out = sp.get();
+  d.plans[nrhs] = std::move(sp);
+  return 0;
+}
+
+// Y = L^{-1} R (scale: D^{-1} L^{-1} R); all: also the Schur front (reduced RHS rows)
+static void run_forward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan& sp, const double* R, double* Yout,
+                        int ndof1, bool scale, bool all) {
+  const std::vector<Op>& ops = all ? sp.fw_all : sp.fw_int;
+  for (const Op& op : ops) {
+    switch (op.kind) {
+      case FW_INIT: launch_fw_init(st, sp.sns.p + op.off, op.n, d.S, R, d.W.p, sp.nrhs, ndof1); break;
+      case FW_DIAG: launch_solve_diag(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, sp.nrhs, 0); break;
+      case FW_GEMM: launch_solve_gemm(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, sp.nrhs, 0); break;
+      case FW_OUT: launch_fw_out(st, sp.sns.p + op.off, op.n, d.S, d.pool.p, d.W.p, Yout, sp.nrhs, scale, ndof1); break;
+    }
+  }
+}
+
+// X = L^{-T} Y on the internal rows; X's boundary rows must be set by the caller
+static void run_backward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan& sp, const double* Yin, double* X) {
+  for (const Op& op : sp.bw) {
+    switch (op.kind) {
+      case BW_INIT: launch_bw_init(st, sp.sns.p + op.off, op.n, d.S, Yin, X, d.W.p, sp.nrhs); break;
+      case BW_DIAG: launch_solve_diag(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, sp.nrhs, 1); break;
+      case BW_GEMM: launch_solve_gemm(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, sp.nrhs, 1); break;
+      case BW_OUT: launch_bw_out(st, sp.sns.p + op.off, op.n, d.S, d.W.p, X, sp.nrhs); break;
+    }
+  }
+}
+
+static double ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+int DeviceReducer::reduce(std::string& err) {
+  if (!uploaded_) {
+    int rc = upload(err);
+    if (rc) return rc;
+  }
+  Impl& d = *d_;
+  const Symbolic& y = sym_;
+  const int neq = y.neq, n1 = y.ndof1, n2 = y.ndof2, schur = y.nsup - 1;
+  const int64_t nnz = y.rowptr[neq];
+  const int nev = opt_.ngen > 0 ? std::max(opt_.neval, opt_.ngen) : 0;
+  const int ngen = opt_.ngen;
+  hipStream_t st = st_;
+  res_ = Results();
+  res_.ndof1 = n1;
+  res_.ndof2 = n2;
+  res_.ngen = ngen;
+  res_.nev = nev;
+  res_.ndim = sam_.ndim_ext + ngen;
+  HIPCHK(hipEventRecord(d.ev[0], st));
+  // ---- element matrices
+  HIPCHK(hipMemsetAsync(d.ebad.p, 0, sizeof(int) * part_.nel, st));
+  launch_hex8(st, (int)d.hexl.size(), d.hexlist.p, d.eptr.p, d.enodes.p, d.xyz.p, d.eprop.p, d.eoff.p, d.Kp.p,
+              d.Mp.p, d.ebad.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(d.ev[1], st));
+  // ---- assembly (CSR K, M and the K entries of the fronts)
+  launch_fill(st, d.pool.p, y.fpool, 0.0);
+  launch_assemble(st, nnz, d.cptr.p, d.cidx.p, d.Kp.p, d.Mp.p, d.Kv.p, d.Mv.p, d.fmap.p, d.pool.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(d.ev[2], st));
+  // ---- multifrontal LDL^T; the Schur front ends up holding K_bb - K_bi K_ii^-1 K_ib
+  {
+    int init[2] = {0, INT_MAX};
+    HIPCHK(hipMemcpyAsync(d.piv.p, init, sizeof init, hipMemcpyHostToDevice, st));
+  }
+  for (const Op& op : d.fops) {
+    switch (op.kind) {
+      case EXTADD: launch_extend_add(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p); break;
+      case PANEL: launch_panel(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.piv.p); break;
+      case UPDATE: launch_update(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p); break;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(d.ev[3], st));
+  // ---- B = -K_ii^{-1} K_ib: back-substitution of [0; I]
+  SolvePlan* pb = nullptr;
+  if (n2 > 0) {
+    if (get_plan(y, d, n2, pb, err)) return -3;
+    HIPCHK(d.W.alloc((size_t)d.wrows * n2));
+    HIPCHK(d.X.alloc((size_t)neq * n2));
+    HIPCHK(d.Z.alloc((size_t)neq * n2));
+    HIPCHK(d.Y.alloc((size_t)neq * n2));
+    launch_identity_rows(st, d.X.p, neq, n2, n1);
+    launch_fill(st, d.Y.p, (int64_t)neq * n2, 0.0);
+    run_backward(st, d, *pb, d.Y.p, d.X.p);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(d.ev[4], st));
+  // error checks of the numeric phase
+  int hbad[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(hbad, d.piv.p, sizeof hbad, hipMemcpyDeviceToHost, st));
+  std::vector<int> eb(part_.nel);
+  HIPCHK(hipMemcpyAsync(eb.data(), d.ebad.p, sizeof(int) * part_.nel, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int e = 0; e < part_.nel; e++)
+    if (eb[e]) {
+      err = "element " + std::to_string(e + 1) + " has a non-positive Jacobian determinant";
+      return -4;
+    }
+  if (hbad[0] > 0) {
+    int eq = y.perm_to_eq[hbad[1]] + 1;
+    err = std::to_string(hbad[0]) + " non-positive pivot(s) in K_ii; first at equation " + std::to_string(eq);
+    return -5;
+  }
+  // ---- component modes (block Lanczos, shift-invert on the resident factor)
+  std::vector<double> Sres;
+  if (nev > 0) {
+    int rc = 0;
+    const int bs = std::max(1, std::min(opt_.lanczos_block, n1));
+    int mcap = opt_.lanczos_maxvec > 0 ? opt_.lanczos_maxvec : std::max(6 * nev + 24, 3 * nev + 12);
+    mcap = std::min(mcap, n1);
+    mcap = std::max(bs, (mcap / bs) * bs);
+    SolvePlan* pl = nullptr;
+    if (get_plan(y, d, bs, pl, err)) return -3;
+    HIPCHK(d.W.alloc((size_t)d.wrows * std::max(bs, n2)));
+    HIPCHK(d.Q.alloc((size_t)neq * mcap));
+    HIPCHK(d.P.alloc((size_t)neq * mcap));
+    HIPCHK(d.Wl.alloc((size_t)neq * bs));
+    HIPCHK(d.Zl.alloc((size_t)neq * bs));
+    HIPCHK(d.Yl.alloc((size_t)neq * bs));
+    HIPCHK(d.Tmp.alloc((size_t)neq * bs));
+    HIPCHK(d.Cc.alloc((size_t)mcap * std::max(bs, nev) + 64));
+    HIPCHK(d.part.alloc((size_t)xty_chunks(neq) * mcap * std::max(bs, nev)));
+    HIPCHK(d.small.alloc((size_t)mcap * mcap + 64));
+    // start block: deterministic pseudo-random values on the internal rows
+    {
+      std::vector<double> q0((size_t)neq * bs, 0.0);
+      uint64_t s = opt_.seed;
+      for (int r = 0; r < n1; r++)
+        for (int j = 0; j < bs; j++) {
+          s += 0x9E3779B97F4A7C15ull;
+          uint64_t z = s;
+          z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+          z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+          z ^= z >> 31;
+          q0[(size_t)r * bs + j] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        }
+      HIPCHK(hipMemcpyAsync(d.Wl.p, q0.data(), q0.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    std::vector<double> G((size_t)bs * bs), R, Ri, Cfull;
+    std::vector<double> H((size_t)mcap * mcap, 0.0);
+    // M-orthonormalise W into block j of Q (and M W into P); returns R with W = Q_j R
+    auto orth_block = [&](int j, std::vector<double>& Rtot) -> int {
+      std::vector<double> Racc((size_t)bs * bs, 0.0);
+      for (int i = 0; i < bs; i++) Racc[(size_t)i * bs + i] = 1.0;
+      for (int pass = 0; pass < 2; pass++) {
+        launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs);
+        launch_xty(st, neq, bs, bs, d.Wl.p, bs, d.Zl.p, bs, d.part.p, d.Cc.p);
+        HIPCHK(hipMemcpyAsync(G.data(), d.Cc.p, sizeof(double) * bs * bs, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int a = 0; a < bs; a++)
+          for (int b = 0; b < a; b++) G[(size_t)a * bs + b] = G[(size_t)b * bs + a] = 0.5 * (G[(size_t)a * bs + b] + G[(size_t)b * bs + a]);
+        if (!chol_upper(bs, G, R)) { err = "Lanczos: block breakdown (Gram matrix not positive definite)"; return -6; }
+        inv_upper(bs, R, Ri);
+        HIPCHK(hipMemcpyAsync(d.small.p, Ri.data(), sizeof(double) * bs * bs, hipMemcpyHostToDevice, st));
+        // Q_j = W R^-1 (via Tmp), P_j = Z R^-1
+        launch_gemm_small(st, neq, bs, bs, d.Wl.p, bs, d.small.p, d.Tmp.p, bs, 0.0, 1.0);
+        launch_gemm_small(st, neq, bs, bs, d.Zl.p, bs, d.small.p, d.Yl.p, bs, 0.0, 1.0);
+        HIPCHK(hipMemcpyAsync(d.Wl.p, d.Tmp.p, sizeof(double) * neq * bs, hipMemcpyDeviceToDevice, st));
+        // Racc = R * Racc
+        std::vector<double> T2((size_t)bs * bs, 0.0);
+        for (int a = 0; a < bs; a++)
+          for (int b = 0; b < bs; b++) {
+            double s = 0;
+            for (int k = 0; k < bs; k++) s += R[(size_t)a * bs + k] * Racc[(size_t)k * bs + b];
+            T2[(size_t)a * bs + b] = s;
+          }
+        Racc.swap(T2);
+      }
+      HIPCHK(hipMemcpy2DAsync(d.Q.p + (size_t)j * bs, sizeof(double) * mcap, d.Wl.p, sizeof(double) * bs,
+                              sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
+      HIPCHK(hipMemcpy2DAsync(d.P.p + (size_t)j * bs, sizeof(double) * mcap, d.Yl.p, sizeof(double) * bs,
+                              sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
+      Rtot.swap(Racc);
+      return 0;
+    };
+    std::vector<double> R0;
+    if ((rc = orth_block(0, R0))) return rc;
+    int nb = 0;  // number of completed blocks in H
+    std::vector<double> theta, Svec, Rlast;
+    bool conv = false;
+    for (int j = 0; (j + 1) * bs <= mcap; j++) {
+      const int mj = (j + 1) * bs;
+      // W = K_ii^{-1} (M Q_j); P_j is stored in Yl after orth_block
+      HIPCHK(hipMemcpy2DAsync(d.Tmp.p, sizeof(double) * bs, d.P.p + (size_t)j * bs, sizeof(double) * mcap,
+                              sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
+      run_forward(st, d, *pl, d.Tmp.p, d.Yl.p, n1, true, false);
+      launch_fill(st, d.Wl.p, (int64_t)neq * bs, 0.0);  // boundary rows stay zero
+      run_backward(st, d, *pl, d.Yl.p, d.Wl.p);
+      // two passes of full reorthogonalisation against Q_0..Q_j
+      Cfull.assign((size_t)mj * bs, 0.0);
+      for (int pass = 0; pass < 2; pass++) {
+        launch_xty(st, neq, mj, bs, d.P.p, mcap, d.Wl.p, bs, d.part.p, d.Cc.p);
+        launch_gemm_small(st, neq, mj, bs, d.Q.p, mcap, d.Cc.p, d.Wl.p, bs, 1.0, -1.0);
+        std::vector<double> C((size_t)mj * bs);
+        HIPCHK(hipMemcpyAsync(C.data(), d.Cc.p, sizeof(double) * mj * bs, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (size_t q = 0; q < C.size(); q++) Cfull[q] += C[q];
+      }
+      for (int a = 0; a < mj; a++)
+        for (int b = 0; b < bs; b++) H[(size_t)a * mcap + j * bs + b] = Cfull[(size_t)a * bs + b];
+      nb = j + 1;
+      if (mj + bs > mcap) break;
+      std::vector<double> Rn;
+      if ((rc = orth_block(j + 1, Rn))) return rc;
+      for (int a = 0; a < bs; a++)
+        for (int b = 0; b < bs; b++) H[(size_t)(mj + a) * mcap + j * bs + b] = Rn[(size_t)a * bs + b];
+      // convergence test on the projected problem once enough vectors exist
+      if (mj >= nev + bs) {
+        std::vector<double> Hs((size_t)mj * mj);
+        for (int a = 0; a < mj; a++)
+          for (int b = 0; b < mj; b++) {
+            double h1 = H[(size_t)a * mcap + b], h2 = H[(size_t)b * mcap + a];
+            Hs[(size_t)a * mj + b] = (a <= b + bs && b <= a + bs) ? 0.5 * (h1 + h2) : 0.0;
+          }
+        std::vector<double> w, V;
+        sym_eig(mj, Hs, w, V);
+        // largest theta first
+        bool ok = true;
+        for (int i = 0; i < nev && ok; i++) {
+          int c = mj - 1 - i;
+          double th = w[c], r2 = 0.0;
+          for (int a = 0; a < bs; a++) {
+            double s = 0;
+            for (int b = 0; b < bs; b++) s += Rn[(size_t)a * bs + b] * V[(size_t)(mj - bs + b) * mj + c];
+            r2 += s * s;
+          }
+          if (!(std::sqrt(r2) <= opt_.tol_eig * std::fabs(th))) ok = false;
+        }
+        if (ok) { conv = true; break; }
+      }
+    }
+    const int m = nb * bs;
+    std::vector<double> Hs((size_t)m * m);
+    for (int a = 0; a < m; a++)
+      for (int b = 0; b < m; b++) {
+        double h1 = H[(size_t)a * mcap + b], h2 = H[(size_t)b * mcap + a];
+        Hs[(size_t)a * m + b] = (a <= b + bs && b <= a + bs) ? 0.5 * (h1 + h2) : 0.0;
+      }
+    std::vector<double> w, V;
+    sym_eig(m, Hs, w, V);
+    (void)conv;
+    res_.lanczos_vectors = m;
+    // Ritz vectors for the nev largest theta (lowest lambda), lambda ascending
+    std::vector<double> S((size_t)m * nev);
+    res_.lambda.resize(nev);
+    res_.eval.resize(nev);
+    for (int i = 0; i < nev; i++) {
+      int c = m - 1 - i;
+      double lam = 1.0 / w[c] + opt_.eig_shift;
+      res_.lambda[i] = lam;
+      res_.eval[i] = lam < 0 ? -std::sqrt(-lam) : std::sqrt(lam);
+      for (int a = 0; a < m; a++) S[(size_t)a * nev + i] = V[(size_t)a * m + c];
+    }
+    HIPCHK(hipMemcpyAsync(d.small.p, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(d.Phi.alloc((size_t)neq * nev));
+    launch_gemm_small(st, neq, m, nev, d.Q.p, mcap, d.small.p, d.Phi.p, nev, 0.0, 1.0);
+    // sign convention: the largest-magnitude component of every mode is positive
+    std::vector<double> ph((size_t)neq * nev);
+    HIPCHK(hipMemcpyAsync(ph.data(), d.Phi.p, sizeof(double) * ph.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> sg((size_t)nev * nev, 0.0);
+    for (int i = 0; i < nev; i++) {
+      double best = -1.0, val = 1.0;
+      for (int q = 0; q < n1; q++) {  // equation order, first maximum wins
+        double v = ph[(size_t)y.eq_to_perm[sam_.meqn1[q] - 1] * nev + i];
+        if (std::fabs(v) > best) { best = std::fabs(v); val = v; }
+      }
+      sg[(size_t)i * nev + i] = val < 0 ? -1.0 : 1.0;
+    }
+    HIPCHK(hipMemcpyAsync(d.small.p, sg.data(), sizeof(double) * sg.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(d.Tmp.alloc((size_t)neq * std::max(bs, nev)));
+    launch_gemm_small(st, neq, nev, nev, d.Phi.p, nev, d.small.p, d.Tmp.p, nev, 0.0, 1.0);
+    HIPCHK(hipMemcpyAsync(d.Phi.p, d.Tmp.p, sizeof(double) * neq * nev, hipMemcpyDeviceToDevice, st));
+  }
+  HIPCHK(hipEventRecord(d.ev[5], st));
+  // ---- reduced mass: Z = M [B; I], M22 = [B; I]^T Z ; modes: K11, M11, M12
+  std::vector<double> M22, K11, M11, M12, Ge, Gg, S22((size_t)n2 * n2);
+  const int maxab = std::max(std::max(n2, 3), std::max(nev, 1));
+  HIPCHK(d.part.alloc((size_t)xty_chunks(neq) * maxab * maxab));
+  HIPCHK(d.acc.alloc((size_t)maxab * maxab));
+  auto xty_host = [&](int na, int nb, const double* X, int ldx, const double* Yv, int ldy,
+                      std::vector<double>& out) -> int {
+    out.assign((size_t)na * nb, 0.0);
+    if (na == 0 || nb == 0) return 0;
+    launch_xty(st, neq, na, nb, X, ldx, Yv, ldy, d.part.p, d.acc.p);
+    HIPCHK(hipMemcpyAsync(out.data(), d.acc.p, sizeof(double) * na * nb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  };
+  if (!opt_.no_mass && n2 > 0) {
+    launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Mv.p, d.X.p, n2, d.Z.p, n2);
+    if (xty_host(n2, n2, d.X.p, n2, d.Z.p, n2, M22)) return -3;
+  }
+  if (ngen > 0) {
+    HIPCHK(d.Yl.alloc((size_t)neq * std::max(ngen, opt_.lanczos_block)));
+    launch_spmm(st, neq, ngen, d.rowptr.p, d.colind.p, d.Kv.p, d.Phi.p, nev, d.Yl.p, ngen);
+    if (xty_host(ngen, ngen, d.Phi.p, nev, d.Yl.p, ngen, K11)) return -3;
+    if (!opt_.no_mass) {
+      launch_spmm(st, neq, ngen, d.rowptr.p, d.colind.p, d.Mv.p, d.Phi.p, nev, d.Yl.p, ngen);
+      if (xty_host(ngen, ngen, d.Phi.p, nev, d.Yl.p, ngen, M11)) return -3;
+      if (n2 > 0 && xty_host(ngen, n2, d.Phi.p, nev, d.Z.p, n2, M12)) return -3;
+    }
+  }
+  HIPCHK(hipEventRecord(d.ev[6], st));
+  // ---- gravity: g = M a ; G_e = [B; I]^T g ; G_g = Phi^T g
+  if (!opt_.no_mass) {
+    HIPCHK(d.Tmp.alloc((size_t)neq * std::max(3, std::max(opt_.lanczos_block, nev))));
+    launch_spmm(st, neq, 3, d.rowptr.p, d.colind.p, d.Mv.p, d.grav.p, 3, d.Tmp.p, 3);
+    if (n2 > 0 && xty_host(n2, 3, d.X.p, n2, d.Tmp.p, 3, Ge)) return -3;
+    if (ngen > 0 && xty_host(ngen, 3, d.Phi.p, nev, d.Tmp.p, 3, Gg)) return -3;
+    if (opt_.want_vgi && n1 > 0) {
+      SolvePlan* p3 = nullptr;
+      if (get_plan(y, d, 3, p3, err)) return -3;
+      HIPCHK(d.W.alloc((size_t)d.wrows * std::max(std::max(n2, 3), opt_.lanczos_block)));
+      HIPCHK(d.Zl.alloc((size_t)neq * std::max(3, opt_.lanczos_block)));
+      HIPCHK(d.Wl.alloc((size_t)neq * std::max(3, opt_.lanczos_block)));
+      run_forward(st, d, *p3, d.Tmp.p, d.Zl.p, n1, true, false);
+      launch_fill(st, d.Wl.p, (int64_t)neq * 3, 0.0);
+      run_backward(st, d, *p3, d.Zl.p, d.Wl.p);
+      std::vector<double> v((size_t)neq * 3);
+      HIPCHK(hipMemcpyAsync(v.data(), d.Wl.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      res_.vgi.assign((size_t)n1 * 3, 0.0);
+      for (int q = 0; q < n1; q++) {  // MEQN1 order: equation q+1
+        int r = y.eq_to_perm[sam_.meqn1[q] - 1];
+        for (int c = 0; c < 3; c++) res_.vgi[(size_t)c * n1 + q] = v[(size_t)r * 3 + c];
+      }
+    }
+  }
+  HIPCHK(hipEventRecord(d.ev[7], st));
+  // ---- Schur complement out of the root front
+  if (n2 > 0) {
+    HIPCHK(hipMemcpyAsync(S22.data(), d.pool.p + y.foff[schur], sizeof(double) * n2 * n2, hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  times_.elements = ev_ms(d.ev[0], d.ev[1]);
+  times_.assembly = ev_ms(d.ev[1], d.ev[2]);
+  times_.factor = ev_ms(d.ev[2], d.ev[3]);
+  times_.bsolve = ev_ms(d.ev[3], d.ev[4]);
+  times_.modes = ev_ms(d.ev[4], d.ev[5]);
+  times_.mass = ev_ms(d.ev[5], d.ev[6]);
+  times_.gravity = ev_ms(d.ev[6], d.ev[7]);
+  times_.total = ev_ms(d.ev[0], d.ev[7]);
+  // ---- reduced matrices in the reference layout (CMSTRS, GRAV): first ndof2
+  // rows in DOF order through dofPosIn2, then the ngen component modes, then
+  // zero padding up to ndim
+  const int nd = res_.ndim;
+  const std::vector<int>& pos2 = sam_.dofPosIn2;
+  res_.SK.assign((size_t)nd * nd, 0.0);
+  res_.SM.assign(opt_.no_mass ? 0 : (size_t)nd * nd, 0.0);
+  res_.G.assign(opt_.no_mass ? 0 : (size_t)nd * 3, 0.0);
+  for (int j = 0; j < n2; j++)
+    for (int i = 0; i < n2; i++) {
+      int a = pos2[i] - 1, b = pos2[j] - 1;
+      double v = a >= b ? S22[a + (size_t)b * n2] : S22[b + (size_t)a * n2];
+      res_.SK[i + (size_t)j * nd] = v;
+      if (!opt_.no_mass) res_.SM[i + (size_t)j * nd] = 0.5 * (M22[(size_t)a * n2 + b] + M22[(size_t)b * n2 + a]);
+    }
+  for (int a = 0; a < ngen; a++)
+    for (int b = 0; b < ngen; b++) {
+      res_.SK[(n2 + a) + (size_t)(n2 + b) * nd] = 0.5 * (K11[(size_t)a * ngen + b] + K11[(size_t)b * ngen + a]);
+      if (!opt_.no_mass) res_.SM[(n2 + a) + (size_t)(n2 + b) * nd] = 0.5 * (M11[(size_t)a * ngen + b] + M11[(size_t)b * ngen + a]);
+    }
+  if (!opt_.no_mass) {
+    for (int a = 0; a < ngen; a++)
+      for (int i = 0; i < n2; i++) {
+        double v = M12[(size_t)a * n2 + (pos2[i] - 1)];
+        res_.SM[(n2 + a) + (size_t)i * nd] = v;
+        res_.SM[i + (size_t)(n2 + a) * nd] = v;
+      }
+    for (int c = 0; c < 3; c++) {
+      for (int i = 0; i < n2; i++) res_.G[i + (size_t)c * nd] = Ge[(size_t)(pos2[i] - 1) * 3 + c];
+      for (int a = 0; a < ngen; a++) res_.G[(n2 + a) + (size_t)c * nd] = Gg[(size_t)a * 3 + c];
+    }
+  }
+  return 0;
+}
+
+int DeviceReducer::get_B(std::vector<double>& B) {
+  const Symbolic& y = sym_;
+  const int n1 = y.ndof1, n2 = y.ndof2;
+  std::vector<double> x((size_t)y.neq * n2);
+  if (n2 == 0) { B.clear(); return 0; }
+  if (hipMemcpy(x.data(), d_->X.p, sizeof(double) * x.size(), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  B.assign((size_t)n1 * n2, 0.0);
+  for (int q = 0; q < n1; q++) {
+    int r = y.eq_to_perm[sam_.meqn1[q] - 1];
+    for (int c = 0; c < n2; c++) B[q + (size_t)c * n1] = x[(size_t)r * n2 + c];
+  }
+  return 0;
+}
+
+int DeviceReducer::get_phi(std::vector<double>& P) {
+  const Symbolic& y = sym_;
+  const int n1 = y.ndof1, nev = res_.nev;
+  P.assign((size_t)n1 * nev, 0.0);
+  if (nev == 0) return 0;
+  std::vector<double> x((size_t)y.neq * nev);
+  if (hipMemcpy(x.data(), d_->Phi.p, sizeof(double) * x.size(), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  for (int q = 0; q < n1; q++) {
+    int r = y.eq_to_perm[sam_.meqn1[q] - 1];
+    for (int c = 0; c < nev; c++) P[q + (size_t)c * n1] = x[(size_t)r * nev + c];
+  }
+  return 0;
+}
+
+int DeviceReducer::get_schur(std::vector<double>& S) {
+  const Symbolic& y = sym_;
+  const int n2 = y.ndof2;
+  std::vector<double> f((size_t)n2 * n2);
+  S.assign((size_t)n2 * n2, 0.0);
+  if (n2 == 0) return 0;
+  if (hipMemcpy(f.data(), d_->pool.p + y.foff[y.nsup - 1], sizeof(double) * f.size(), hipMemcpyDeviceToHost) != hipSuccess)
+    return -3;
+  for (int j = 0; j < n2; j++)
+    for (int i = 0; i < n2; i++) S[i + (size_t)j * n2] = i >= j ? f[i + (size_t)j * n2] : f[j + (size_t)i * n2];
+  return 0;
+}
+
+int DeviceReducer::get_csr(std::vector<int>& rowptr, std::vector<int>& colind, std::vector<double>& K,
+                           std::vector<double>& M) {
+  const Symbolic& y = sym_;
+  rowptr = y.rowptr;
+  colind = y.colind;
+  size_t nnz = y.rowptr[y.neq];
+  K.resize(nnz);
+  M.resize(nnz);
+  if (hipMemcpy(K.data(), d_->Kv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (hipMemcpy(M.data(), d_->Mv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  return 0;
+}
+
+int DeviceReducer::get_element_matrices(std::vector<double>& Kp, std::vector<double>& Mp) {
+  Kp.resize(sym_.elem_pool);
+  Mp.resize(sym_.elem_pool);
+  if (hipMemcpy(Kp.data(), d_->Kp.p, sizeof(double) * Kp.size(), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (hipMemcpy(Mp.data(), d_->Mp.p, sizeof(double) * Mp.size(), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  return 0;
+}
+
+int DeviceReducer::solve_kii(int nrhs, const double* R_eq, double* X_eq, std::string& err) {
+  Impl& d = *d_;
+  const Symbolic& y = sym_;
+  const int neq = y.neq, n1 = y.ndof1;
+  SolvePlan* p = nullptr;
+  if (get_plan(y, d, nrhs, p, err)) return -3;
+  HIPCHK(d.W.alloc((size_t)d.wrows * std::max<int64_t>(nrhs, std::max(y.ndof2, opt_.lanczos_block))));
+  DBuf<double> r, t, x;
+  std::vector<double> h((size_t)neq * nrhs, 0.0);
+  for (int q = 0; q < n1; q++) {
+    int rr = y.eq_to_perm[sam_.meqn1[q] - 1];
+    for (int c = 0; c < nrhs; c++) h[(size_t)rr * nrhs + c] = R_eq[q + (size_t)c * n1];
+  }
+  HIPCHK(r.upload(h));
+  HIPCHK(t.alloc(h.size()));
+  HIPCHK(x.alloc(h.size()));
+  run_forward(st_, d, *p, r.p, t.p, n1, true, false);
+  launch_fill(st_, x.p, (int64_t)h.size(), 0.0);
+  run_backward(st_, d, *p, t.p, x.p);
+  HIPCHK(hipMemcpyAsync(h.data(), x.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  for (int q = 0; q < n1; q++) {
+    int rr = y.eq_to_perm[sam_.meqn1[q] - 1];
+    for (int c = 0; c < nrhs; c++) X_eq[q + (size_t)c * n1] = h[(size_t)rr * nrhs + c];
+  }
+  return 0;
+}
+
+}  // namespace fgpu

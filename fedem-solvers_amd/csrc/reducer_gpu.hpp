@@ -1,0 +1,85 @@
+// Device-side part reduction (Guyan + fixed-interface component modes).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <map>
+#include <memory>
+#include "fgpu.hpp"
+#include "kern_common.hpp"
+
+namespace fgpu {
+
+struct ReduceOptions {
+  int ngen = 0;            // component modes kept in the reduced matrices
+  int neval = 0;           // eigenpairs computed (max(neval, ngen) when ngen > 0)
+  double eig_shift = 0.0;  // eigenshift
+  double tol_eig = 1e-12;  // relative residual for Lanczos convergence
+  int lanczos_block = 8;
+  int lanczos_maxvec = 0;  // 0: min(ndof1, 6*nev + 24)
+  bool want_vgi = false;   // K_ii^{-1} g_i (dispfile)
+  bool no_mass = false;
+  uint64_t seed = 0x5eed;  // Lanczos start block
+  SymbolicOptions sym;
+};
+
+struct PhaseTimes {  // milliseconds, measured with HIP events on the work stream
+  double elements = 0, assembly = 0, factor = 0, bsolve = 0, modes = 0, mass = 0, gravity = 0, total = 0;
+  double factor_kernel_ms = 0;  // sum of update-kernel (MFMA) time inside factor
+};
+
+struct Results {
+  int ndim = 0, ndof1 = 0, ndof2 = 0, ngen = 0, nev = 0;
+  std::vector<double> SK, SM, G;   // ndim x ndim, ndim x ndim, ndim x 3 (column-major)
+  std::vector<double> eval;        // nev angular frequencies sqrt(lambda) (EIGVAL)
+  std::vector<double> lambda;      // nev eigenvalues
+  std::vector<double> vgi;         // ndof1 x 3 (meqn1 order)
+  int lanczos_vectors = 0;
+  double mass_fe = 0;              // sum of element masses
+};
+
+class DeviceReducer {
+ public:
+  DeviceReducer();
+  ~DeviceReducer();
+  // Host analysis (SAM, symbolic, schedules) and upload of the part and all
+  // structure arrays.  Called once per part topology.
+  int analyze(const Part& p, const ReduceOptions& o, std::string& err);
+  // Device upload of the part and structure (first reduce() does it too).
+  int upload(std::string& err);
+  // One full numeric reduction on the device from the resident part data.
+  int reduce(std::string& err);
+  // Copies of device results in the reference layout (on demand).
+  int get_B(std::vector<double>& B);     // ndof1 x ndof2, MEQN1 rows, MEQN2 columns
+  int get_phi(std::vector<double>& P);   // ndof1 x nev, MEQN1 rows
+  int get_schur(std::vector<double>& S); // ndof2 x ndof2 (MEQN2 order)
+  int get_csr(std::vector<int>& rowptr, std::vector<int>& colind, std::vector<double>& K,
+              std::vector<double>& M);     // perm-space CSR
+  int get_element_matrices(std::vector<double>& Kp, std::vector<double>& Mp);
+  // Extra solves on the resident factor (load cases / recovery)
+  int solve_kii(int nrhs, const double* R_eq, double* X_eq, std::string& err);  // MEQN1 rows
+
+  const Sam& sam() const { return sam_; }
+  const Symbolic& sym() const { return sym_; }
+  const Part& part() const { return part_; }
+  const Results& results() const { return res_; }
+  const PhaseTimes& times() const { return times_; }
+  hipStream_t stream() const { return st_; }
+  double update_kernel_flops() const { return upd_flops_; }
+  int64_t update_launches() const { return upd_launches_; }
+
+  struct Impl;
+
+ private:
+  Part part_;
+  Sam sam_;
+  Symbolic sym_;
+  ReduceOptions opt_;
+  Results res_;
+  PhaseTimes times_;
+  hipStream_t st_ = nullptr;
+  bool uploaded_ = false;
+  double upd_flops_ = 0;
+  int64_t upd_launches_ = 0;
+  std::unique_ptr<Impl> d_;
+};
+
+}  // namespace fgpu

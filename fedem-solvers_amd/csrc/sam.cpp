@@ -1,0 +1,86 @@
+// SAM DOF numbering for a part: MADOF, MSC, MEQN, MEQN1, MEQN2, dofPosIn2.
+//
+// Reference: samReducerModule.f90:227-569 (initiateSAM, nodal arrays from
+// FFl), asmExtensionModule.f90:365-377 (SPRPRP turns unconnected internal
+// DOFs into fixed ones), asmExtensionModule.f90:910-980 (csCreateEqPartition)
+// and samModule.f90:892-991 (invertEqPartition).
+//
+// MEQN itself is produced by SAM's SPRPRP, whose source is not part of the
+// reference tree.  The numbering used here follows the one property the
+// reference code relies on (csCreateEqPartition expects the status-2
+// equations after all status-1 ones): status-1 DOFs are numbered 1..ndof1 in
+// DOF order, status-2 DOFs ndof1+1..neq in DOF order, fixed DOFs get 0.
+#include <algorithm>
+#include "fgpu.hpp"
+
+namespace fgpu {
+
+int build_sam(const Part& p, Sam& s, std::string& err) {
+  s = Sam();
+  s.nnod = p.nnod;
+  // DOFs per node: 6 for external nodes (superelement triads carry 6 DOFs) and
+  // for nodes of beams/shells, 3 for nodes of solids only.
+  std::vector<int> maxdn(p.nnod, 0);
+  for (int e = 0; e < p.nel; e++) {
+    int d = elem_ndn(p.etype[e]);
+    for (int k = p.eptr[e]; k < p.eptr[e + 1]; k++) {
+      int n = p.enodes[k];
+      if (n < 0 || n >= p.nnod) { err = "element node index out of range"; return -1; }
+      maxdn[n] = std::max(maxdn[n], d);
+    }
+  }
+  s.nndof.resize(p.nnod);
+  s.madof.resize(p.nnod + 1);
+  s.madof[0] = 1;
+  for (int i = 0; i < p.nnod; i++) {
+    int nd = (p.status[i] == 1 || maxdn[i] == 0) ? 6 : maxdn[i];
+    s.nndof[i] = nd;
+    s.madof[i + 1] = s.madof[i] + nd;
+  }
+  s.ndof = s.madof[p.nnod] - 1;
+  s.msc.assign(s.ndof, 0);
+  for (int i = 0; i < p.nnod; i++) {
+    int st = p.status[i];
+    if (st == 1) { s.nenod++; s.ndim_ext += s.nndof[i]; }
+    for (int j = 0; j < s.nndof[i]; j++) {
+      int idof = s.madof[i] - 1 + j;
+      int m;
+      if (st < 0 && ((-st) >> j) & 1) m = 0;
+      else if (st == 1) m = 2;
+      else m = 1;
+      // unconnected internal DOF -> fixed (SPRPRP behaviour)
+      if (m == 1 && j >= maxdn[i]) m = 0;
+      s.msc[idof] = m;
+    }
+  }
+  s.meqn.assign(s.ndof, 0);
+  int ieq = 0;
+  for (int d = 0; d < s.ndof; d++)
+    if (s.msc[d] == 1) s.meqn[d] = ++ieq;
+  s.ndof1 = ieq;
+  for (int d = 0; d < s.ndof; d++)
+    if (s.msc[d] == 2) s.meqn[d] = ++ieq;
+  s.neq = ieq;
+  s.ndof2 = s.neq - s.ndof1;
+  // csCreateEqPartition: MEQN1/MEQN2 list the status-1/2 equations ascending
+  std::vector<int> eqst(s.neq + 1, 0);
+  for (int d = 0; d < s.ndof; d++)
+    if (s.msc[d] > 0) eqst[s.meqn[d]] = s.msc[d];
+  for (int q = 1; q <= s.neq; q++) {
+    if (eqst[q] == 1) s.meqn1.push_back(q);
+    else s.meqn2.push_back(q);
+  }
+  // invertEqPartition: dofPosIn2(i2) = findloc(MEQN2, MEQN(idof)) over the
+  // status-2 DOFs in DOF order
+  std::vector<int> pos2(s.neq + 1, 0);
+  for (int k = 0; k < s.ndof2; k++) pos2[s.meqn2[k]] = k + 1;
+  for (int d = 0; d < s.ndof; d++)
+    if (s.msc[d] == 2) s.dofPosIn2.push_back(pos2[s.meqn[d]]);
+  if ((int)s.meqn1.size() != s.ndof1 || (int)s.meqn2.size() != s.ndof2) {
+    err = "equation partition mismatch";
+    return -1;
+  }
+  return 0;
+}
+
+}  // namespace fgpu

@@ -1,0 +1,212 @@
+"""ctypes bindings of the GPU plug-in seam of libfedem_reducer_core.so.
+
+The product path has no CPU fallback: if the HIP library is missing, loading
+fails with an error naming the build step.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import LIB_PATH
+
+_lib = None
+
+NPROP = 16
+HEX8, ANDES4, BEAM2 = 44, 24, 11
+
+
+def load_library(path=None):
+    """Load (once) and return the reducer library."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built; run __graft_entry__.build() (make -C fedem-solvers_amd)")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    lib.fgpu_create.argtypes = [ctypes.POINTER(P)]
+    lib.fgpu_destroy.argtypes = [P]
+    lib.fgpu_destroy.restype = None
+    lib.fgpu_last_error.argtypes = [P]
+    lib.fgpu_last_error.restype = ctypes.c_char_p
+    lib.fgpu_set_part.argtypes = [P, I, P, P, I, P, P, P, P]
+    lib.fgpu_analyze.argtypes = [P, I, I, I]
+    for name in ("fgpu_get_sizes",):
+        getattr(lib, name).argtypes = [P, P]
+    lib.fgpu_get_sam.argtypes = [P] + [P] * 6
+    lib.fgpu_reduce.argtypes = [P]
+    lib.fgpu_get_reduced.argtypes = [P, P, P, P]
+    lib.fgpu_get_eigen.argtypes = [P, P, P, P]
+    lib.fgpu_get_B.argtypes = [P, P]
+    lib.fgpu_get_phi.argtypes = [P, P]
+    lib.fgpu_get_vgi.argtypes = [P, P]
+    lib.fgpu_solve_internal.argtypes = [P, I, P, P]
+    lib.fgpu_get_times.argtypes = [P, P]
+    lib.fgpu_get_info.argtypes = [P, P]
+    lib.fgpu_get_csr.argtypes = [P, P, P, P, P, P]
+    lib.fgpu_get_element_matrices.argtypes = [P, P, P, P]
+    lib.fgpu_get_supernodes.argtypes = [P, P, P, P, P, P]
+    lib.fgpu_get_perm.argtypes = [P, P]
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class ReducerError(RuntimeError):
+    pass
+
+
+class GpuReducer:
+    """One reducer handle: a part, its symbolic analysis and device buffers."""
+
+    def __init__(self, lib=None):
+        self.lib = lib or load_library()
+        self.h = ctypes.c_void_p()
+        if self.lib.fgpu_create(ctypes.byref(self.h)) != 0:
+            raise ReducerError("fgpu_create failed")
+
+    def close(self):
+        if self.h:
+            self.lib.fgpu_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.fgpu_last_error(self.h).decode()
+            raise ReducerError(f"{what} returned {rc}: {msg}")
+
+    def set_part(self, part):
+        xyz = np.ascontiguousarray(part["xyz"], dtype=np.float64)
+        status = np.ascontiguousarray(part["status"], dtype=np.int32)
+        melcon = np.ascontiguousarray(part["melcon"], dtype=np.int32)
+        mpmnpc = np.ascontiguousarray(part["mpmnpc"], dtype=np.int32)
+        mmnpc = np.ascontiguousarray(part["mmnpc"], dtype=np.int32)
+        eprop = np.ascontiguousarray(part["eprop"], dtype=np.float64)
+        self._keep = (xyz, status, melcon, mpmnpc, mmnpc, eprop)
+        self._check(self.lib.fgpu_set_part(self.h, len(status), _ptr(xyz), _ptr(status), len(melcon),
+                                           _ptr(melcon), _ptr(mpmnpc), _ptr(mmnpc), _ptr(eprop)),
+                    "fgpu_set_part")
+
+    def analyze(self, ngen=0, neval=0, vgi=False, nomass=False):
+        flags = (1 if vgi else 0) | (2 if nomass else 0)
+        self._check(self.lib.fgpu_analyze(self.h, ngen, neval, flags), "fgpu_analyze")
+        s = np.zeros(8, np.int32)
+        self._check(self.lib.fgpu_get_sizes(self.h, _ptr(s)), "fgpu_get_sizes")
+        keys = ("nnod", "ndof", "neq", "ndof1", "ndof2", "ndim", "nenod", "nel")
+        self.sizes = dict(zip(keys, (int(v) for v in s)))
+        return self.sizes
+
+    def sam(self):
+        z = self.sizes
+        madof = np.zeros(z["nnod"] + 1, np.int32)
+        msc = np.zeros(z["ndof"], np.int32)
+        meqn = np.zeros(z["ndof"], np.int32)
+        meqn1 = np.zeros(z["ndof1"], np.int32)
+        meqn2 = np.zeros(z["ndof2"], np.int32)
+        pos2 = np.zeros(z["ndof2"], np.int32)
+        self._check(self.lib.fgpu_get_sam(self.h, _ptr(madof), _ptr(msc), _ptr(meqn), _ptr(meqn1),
+                                          _ptr(meqn2), _ptr(pos2)), "fgpu_get_sam")
+        return dict(madof=madof, msc=msc, meqn=meqn, meqn1=meqn1, meqn2=meqn2, dofPosIn2=pos2)
+
+    def supernodes(self):
+        inf = self.info()
+        n = inf["nsup"]
+        col0, ncol, nrow, parent = (np.zeros(n, np.int32) for _ in range(4))
+        self._check(self.lib.fgpu_get_supernodes(self.h, _ptr(col0), _ptr(ncol), _ptr(nrow), _ptr(parent), None),
+                    "fgpu_get_supernodes")
+        rows = np.zeros(int(nrow.sum()), np.int32)
+        self._check(self.lib.fgpu_get_supernodes(self.h, None, None, None, None, _ptr(rows)), "fgpu_get_supernodes")
+        perm = np.zeros(self.sizes["neq"], np.int32)
+        self._check(self.lib.fgpu_get_perm(self.h, _ptr(perm)), "fgpu_get_perm")
+        return dict(col0=col0, ncol=ncol, nrow=nrow, parent=parent, rows=rows, perm_to_eq=perm)
+
+    def reduce(self):
+        self._check(self.lib.fgpu_reduce(self.h), "fgpu_reduce")
+
+    def reduced(self):
+        nd = self.sizes["ndim"]
+        SK = np.zeros((nd, nd), order="F")
+        SM = np.zeros((nd, nd), order="F")
+        G = np.zeros((nd, 3), order="F")
+        self._check(self.lib.fgpu_get_reduced(self.h, _ptr(SK), _ptr(SM), _ptr(G)), "fgpu_get_reduced")
+        return SK, SM, G
+
+    def eigen(self):
+        n = ctypes.c_int(0)
+        self._check(self.lib.fgpu_get_eigen(self.h, ctypes.byref(n), None, None), "fgpu_get_eigen")
+        lam = np.zeros(n.value)
+        evl = np.zeros(n.value)
+        self._check(self.lib.fgpu_get_eigen(self.h, ctypes.byref(n), _ptr(lam), _ptr(evl)), "fgpu_get_eigen")
+        return lam, evl
+
+    def B(self):
+        z = self.sizes
+        b = np.zeros((z["ndof1"], z["ndof2"]), order="F")
+        self._check(self.lib.fgpu_get_B(self.h, _ptr(b)), "fgpu_get_B")
+        return b
+
+    def phi(self):
+        lam, _ = self.eigen()
+        p = np.zeros((self.sizes["ndof1"], len(lam)), order="F")
+        if len(lam):
+            self._check(self.lib.fgpu_get_phi(self.h, _ptr(p)), "fgpu_get_phi")
+        return p
+
+    def vgi(self):
+        v = np.zeros((self.sizes["ndof1"], 3), order="F")
+        self._check(self.lib.fgpu_get_vgi(self.h, _ptr(v)), "fgpu_get_vgi")
+        return v
+
+    def solve_internal(self, R):
+        R = np.asfortranarray(R, dtype=np.float64)
+        if R.ndim == 1:
+            R = R[:, None]
+        X = np.zeros_like(R, order="F")
+        self._check(self.lib.fgpu_solve_internal(self.h, R.shape[1], _ptr(R), _ptr(X)), "fgpu_solve_internal")
+        return X
+
+    def times(self):
+        t = np.zeros(8)
+        self._check(self.lib.fgpu_get_times(self.h, _ptr(t)), "fgpu_get_times")
+        keys = ("elements", "assembly", "factor", "bsolve", "modes", "mass", "gravity", "total")
+        return dict(zip(keys, t.tolist()))
+
+    def info(self):
+        v = np.zeros(10, np.int64)
+        self._check(self.lib.fgpu_get_info(self.h, _ptr(v)), "fgpu_get_info")
+        keys = ("nsup", "nlevels", "front_pool", "nnzL", "factor_flops", "max_front", "nnz",
+                "lanczos_vectors", "elem_pool", "update_flops")
+        return dict(zip(keys, (int(x) for x in v)))
+
+    def csr(self):
+        inf = self.info()
+        neq = self.sizes["neq"]
+        rp = np.zeros(neq + 1, np.int32)
+        ci = np.zeros(inf["nnz"], np.int32)
+        K = np.zeros(inf["nnz"])
+        M = np.zeros(inf["nnz"])
+        pe = np.zeros(neq, np.int32)
+        self._check(self.lib.fgpu_get_csr(self.h, _ptr(rp), _ptr(ci), _ptr(K), _ptr(M), _ptr(pe)), "fgpu_get_csr")
+        return rp, ci, K, M, pe
+
+    def element_matrices(self):
+        inf = self.info()
+        Kp = np.zeros(inf["elem_pool"])
+        Mp = np.zeros(inf["elem_pool"])
+        off = np.zeros(self.sizes["nel"] + 1, np.int64)
+        self._check(self.lib.fgpu_get_element_matrices(self.h, _ptr(Kp), _ptr(Mp), _ptr(off)),
+                    "fgpu_get_element_matrices")
+        return Kp, Mp, off

@@ -1,0 +1,129 @@
+/*
+ * C ABI of the MI355X FE part reducer (libfedem_reducer_core.so).
+ *
+ * Two layers are exported:
+ *
+ * 1. The drop-in entry points of the reference's reducer library, with the
+ *    same names, arguments, meaning and return conventions
+ *    (src/vpmReducer/reducerInterface.C:52-222), so that an unmodified
+ *    fedempy.reducer.FedemReducer (PythonAPI/src/fedempy/reducer.py:39-119)
+ *    can load this library.
+ *
+ * 2. The storage-type plug-in seam the Fortran host drives (the reference
+ *    dispatches its equation-solver work on SysMatrixType%storageType,
+ *    sysMatrixTypeModule.f90:122-127; this library is the GPU storage type).
+ *    Plain pointers and sizes only; 0 on success, negative on error (the
+ *    reference's ierr convention), message via fgpu_last_error().
+ *
+ * Index conventions follow SAM: node numbers and equation numbers are
+ * 1-based, matrices are column-major (Fortran order).
+ */
+#ifndef FEDEM_REDUCER_GPU_H
+#define FEDEM_REDUCER_GPU_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 1. drop-in reducer entry points ---------------------------------- */
+
+/* reducerInterface.C:52 initSolverArgs: registers (first=true) or updates the
+ * command-line options.  Returns 0. */
+int initSolverArgs(int argc, char** argv, bool first, bool muted);
+/* reducerInterface.C:207 reducePart: reduces the part named by -linkfile and
+ * writes the _S/_M/_G/_B/_E/_V/_L .fmx files.  Returns 0 or ierr < 0. */
+int reducePart(bool first, bool last);
+/* reducerInterface.C:147 solvePart: direct static solve of the part. */
+int solvePart(void);
+/* reducerInterface.C:136 solvePartDis: as solvePart, returning the
+ * displacements of n nodes in displ (3*n doubles). */
+int solvePartDis(int n, const int* nodes, double* displ);
+/* reducerInterface.C:159 freeSingeltons */
+void freeSingeltons(void);
+
+/* ---- 2. plug-in seam -------------------------------------------------- */
+
+typedef struct fgpu_reducer fgpu_reducer;
+
+/* element type codes are the reference's MELCON values */
+enum { FGPU_BEAM2 = 11, FGPU_ANDES4 = 24, FGPU_HEX8 = 44 };
+/* per-element property record, FGPU_NPROP doubles per element */
+enum {
+  FGPU_P_E = 0, FGPU_P_NU = 1, FGPU_P_RHO = 2, FGPU_P_G = 3,
+  FGPU_P_T1 = 4, /* shell thickness of nodes 1..4 at 4..7 */
+  FGPU_P_A = 4, FGPU_P_IY = 5, FGPU_P_IZ = 6, FGPU_P_IT = 7, /* beam section */
+  FGPU_P_DX = 8, FGPU_P_DY = 9, FGPU_P_DZ = 10,              /* beam local z */
+  FGPU_P_KY = 11, FGPU_P_KZ = 12, FGPU_P_NSM = 13,
+  FGPU_NPROP = 16
+};
+
+/* Creates / destroys a reducer handle (one GPU stream, device buffers). */
+int fgpu_create(fgpu_reducer** h);
+void fgpu_destroy(fgpu_reducer* h);
+const char* fgpu_last_error(const fgpu_reducer* h);
+
+/* The FE part: the data samReducerModule.f90:227-569 (initiateSAM) gets from
+ * FFl.  status per node: 1 external, 0 internal, negative = fixed-DOF bitmask
+ * -(sum of 2^i over fixed local DOFs i) (testModels.C:512-525).
+ * mpmnpc: nel+1 1-based pointers into mmnpc (1-based node numbers). */
+int fgpu_set_part(fgpu_reducer* h, int nnod, const double* xyz, const int* status, int nel,
+                  const int* melcon, const int* mpmnpc, const int* mmnpc, const double* eprop);
+
+/* Options of the reduction: ngen (component modes kept), neval (eigenpairs
+ * computed), flags bit0 = dispfile (vgi), bit1 = nomass.  Performs SAM
+ * numbering (csAllocPointerArrays/csPreassemble, asmExtensionModule.f90:227,
+ * 678) and the symbolic analysis, and uploads the part. */
+int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags);
+
+/* sizes[0..7] = nnod, ndof, neq, ndof1, ndof2, ndim, nenod, nel */
+int fgpu_get_sizes(const fgpu_reducer* h, int* sizes);
+/* SAM arrays (samModule.f90:27-64): madof(nnod+1), msc(ndof), meqn(ndof),
+ * meqn1(ndof1), meqn2(ndof2), dofPosIn2(ndof2); any pointer may be NULL. */
+int fgpu_get_sam(const fgpu_reducer* h, int* madof, int* msc, int* meqn, int* meqn1, int* meqn2,
+                 int* dofPosIn2);
+
+/* One numeric reduction on the GPU: element matrices, assembly, LDL^T with
+ * the Schur complement (csSolveSupEl, solExtensionModule.f90:558-1033), B,
+ * component modes (EIGVAL, cmstrs.f90:53), CMSTRS mass/stiffness reduction
+ * (cmstrs.f90:511) and GRAV (cmstrs.f90:369). */
+int fgpu_reduce(fgpu_reducer* h);
+
+/* Reduced matrices in the reference layout (ndim x ndim column-major;
+ * G is ndim x 3).  Any pointer may be NULL. */
+int fgpu_get_reduced(const fgpu_reducer* h, double* SK, double* SM, double* G);
+/* nev eigenvalues lambda and angular frequencies sqrt(lambda) (EIGVAL). */
+int fgpu_get_eigen(const fgpu_reducer* h, int* nev, double* lambda, double* evl);
+/* B (ndof1 x ndof2, MEQN1 rows, MEQN2 columns), the _B.fmx content. */
+int fgpu_get_B(fgpu_reducer* h, double* B);
+/* component modes (ndof1 x nev, MEQN1 rows), the _E.fmx content. */
+int fgpu_get_phi(fgpu_reducer* h, double* phi);
+/* vgi = K_ii^-1 g_i (ndof1 x 3), the _V.fmx content (flags bit0). */
+int fgpu_get_vgi(const fgpu_reducer* h, double* vgi);
+/* X = K_ii^-1 R on the resident factor, R and X ndof1 x nrhs (MEQN1 rows). */
+int fgpu_solve_internal(fgpu_reducer* h, int nrhs, const double* R, double* X);
+
+/* Diagnostics: times[0..7] (ms) = elements, assembly, factor, B, modes,
+ * mass, gravity, total; info[0..9] = nsup, nlevels, front pool doubles,
+ * nnz(L), factor flops, max front, nnz(K), Lanczos vectors, packed element
+ * pool doubles, trailing-update (MFMA) flops. */
+int fgpu_get_times(const fgpu_reducer* h, double* times);
+int fgpu_get_info(const fgpu_reducer* h, int64_t* info);
+/* Assembled K and M in the device's permuted CSR (rows/cols 0-based perm
+ * indices), perm_to_eq maps a perm index to a 0-based equation number. */
+int fgpu_get_csr(fgpu_reducer* h, int* rowptr, int* colind, double* K, double* M, int* perm_to_eq);
+/* Packed element matrices as computed on the device (node-pair blocks). */
+int fgpu_get_element_matrices(fgpu_reducer* h, double* Kp, double* Mp, int64_t* offsets);
+/* Symbolic structure for host-side verification: per supernode col0, ncol,
+ * nrow, parent; rows (nrow each, perm indices). */
+int fgpu_get_supernodes(const fgpu_reducer* h, int* col0, int* ncol, int* nrow, int* parent, int* rows);
+/* perm_to_eq[p] = 0-based equation number at device position p (neq). */
+int fgpu_get_perm(const fgpu_reducer* h, int* perm_to_eq);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,232 @@
+"""ORACLE — test infrastructure only (see oracle/__init__.py).
+
+CPU restatement of the reducer's numbering, assembly and CMS reduction:
+
+  sam_numbering     samReducerModule.f90:227-569 (nodal DOF counts, MADOF,
+                    MSC), asmExtensionModule.f90:365-377 (unconnected
+                    internal DOFs become fixed), :910-980 csCreateEqPartition,
+                    samModule.f90:892-991 invertEqPartition.  MEQN itself comes
+                    from SAM's SPRPRP, absent from the reference tree: parity
+                    unpinned, the rule used is status-1 equations first then
+                    status-2, each in DOF order (csCreateEqPartition's
+                    expectation, asmExtensionModule.f90:936-947).
+  assemble          lhsINADD element loop + csAddEM (inadd.f90:103-282)
+  reduce            CMSTRS (cmstrs.f90:511-1062), GRAVfull/GRAV
+                    (cmstrs.f90:286-474), EIGVAL (cmstrs.f90:53-263) with the
+                    dense LAPACK path of the reference (DSYGVX, used when
+                    tolEigval <= 0) or a shift-invert Lanczos (scipy eigsh)
+                    for large parts; the small dense reducer of the
+                    reference tests (smallReducer.f90:150-225) is the same
+                    algebra: B = -K_ii^-1 K_ie, K = K_ee + K_ie^T B,
+                    M = M_ee + M_ie^T B + B^T M_ie + B^T M_ii B.
+"""
+import ctypes
+import os
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so missing: make -C oracle")
+        _lib = ctypes.CDLL(path)
+        D = ctypes.POINTER(ctypes.c_double)
+        _lib.hex8_stiff.argtypes = [D, D, D, ctypes.c_double, ctypes.c_double, D]
+        _lib.hex8_mass.argtypes = [D, D, D, ctypes.c_double, D]
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def hex8_matrices(X, Y, Z, E, nu, rho):
+    """(K, M) 24x24 of one HEX8 element (Femlib HEXA31 NIP=3 and HEXA35)."""
+    X, Y, Z = (np.ascontiguousarray(v, np.float64) for v in (X, Y, Z))
+    K = np.zeros(576)
+    M = np.zeros(576)
+    rc = lib().hex8_stiff(_dp(X), _dp(Y), _dp(Z), E, nu, _dp(K))
+    if rc != 0:
+        raise ValueError(f"hex8_stiff error {rc}")
+    lib().hex8_mass(_dp(X), _dp(Y), _dp(Z), rho, _dp(M))
+    return K.reshape(24, 24, order="F"), M.reshape(24, 24, order="F")
+
+
+ELEM_NODES = {44: 8, 24: 4, 11: 2}
+ELEM_NDN = {44: 3, 24: 6, 11: 6}
+
+
+def sam_numbering(part):
+    status = np.asarray(part["status"])
+    melcon = np.asarray(part["melcon"])
+    mp = np.asarray(part["mpmnpc"]) - 1
+    mm = np.asarray(part["mmnpc"]) - 1
+    nnod = len(status)
+    maxdn = np.zeros(nnod, int)
+    for e in range(len(melcon)):
+        for n in mm[mp[e]:mp[e + 1]]:
+            maxdn[n] = max(maxdn[n], ELEM_NDN[int(melcon[e])])
+    nndof = np.where((status == 1) | (maxdn == 0), 6, maxdn)
+    madof = np.concatenate([[1], 1 + np.cumsum(nndof)])
+    ndof = int(madof[-1] - 1)
+    msc = np.zeros(ndof, int)
+    for i in range(nnod):
+        for j in range(nndof[i]):
+            st = int(status[i])
+            if st < 0 and ((-st) >> j) & 1:
+                m = 0
+            elif st == 1:
+                m = 2
+            else:
+                m = 1
+            if m == 1 and j >= maxdn[i]:
+                m = 0
+            msc[madof[i] - 1 + j] = m
+    meqn = np.zeros(ndof, int)
+    n1 = int(np.sum(msc == 1))
+    meqn[msc == 1] = np.arange(1, n1 + 1)
+    meqn[msc == 2] = np.arange(n1 + 1, n1 + 1 + int(np.sum(msc == 2)))
+    neq = int(np.count_nonzero(meqn))
+    status_of_eq = np.zeros(neq + 1, int)
+    status_of_eq[meqn[msc > 0]] = msc[msc > 0]
+    meqn1 = np.nonzero(status_of_eq == 1)[0]
+    meqn2 = np.nonzero(status_of_eq == 2)[0]
+    pos2 = np.zeros(neq + 1, int)
+    pos2[meqn2] = np.arange(1, len(meqn2) + 1)
+    dofpos2 = pos2[meqn[msc == 2]]
+    ndim_ext = int(np.sum(nndof[status == 1]))
+    return dict(madof=madof, msc=msc, meqn=meqn, meqn1=meqn1, meqn2=meqn2, dofPosIn2=dofpos2,
+                nndof=nndof, neq=neq, ndof1=len(meqn1), ndof2=len(meqn2), ndim_ext=ndim_ext)
+
+
+def element_matrices(part, e):
+    t = int(part["melcon"][e])
+    mp = part["mpmnpc"]
+    nodes = part["mmnpc"][mp[e] - 1:mp[e + 1] - 1] - 1
+    x = part["xyz"][nodes]
+    pr = part["eprop"][e]
+    if t == 44:
+        K, M = hex8_matrices(x[:, 0], x[:, 1], x[:, 2], pr[0], pr[1], pr[2])
+        return nodes, 3, K, M
+    raise NotImplementedError(f"element type {t}")
+
+
+def assemble(part, s):
+    """Global K, M (CSR, 0-based equation order) and the element masses."""
+    rows, cols, kv, mv = [], [], [], []
+    emass = 0.0
+    for e in range(len(part["melcon"])):
+        nodes, ndn, K, M = element_matrices(part, e)
+        eq = np.full(len(nodes) * ndn, -1)
+        for a, n in enumerate(nodes):
+            for j in range(ndn):
+                if j < s["nndof"][n]:
+                    eq[a * ndn + j] = s["meqn"][s["madof"][n] - 1 + j] - 1
+        ok = eq >= 0
+        ii, jj = np.meshgrid(eq[ok], eq[ok], indexing="ij")
+        rows.append(ii.ravel())
+        cols.append(jj.ravel())
+        kv.append(K[np.ix_(ok, ok)].ravel())
+        mv.append(M[np.ix_(ok, ok)].ravel())
+        emass += M[0::3, :].sum() if ndn == 3 else 0.0
+    n = s["neq"]
+    r, c = np.concatenate(rows), np.concatenate(cols)
+    Kg = sp.csr_matrix((np.concatenate(kv), (r, c)), shape=(n, n))
+    Mg = sp.csr_matrix((np.concatenate(mv), (r, c)), shape=(n, n))
+    return Kg, Mg, emass
+
+
+def gravity_dirs(part, s):
+    a = np.zeros((s["neq"], 3))
+    for i in range(len(part["status"])):
+        for j in range(min(3, s["nndof"][i])):
+            q = s["meqn"][s["madof"][i] - 1 + j]
+            if q > 0:
+                a[q - 1, j] = 1.0
+    return a
+
+
+def fix_signs(phi, eq_order=None):
+    """Largest-magnitude component of each mode positive (first in equation
+    order on exact ties)."""
+    phi = phi.copy()
+    for i in range(phi.shape[1]):
+        k = int(np.argmax(np.abs(phi[:, i])))
+        if phi[k, i] < 0:
+            phi[:, i] = -phi[:, i]
+    return phi
+
+
+def modes(Kii, Mii, nev, dense_limit=3000):
+    """Lowest nev eigenpairs of Kii phi = lambda Mii phi, M-orthonormal."""
+    n = Kii.shape[0]
+    if n <= dense_limit:
+        w, V = sla.eigh(Kii.toarray(), Mii.toarray(), subset_by_index=[0, nev - 1])
+    else:
+        w, V = spla.eigsh(Kii.tocsc(), k=nev, M=Mii.tocsc(), sigma=0.0, which="LM", tol=0)
+        o = np.argsort(w)
+        w, V = w[o], V[:, o]
+        nrm = np.sqrt(np.einsum("ij,ij->j", V, Mii @ V))
+        V = V / nrm
+    return w, fix_signs(V)
+
+
+def reduce(part, ngen=0, neval=0, want_vgi=False, dense_limit=3000):
+    """Reduced SK, SM, G (CMSTRS/GRAV layout), B, modes and vgi of a part."""
+    s = sam_numbering(part)
+    K, M, emass = assemble(part, s)
+    i1 = s["meqn1"] - 1
+    i2 = s["meqn2"] - 1
+    Kii, Kib, Kbb = K[i1][:, i1], K[i1][:, i2], K[i2][:, i2].toarray()
+    Mii, Mib, Mbb = M[i1][:, i1], M[i1][:, i2], M[i2][:, i2].toarray()
+    n1, n2 = len(i1), len(i2)
+    if n1 <= dense_limit:
+        lu = sla.cho_factor(Kii.toarray())
+        solve = lambda R: sla.cho_solve(lu, R)
+    else:
+        f = spla.splu(Kii.tocsc())
+        solve = lambda R: f.solve(np.asarray(R))
+    B = -solve(Kib.toarray()) if n1 else np.zeros((0, n2))
+    S22 = Kbb + Kib.T @ B
+    S22 = 0.5 * (S22 + S22.T)
+    nev = max(neval, ngen) if ngen > 0 else 0
+    lam = phi = None
+    if nev > 0:
+        lam, phi = modes(Kii, Mii, nev, dense_limit)
+    MB = Mii @ B
+    M22 = B.T @ MB + Mib.T @ B + B.T @ Mib + Mbb
+    M22 = 0.5 * (M22 + M22.T)
+    a = gravity_dirs(part, s)
+    g = M @ a
+    Ge = B.T @ g[i1] + g[i2]
+    ndim = s["ndim_ext"] + ngen
+    pos = s["dofPosIn2"] - 1
+    SK = np.zeros((ndim, ndim))
+    SM = np.zeros((ndim, ndim))
+    G = np.zeros((ndim, 3))
+    SK[:n2, :n2] = S22[np.ix_(pos, pos)]
+    SM[:n2, :n2] = M22[np.ix_(pos, pos)]
+    G[:n2] = Ge[pos]
+    if ngen > 0:
+        P = phi[:, :ngen]
+        K11 = P.T @ (Kii @ P)
+        M11 = P.T @ (Mii @ P)
+        M12 = P.T @ MB + (Mib.T @ P).T
+        SK[n2:n2 + ngen, n2:n2 + ngen] = 0.5 * (K11 + K11.T)
+        SM[n2:n2 + ngen, n2:n2 + ngen] = 0.5 * (M11 + M11.T)
+        SM[n2:n2 + ngen, :n2] = M12[:, pos]
+        SM[:n2, n2:n2 + ngen] = M12[:, pos].T
+        G[n2:n2 + ngen] = P.T @ g[i1]
+    out = dict(SK=SK, SM=SM, G=G, B=B, S22=S22, sam=s, K=K, M=M, emass=emass, lam=lam, phi=phi)
+    if want_vgi:
+        out["vgi"] = solve(g[i1])
+    return out
