@@ -10,6 +10,7 @@ struct fgpu_reducer {
   fgpu::Part part;
   bool have_part = false, analyzed = false, reduced = false;
   int ngen = 0;
+  fgpu::ReduceOptions opts;
   std::string err;
 };
 
@@ -81,7 +82,7 @@ int fgpu_set_part(fgpu_reducer* h, int nnod, const double* xyz, const int* statu
 int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags) {
   GUARD(h)
   if (!h->have_part) { h->err = "fgpu_analyze: no part"; return -1; }
-  fgpu::ReduceOptions o;
+  fgpu::ReduceOptions o = h->opts;
   o.ngen = ngen < 0 ? 0 : ngen;
   h->ngen = o.ngen;
   o.neval = neval < 0 ? 0 : neval;
@@ -92,6 +93,21 @@ int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags) {
   h->reduced = false;
   return rc;
   END_GUARD(h)
+}
+
+int fgpu_set_option(fgpu_reducer* h, const char* name, double value) {
+  if (!h || !name) return -1;
+  std::string n(name);
+  fgpu::ReduceOptions& o = h->opts;
+  if (n == "lanczosBlock") o.lanczos_block = (int)value;
+  else if (n == "lanczosMaxVectors") o.lanczos_maxvec = (int)value;
+  else if (n == "lanczosTol") o.tol_eig = value;
+  else if (n == "eigenshift") o.eig_shift = value;
+  else if (n == "seed") o.seed = (uint64_t)value;
+  else if (n == "leafNodes") o.sym.leaf_nodes = (int)value;
+  else if (n == "timeUpdates") o.time_updates = value != 0.0;
+  else { h->err = "fgpu_set_option: unknown option " + n; return -1; }
+  return 0;
 }
 
 int fgpu_get_sizes(const fgpu_reducer* h, int* s) {
@@ -176,7 +192,8 @@ int fgpu_solve_internal(fgpu_reducer* h, int nrhs, const double* R, double* X) {
 int fgpu_get_times(const fgpu_reducer* h, double* t) {
   if (!h || !h->reduced || !t) return -1;
   const fgpu::PhaseTimes& p = h->red.times();
-  double v[8] = {p.elements, p.assembly, p.factor, p.bsolve, p.modes, p.mass, p.gravity, p.total};
+  double v[10] = {p.elements, p.assembly, p.factor, p.bsolve, p.modes, p.mass, p.gravity, p.total,
+                  p.update_kernel_ms, (double)p.update_launches};
   std::memcpy(t, v, sizeof v);
   return 0;
 }
@@ -232,6 +249,13 @@ int fgpu_get_perm(const fgpu_reducer* h, int* perm_to_eq) {
   if (!h || !h->analyzed || !perm_to_eq) return -1;
   const std::vector<int>& pe = h->red.sym().perm_to_eq;
   std::memcpy(perm_to_eq, pe.data(), pe.size() * sizeof(int));
+  return 0;
+}
+
+int fgpu_get_lanczos(const fgpu_reducer* h, int* nvec, double* resid) {
+  if (!h || !h->reduced) return -1;
+  if (nvec) *nvec = h->red.results().lanczos_vectors;
+  if (resid) *resid = h->red.results().lanczos_resid;
   return 0;
 }
 

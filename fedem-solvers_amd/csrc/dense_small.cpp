@@ -35,10 +35,11 @@ void inv_upper(int n, const std::vector<double>& R, std::vector<double>& Ri) {
 
 void sym_eig(int n, const std::vector<double>& Ain, std::vector<double>& w, std::vector<double>& V) {
   std::vector<double> A(Ain);
-  V.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; i++) V[(size_t)i * n + i] = 1.0;
-  std::vector<double> d(n), e(n, 0.0), v(n), p(n), q(n);
-  // Householder reduction: for column k, annihilate A[k+2:, k]
+  // Zt = V^T is accumulated row-major, so both the Householder products and
+  // the QL rotations below update contiguous rows.
+  std::vector<double> Zt((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) Zt[(size_t)i * n + i] = 1.0;
+  std::vector<double> d(n), e(n, 0.0), v(n), p(n), q(n), zv(n);
   for (int k = 0; k + 2 < n; k++) {
     double nrm = 0.0;
     for (int i = k + 1; i < n; i++) nrm += A[(size_t)i * n + k] * A[(size_t)i * n + k];
@@ -54,23 +55,32 @@ void sym_eig(int n, const std::vector<double>& Ain, std::vector<double>& w, std:
     if (vn == 0.0) continue;
     vn = std::sqrt(vn);
     for (int i = k + 1; i < n; i++) v[i] /= vn;
-    // A <- H A H with H = I - 2 v v^T (acting on rows/cols k..n-1)
+    // A <- H A H with H = I - 2 v v^T (rows/cols k..n-1; A symmetric)
     for (int i = k; i < n; i++) {
+      const double* Ai = &A[(size_t)i * n];
       double s = 0.0;
-      for (int j = k + 1; j < n; j++) s += A[(size_t)i * n + j] * v[j];
+      for (int j = k + 1; j < n; j++) s += Ai[j] * v[j];
       p[i] = 2.0 * s;
     }
     double K = 0.0;
     for (int i = k + 1; i < n; i++) K += v[i] * p[i];
     for (int i = k; i < n; i++) q[i] = p[i] - K * v[i];
-    for (int i = k; i < n; i++)
-      for (int j = k; j < n; j++) A[(size_t)i * n + j] -= v[i] * q[j] + q[i] * v[j];
-    // accumulate V <- V H
-    for (int i = 0; i < n; i++) {
-      double s = 0.0;
-      for (int j = k + 1; j < n; j++) s += V[(size_t)i * n + j] * v[j];
-      s *= 2.0;
-      for (int j = k + 1; j < n; j++) V[(size_t)i * n + j] -= s * v[j];
+    for (int i = k; i < n; i++) {
+      double* Ai = &A[(size_t)i * n];
+      const double vi = v[i], qi = q[i];
+      for (int j = k; j < n; j++) Ai[j] -= vi * q[j] + qi * v[j];
+    }
+    // V <- V H  <=>  Zt <- H Zt: rows k+1.. of Zt minus 2 v (v^T Zt)
+    for (int j = 0; j < n; j++) zv[j] = 0.0;
+    for (int i = k + 1; i < n; i++) {
+      const double* Zi = &Zt[(size_t)i * n];
+      const double vi = v[i];
+      for (int j = 0; j < n; j++) zv[j] += vi * Zi[j];
+    }
+    for (int i = k + 1; i < n; i++) {
+      double* Zi = &Zt[(size_t)i * n];
+      const double vi2 = 2.0 * v[i];
+      for (int j = 0; j < n; j++) Zi[j] -= vi2 * zv[j];
     }
   }
   for (int i = 0; i < n; i++) d[i] = A[(size_t)i * n + i];
@@ -110,10 +120,12 @@ void sym_eig(int n, const std::vector<double>& Ain, std::vector<double>& w, std:
         pp = s * r;
         d[i + 1] = g + pp;
         g = c * r - b;
+        double* z0 = &Zt[(size_t)i * n];
+        double* z1 = &Zt[(size_t)(i + 1) * n];
         for (int k = 0; k < n; k++) {
-          double fz = V[(size_t)k * n + i + 1];
-          V[(size_t)k * n + i + 1] = s * V[(size_t)k * n + i] + c * fz;
-          V[(size_t)k * n + i] = c * V[(size_t)k * n + i] - s * fz;
+          const double fz = z1[k];
+          z1[k] = s * z0[k] + c * fz;
+          z0[k] = c * z0[k] - s * fz;
         }
       }
       if (early) continue;
@@ -122,17 +134,189 @@ void sym_eig(int n, const std::vector<double>& Ain, std::vector<double>& w, std:
       e[m] = 0.0;
     }
   }
-  // sort ascending
   std::vector<int> idx(n);
   for (int i = 0; i < n; i++) idx[i] = i;
   std::sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
   w.resize(n);
-  std::vector<double> Vs((size_t)n * n);
+  V.assign((size_t)n * n, 0.0);
   for (int j = 0; j < n; j++) {
     w[j] = d[idx[j]];
-    for (int i = 0; i < n; i++) Vs[(size_t)i * n + j] = V[(size_t)i * n + idx[j]];
+    const double* Zr = &Zt[(size_t)idx[j] * n];
+    for (int i = 0; i < n; i++) V[(size_t)i * n + j] = Zr[i];
   }
-  V.swap(Vs);
+}
+
+namespace {
+
+// Householder tridiagonalisation of A (row-major, destroyed); reflector k is
+// stored in R row k (entries k+1..n-1, unit 2-norm), beta[k] = 0 when skipped.
+void tridiag(int n, std::vector<double>& A, std::vector<double>& R, std::vector<char>& used, std::vector<double>& d,
+             std::vector<double>& e) {
+  R.assign((size_t)n * n, 0.0);
+  used.assign(n, 0);
+  std::vector<double> p(n), q(n);
+  for (int k = 0; k + 2 < n; k++) {
+    double nrm = 0.0;
+    for (int i = k + 1; i < n; i++) nrm += A[(size_t)i * n + k] * A[(size_t)i * n + k];
+    nrm = std::sqrt(nrm);
+    if (nrm == 0.0) continue;
+    double x0 = A[(size_t)(k + 1) * n + k];
+    double alpha = x0 > 0 ? -nrm : nrm;
+    double* v = &R[(size_t)k * n];
+    for (int i = k + 1; i < n; i++) v[i] = A[(size_t)i * n + k];
+    v[k + 1] -= alpha;
+    double vn = 0.0;
+    for (int i = k + 1; i < n; i++) vn += v[i] * v[i];
+    if (vn == 0.0) continue;
+    vn = std::sqrt(vn);
+    for (int i = k + 1; i < n; i++) v[i] /= vn;
+    used[k] = 1;
+    for (int i = k; i < n; i++) {
+      const double* Ai = &A[(size_t)i * n];
+      double s = 0.0;
+      for (int j = k + 1; j < n; j++) s += Ai[j] * v[j];
+      p[i] = 2.0 * s;
+    }
+    double K = 0.0;
+    for (int i = k + 1; i < n; i++) K += v[i] * p[i];
+    for (int i = k; i < n; i++) q[i] = p[i] - K * v[i];
+    for (int i = k; i < n; i++) {
+      double* Ai = &A[(size_t)i * n];
+      const double vi = v[i], qi = q[i];
+      for (int j = k; j < n; j++) Ai[j] -= vi * q[j] + qi * v[j];
+    }
+  }
+  d.resize(n);
+  e.assign(n, 0.0);
+  for (int i = 0; i < n; i++) d[i] = A[(size_t)i * n + i];
+  for (int i = 0; i + 1 < n; i++) e[i] = A[(size_t)(i + 1) * n + i];
+}
+
+// eigenvalues of the symmetric tridiagonal (d, e) by implicit QL (no vectors)
+void tql_values(int n, std::vector<double> d, std::vector<double> e, std::vector<double>& w) {
+  for (int l = 0; l < n; l++) {
+    int iter = 0;
+    while (true) {
+      int m = l;
+      for (; m < n - 1; m++) {
+        double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= 1e-300 + 2.2e-16 * dd) break;
+      }
+      if (m == l || ++iter > 60) break;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+      double s = 1.0, c = 1.0, pp = 0.0;
+      bool early = false;
+      for (int i = m - 1; i >= l; i--) {
+        double f = s * e[i], b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) { d[i + 1] -= pp; e[m] = 0.0; early = true; break; }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - pp;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        pp = s * r;
+        d[i + 1] = g + pp;
+        g = c * r - b;
+      }
+      if (early) continue;
+      d[l] -= pp;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  w = d;
+  std::sort(w.begin(), w.end());
+}
+
+// solve (T - sigma I) x = b for tridiagonal T with partial pivoting
+void trisolve(int n, const std::vector<double>& d, const std::vector<double>& e, double sigma, std::vector<double>& x) {
+  // rows: l (sub), dm (diag), u1 (super), u2 (second super after pivoting)
+  std::vector<double> dm(n), u1(n, 0.0), u2(n, 0.0), l(n, 0.0);
+  std::vector<char> swp(n, 0);
+  for (int i = 0; i < n; i++) dm[i] = d[i] - sigma;
+  for (int i = 0; i + 1 < n; i++) u1[i] = e[i];
+  std::vector<double> sub(n, 0.0);
+  for (int i = 0; i + 1 < n; i++) sub[i] = e[i];
+  const double tiny = 1e-300;
+  for (int i = 0; i + 1 < n; i++) {
+    // eliminate sub[i] (row i+1, col i)
+    if (std::fabs(sub[i]) > std::fabs(dm[i])) {
+      swp[i] = 1;
+      // swap rows i and i+1 (columns i, i+1, i+2)
+      double a0 = dm[i], a1 = u1[i], a2 = u2[i];
+      double b0 = sub[i], b1 = dm[i + 1], b2 = (i + 1 < n - 1) ? u1[i + 1] : 0.0;
+      dm[i] = b0; u1[i] = b1; u2[i] = b2;
+      double m = a0 / b0;
+      l[i] = m;
+      dm[i + 1] = a1 - m * b1;
+      if (i + 1 < n - 1) u1[i + 1] = a2 - m * b2;
+    } else {
+      double piv = std::fabs(dm[i]) > tiny ? dm[i] : tiny;
+      dm[i] = piv;
+      double m = sub[i] / piv;
+      l[i] = m;
+      dm[i + 1] -= m * u1[i];
+    }
+  }
+  if (std::fabs(dm[n - 1]) < tiny) dm[n - 1] = tiny;
+  for (int i = 0; i + 1 < n; i++) {
+    if (swp[i]) std::swap(x[i], x[i + 1]);
+    x[i + 1] -= l[i] * x[i];
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = x[i];
+    if (i + 1 < n) s -= u1[i] * x[i + 1];
+    if (i + 2 < n) s -= u2[i] * x[i + 2];
+    x[i] = s / dm[i];
+  }
+}
+
+}  // namespace
+
+void sym_eig_top(int n, const std::vector<double>& Ain, int k, std::vector<double>& w, std::vector<double>& V) {
+  std::vector<double> A(Ain), R, d, e;
+  std::vector<char> used;
+  tridiag(n, A, R, used, d, e);
+  tql_values(n, d, e, w);
+  k = std::min(k, n);
+  V.assign((size_t)n * k, 0.0);
+  double scale = 0.0;
+  for (int i = 0; i < n; i++) scale = std::max(scale, std::fabs(w[i]));
+  std::vector<std::vector<double>> Z;  // tridiagonal eigenvectors
+  for (int c = 0; c < k; c++) {
+    const double lam = w[n - 1 - c];
+    std::vector<double> x(n);
+    for (int i = 0; i < n; i++) x[i] = 1.0 + 0.01 * ((i * 7919) % 101);
+    const double sigma = lam + 1e-14 * std::max(scale, std::fabs(lam));
+    for (int it = 0; it < 4; it++) {
+      trisolve(n, d, e, sigma, x);
+      // keep members of a cluster orthogonal to the ones already computed
+      for (int q = 0; q < c; q++) {
+        if (std::fabs(w[n - 1 - q] - lam) > 1e-6 * std::max(scale, 1e-300)) continue;
+        double s = 0.0;
+        for (int i = 0; i < n; i++) s += Z[q][i] * x[i];
+        for (int i = 0; i < n; i++) x[i] -= s * Z[q][i];
+      }
+      double nr = 0.0;
+      for (int i = 0; i < n; i++) nr += x[i] * x[i];
+      nr = std::sqrt(nr);
+      for (int i = 0; i < n; i++) x[i] /= nr;
+    }
+    Z.push_back(x);
+    // back-transform: y = H_0 H_1 ... H_{n-3} x
+    for (int r = n - 3; r >= 0; r--) {
+      if (!used[r]) continue;
+      const double* v = &R[(size_t)r * n];
+      double s = 0.0;
+      for (int i = r + 1; i < n; i++) s += v[i] * x[i];
+      s *= 2.0;
+      for (int i = r + 1; i < n; i++) x[i] -= s * v[i];
+    }
+    for (int i = 0; i < n; i++) V[(size_t)i * k + c] = x[i];
+  }
 }
 
 }  // namespace fgpu

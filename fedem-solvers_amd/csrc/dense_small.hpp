@@ -14,5 +14,11 @@ void inv_upper(int n, const std::vector<double>& R, std::vector<double>& Ri);
 // Householder tridiagonalisation followed by implicit QL with Wilkinson-type
 // shifts (Golub & Van Loan, "Matrix Computations", sections 8.3 and 8.3.5).
 void sym_eig(int n, const std::vector<double>& A, std::vector<double>& w, std::vector<double>& V);
+// All eigenvalues w (ascending) and the eigenvectors of the k largest only,
+// V (n x k row-major, column c belongs to w[n-1-c]): Householder
+// tridiagonalisation with stored reflectors, implicit QL for the
+// eigenvalues, inverse iteration on the tridiagonal (clusters kept
+// orthogonal), back-transformation by the reflectors.
+void sym_eig_top(int n, const std::vector<double>& A, int k, std::vector<double>& w, std::vector<double>& V);
 
 }  // namespace fgpu

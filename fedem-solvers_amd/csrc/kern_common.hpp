@@ -19,8 +19,10 @@ struct FTask {
 struct SolveTask {
   int sn;
   int r0, rn;  // target rows of the front work buffer
-  int k0, kn;  // source rows (K dimension)
+  int k0, kn;  // source rows (K dimension); for reduce tasks: first slot, count
   int j0;      // first right-hand side of this task
+  int slot;    // -1: apply to the target directly; >= 0: write a 64x64 partial
+  int pad;
 };
 
 struct DevSym {

@@ -179,14 +179,14 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
 }
 
 // ---------------------------------------------------------------- extend-add
-// Task: sn = parent front, a = first parent column of a 64-column block.  The
+// Task: sn = parent front, a = first parent column of a 16-column block.  The
 // workgroup adds, child after child, every child update-matrix column whose
 // parent column falls in its block.  Blocks are disjoint, so no two
 // workgroups touch the same parent entry; children are added in list order.
 __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ tasks, DevSym S,
                                                     double* __restrict__ pool) {
   const FTask t = tasks[blockIdx.x];
-  const int sn = t.sn, j0 = t.a, j1 = t.a + 64;
+  const int sn = t.sn, j0 = t.a, j1 = t.a + 16;
   const int ldp = S.nrow[sn];
   double* P = pool + S.foff[sn];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -211,73 +211,159 @@ __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ ta
   }
 }
 
+// ---------------------------------------------------------------- MFMA tile
+// acc[x][y] += sum_m sA[wr+16x+(lane&15)][m] * sB[wc+16y+...][m] on
+// v_mfma_f64_16x16x4f64 for a 64x64 tile split in four 32x32 wave quarters.
+// The A operand of the instruction takes the sB rows and its B operand the
+// sA rows; the 16x16 result D[p][q] sits in lane q + 16*(p % 4), register
+// p / 4, so for fixed register consecutive lanes hold consecutive rows of sA
+// (coalesced stores into column-major fronts).  Result element of
+// acc[x][y][r]: row  wr + 16x + (lane & 15),  column  wc + 16y + (lane >> 4) + 4r.
+__device__ __forceinline__ void mfma_tile(const double (*sA)[65], const double (*sB)[65], int kbr,
+                                          f64x4 (&acc)[2][2], int lane, int wr, int wc) {
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int m0 = 0; m0 < kbr; m0 += 4) {
+    double ab[2], aa[2];
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      ab[x] = sB[wc + 16 * x + lr][m0 + lk];
+      aa[x] = sA[wr + 16 * x + lr][m0 + lk];
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[y], aa[x], acc[x][y], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void zero_acc(f64x4 (&acc)[2][2]) {
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
+}
+
 // ---------------------------------------------------------------- panel
-// Task: sn, a = row chunk, k0 = first panel column.  Every workgroup factors
-// the (kb x kb, kb <= 64) diagonal block in LDS (redundantly, it is cheap),
-// chunk 0 writes it back, and each workgroup then solves its 256 rows of the
-// panel: L_r = F_r L_kk^{-T} D^{-1}.
-__global__ __launch_bounds__(256) void k_panel(const FTask* __restrict__ tasks, DevSym S,
-                                               double* __restrict__ pool, int* __restrict__ piv) {
-  __shared__ double sA[64][65];
+// k_diag task: sn, k0 = first panel column.  One wave: lane i keeps row i
+// of the kb x kb (kb <= 64) diagonal block in registers and the block is
+// factored right-looking (LDL^T), one column exchanged through LDS per step.
+// Then lane j builds column j of the inverse of the unit lower factor.  L and
+// D go back into the front, the inverse into the diagonal tile of the
+// pivot-block inverse pool.
+__global__ __launch_bounds__(64) void k_diag(const FTask* __restrict__ tasks, DevSym S,
+                                             double* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                             double* __restrict__ ipool, int* __restrict__ piv) {
+  __shared__ double sL[64][65];
+  __shared__ double scol[64];
   const FTask t = tasks[blockIdx.x];
-  const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn];
-  const int kb = min(64, S.ncol[sn] - k0);
+  const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
+  const int kb = min(64, nc - k0);
   double* F = pool + S.foff[sn];
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < kb * kb; idx += 256) {
-    int i = idx % kb, j = idx / kb;
-    sA[i][j] = (i >= j) ? F[(int64_t)(k0 + i) + (int64_t)(k0 + j) * ld] : 0.0;
-  }
-  __syncthreads();
-  for (int j = 0; j < kb; j++) {
-    const double d = sA[j][j];
-    for (int i = j + 1 + tid; i < kb; i += 256) sA[i][j] = sA[i][j] / d;
-    __syncthreads();
-    const int m = kb - j - 1;
-    for (int idx = tid; idx < m * m; idx += 256) {
-      int i = j + 1 + idx / m, q = j + 1 + idx % m;
-      if (q <= i) sA[i][q] -= sA[i][j] * d * sA[q][j];
-    }
-    __syncthreads();
-  }
-  if (t.a == 0) {
-    for (int idx = tid; idx < kb * kb; idx += 256) {
-      int i = idx % kb, j = idx / kb;
-      if (i >= j) F[(int64_t)(k0 + i) + (int64_t)(k0 + j) * ld] = sA[i][j];
-    }
-    if (tid < kb) {
-      double d = sA[tid][tid];
-      if (!(d > 0.0)) {  // K_ii must be positive definite
-        atomicAdd(&piv[0], 1);
-        atomicMin(&piv[1], S.col0[sn] + k0 + tid);
-      }
-    }
-  }
-  const int r = k0 + kb + t.a * 256 + tid;
-  if (r >= ld) return;
-  double w[64];
+  double* Xg = ipool + ioff[sn];
+  const int lane = threadIdx.x;
+  const bool act = lane < kb;
+  double a[64];
+  double myd = 1.0;
+#pragma unroll
+  for (int m = 0; m < 64; m++) a[m] = (act && m <= lane && m < kb) ? F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] : 0.0;
 #pragma unroll
   for (int j = 0; j < 64; j++) {
     if (j < kb) {
-      double f = F[(int64_t)r + (int64_t)(k0 + j) * ld];
+      const double dj = __shfl(a[j], j);
+      if (lane == j) myd = dj;
+      const double l = (lane > j) ? a[j] / dj : 0.0;
+      if (lane > j) a[j] = l;
+      scol[lane] = l;
+      __syncthreads();
 #pragma unroll
-      for (int m = 0; m < 64; m++)
-        if (m < j) f -= w[m] * sA[j][m];
-      w[j] = f;
+      for (int m = j + 1; m < 64; m++)
+        if (m <= lane) a[m] -= l * dj * scol[m];
+      __syncthreads();
     }
   }
+  if (act) {
+    if (!(myd > 0.0)) {  // K_ii must be positive definite
+      atomicAdd(&piv[0], 1);
+      atomicMin(&piv[1], S.col0[sn] + k0 + lane);
+    }
 #pragma unroll
-  for (int j = 0; j < 64; j++)
-    if (j < kb) F[(int64_t)r + (int64_t)(k0 + j) * ld] = w[j] / sA[j][j];
+    for (int m = 0; m < 64; m++)
+      if (m <= lane) F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] = a[m];
+  }
+#pragma unroll
+  for (int m = 0; m < 64; m++) sL[lane][m] = (m < lane) ? a[m] : 0.0;
+  __syncthreads();
+  double x[64];
+#pragma unroll
+  for (int i = 0; i < 64; i++) {
+    double v = (i == lane) ? 1.0 : 0.0;
+    if (i > lane) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < i; k++) s += sL[i][k] * x[k];
+      v = -s;
+    }
+    x[i] = v;
+  }
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < 64; i++)
+      if (i >= lane && i < kb) Xg[(int64_t)(k0 + i) + (int64_t)(k0 + lane) * nc] = x[i];
+  }
+}
+
+// k_trsm task: sn, a = 64-row tile below the diagonal block, k0 = panel.
+// L_r = F_r L_kk^{-T} D^{-1}, as an FP64 MFMA product with the inverse of the
+// diagonal block built by k_diag.
+__global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, DevSym S,
+                                              double* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                              const double* __restrict__ ipool) {
+  __shared__ double sA[64][65];
+  __shared__ double sB[64][65];
+  const FTask t = tasks[blockIdx.x];
+  const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
+  const int kb = min(64, nc - k0);
+  const int R0 = k0 + kb + 64 * t.a;
+  double* F = pool + S.foff[sn];
+  const double* Xg = ipool + ioff[sn];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int kbr = (kb + 3) & ~3;
+  {
+    double ra[16], rb[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
+      ra[q] = (m < kb && R0 + i < ld) ? F[(int64_t)(R0 + i) + (int64_t)(k0 + m) * ld] : 0.0;
+      // sB[j][m] = Linv_kk[j][m] (lower)
+      rb[q] = (m < kb && i < kb && i >= m) ? Xg[(int64_t)(k0 + i) + (int64_t)(k0 + m) * nc] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
+      sA[i][m] = ra[q];
+      sB[i][m] = rb[q];
+    }
+  }
+  __syncthreads();
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  f64x4 acc[2][2];
+  zero_acc(acc);
+  mfma_tile(sA, sB, kbr, acc, lane, wr, wc);
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = R0 + wr + 16 * x + lr;
+        const int j = wc + 16 * y + lk + 4 * r;
+        if (i < ld && j < kb) F[(int64_t)i + (int64_t)(k0 + j) * ld] = acc[x][y][r] / F[(int64_t)(k0 + j) * (ld + 1)];
+      }
 }
 
 // ---------------------------------------------------------------- update
 // Task: sn, a = ti, b = tj (64x64 tiles of the trailing lower triangle),
-// k0 = panel.  F[I,J] -= (L_I D) L_J^T with the panel of width kb <= 64,
-// on v_mfma_f64_16x16x4_f64.  Four waves, each a 32x32 quarter of the tile.
-// The MFMA's A operand takes L_J rows and its B operand the D-scaled L_I rows,
-// so the 16x16 result lands with consecutive lanes on consecutive rows i of
-// one column j of the column-major front (coalesced stores).
+// k0 = panel.  F[I,J] -= (L_I D) L_J^T with the panel of width kb <= 64.
 __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks, DevSym S,
                                                 double* __restrict__ pool) {
   __shared__ double sI[64][65];
@@ -290,227 +376,312 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
   double* F = pool + S.foff[sn];
   const int tid = threadIdx.x;
   const int kbr = (kb + 3) & ~3;
-  for (int idx = tid; idx < 64 * kbr; idx += 256) {
-    int i = idx & 63, m = idx >> 6;
-    double vi = 0.0, vj = 0.0;
-    if (m < kb) {
-      double d = F[(int64_t)(k0 + m) * (ld + 1)];
-      if (I0 + i < ld) vi = F[(int64_t)(I0 + i) + (int64_t)(k0 + m) * ld] * d;
-      if (J0 + i < ld) vj = F[(int64_t)(J0 + i) + (int64_t)(k0 + m) * ld];
+  {
+    double ri[16], rj[16], rd[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
+      const bool okm = m < kb;
+      rd[q] = okm ? F[(int64_t)(k0 + m) * (ld + 1)] : 0.0;
+      ri[q] = (okm && I0 + i < ld) ? F[(int64_t)(I0 + i) + (int64_t)(k0 + m) * ld] : 0.0;
+      rj[q] = (okm && J0 + i < ld) ? F[(int64_t)(J0 + i) + (int64_t)(k0 + m) * ld] : 0.0;
     }
-    sI[i][m] = vi;
-    sJ[i][m] = vj;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
+      sI[i][m] = ri[q] * rd[q];
+      sJ[i][m] = rj[q];
+    }
   }
   __syncthreads();
   const int lane = tid & 63, wv = tid >> 6;
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
   f64x4 acc[2][2];
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++) acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
+  zero_acc(acc);
+  mfma_tile(sI, sJ, kbr, acc, lane, wr, wc);
   const int lr = lane & 15, lk = lane >> 4;
-  for (int m0 = 0; m0 < kbr; m0 += 4) {
-    double aj[2], bi[2];
-#pragma unroll
-    for (int x = 0; x < 2; x++) {
-      aj[x] = sJ[wc + 16 * x + lr][m0 + lk];
-      bi[x] = sI[wr + 16 * x + lr][m0 + lk];
-    }
-#pragma unroll
-    for (int x = 0; x < 2; x++)      // i blocks
-#pragma unroll
-      for (int y = 0; y < 2; y++)    // j blocks
-        acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(aj[y], bi[x], acc[x][y], 0, 0, 0);
-  }
 #pragma unroll
   for (int x = 0; x < 2; x++)
 #pragma unroll
     for (int y = 0; y < 2; y++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        // v_mfma_f64_16x16x4f64 result D[p][q] sits in lane q + 16*(p % 4),
-        // register p / 4; here p runs over j and q over i
         const int i = I0 + wr + 16 * x + lr;
         const int j = J0 + wc + 16 * y + lk + 4 * r;
         if (i < ld && j < ld && i >= j) F[(int64_t)i + (int64_t)j * ld] -= acc[x][y][r];
       }
 }
 
-// ---------------------------------------------------------------- solves
-// Work buffers: front s owns rows woff[s] .. woff[s]+nrow[s]-1 of a
-// row-major (rows x nrhs) array W.  Forward mode uses L, backward mode L^T.
+// ---------------------------------------------------------------- inverse
+// Explicit inverse X = L_PP^{-1} of the unit lower-triangular pivot block of
+// every front, kept in its own pool (ioff[s], ncol x ncol column-major, lower
+// triangle).  With it each triangular solve of a front is a GEMM, which turns
+// the forward/backward passes of the block Lanczos and of the B computation
+// into a few wide launches per tree level instead of a sequence of dependent
+// 64-column substitutions.
 //
-// k_solve_gemm task: W[r0+i][j0+jj] -= sum_k op(L)[r0+i][k0+k] W[k0+k][j0+jj]
-//   forward : op(L)[i][k] = F[i + k*ld]   (rows below the source block)
-//   backward: op(L)[i][k] = F[k + i*ld]   (rows above the source block)
-__global__ __launch_bounds__(256) void k_solve_gemm(const SolveTask* __restrict__ tasks, DevSym S,
-                                                    const double* __restrict__ pool,
-                                                    double* __restrict__ W, int nrhs, int backward) {
-  __shared__ double sL[64][65];
-  __shared__ double sW[64][64];
-  const SolveTask t = tasks[blockIdx.x];
-  const int sn = t.sn, ld = S.nrow[sn];
+// k_inv_off task: sn, a = tile column j, b = tile row i > j:
+//   X_ij = - X_ii * sum_{k=j}^{i-1} L_ik X_kj
+// (all tiles at smaller distance i-j are complete).
+__global__ __launch_bounds__(256) void k_inv_off(const FTask* __restrict__ tasks, DevSym S,
+                                                 const double* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                                 double* __restrict__ ipool) {
+  __shared__ double sA[64][65];
+  __shared__ double sB[64][65];
+  const FTask t = tasks[blockIdx.x];
+  const int sn = t.sn, ld = S.nrow[sn], nc = S.ncol[sn];
+  const int rj = 64 * t.a, nj = min(64, nc - rj);
+  const int ri = 64 * t.b, ni = min(64, nc - ri);
   const double* F = pool + S.foff[sn];
-  double* Wf = W + S.woff[sn] * (int64_t)nrhs;
+  double* X = ipool + ioff[sn];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int nj = min(64, nrhs - t.j0);
-  double acc[16];
-#pragma unroll
-  for (int q = 0; q < 16; q++) acc[q] = 0.0;
-  for (int kc = 0; kc < t.kn; kc += 64) {
-    const int kn = min(64, t.kn - kc);
-    __syncthreads();
-    for (int idx = tid; idx < 64 * 64; idx += 256) {
-      int i, k;
-      double v = 0.0;
-      if (!backward) {
-        i = idx & 63; k = idx >> 6;
-        if (i < t.rn && k < kn) v = F[(int64_t)(t.r0 + i) + (int64_t)(t.k0 + kc + k) * ld];
-      } else {
-        k = idx & 63; i = idx >> 6;
-        if (i < t.rn && k < kn) v = F[(int64_t)(t.k0 + kc + k) + (int64_t)(t.r0 + i) * ld];
-      }
-      sL[i][k] = v;
-      int kk = idx >> 6, jj = idx & 63;
-      sW[kk][jj] = (kk < kn && jj < nj) ? Wf[(int64_t)(t.k0 + kc + kk) * nrhs + t.j0 + jj] : 0.0;
-    }
-    __syncthreads();
-    for (int k = 0; k < kn; k++) {
-      const double wk = sW[k][lane];
-#pragma unroll
-      for (int q = 0; q < 16; q++) acc[q] += sL[wv + 4 * q][k] * wk;
-    }
-  }
-  if (lane < nj) {
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  f64x4 acc[2][2];
+  zero_acc(acc);
+  for (int kt = t.a; kt < t.b; kt++) {
+    const int rk = 64 * kt, nk = min(64, nc - rk);
+    double ra[16], rb[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      int i = wv + 4 * q;
-      if (i < t.rn) Wf[(int64_t)(t.r0 + i) * nrhs + t.j0 + lane] -= acc[q];
+      const int idx = tid + 256 * q, r = idx & 63, m = idx >> 6;
+      // sA[r][m] = L[ri+r][rk+m]  (rows of tile i, columns of tile k)
+      ra[q] = (r < ni && m < nk) ? F[(int64_t)(ri + r) + (int64_t)(rk + m) * ld] : 0.0;
+      // sB[c][m] = X[rk+m][rj+c]  (transposed, lower part only on the diagonal tile)
+      const int c = idx >> 6, mm = idx & 63;
+      const bool ok = c < nj && mm < nk && (kt > t.a || mm >= c);
+      rb[q] = ok ? X[(int64_t)(rk + mm) + (int64_t)(rj + c) * nc] : 0.0;
     }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q;
+      sA[idx & 63][idx >> 6] = ra[q];
+      sB[idx >> 6][idx & 63] = rb[q];
+    }
+    __syncthreads();
+    mfma_tile(sA, sB, 64, acc, lane, wr, wc);
   }
-}
-
-// k_solve_diag task: unit-triangular solve of the rn x rn diagonal block at
-// rows r0 for right-hand sides j0..j0+63 (thread per right-hand side).
-__global__ __launch_bounds__(64) void k_solve_diag(const SolveTask* __restrict__ tasks, DevSym S,
-                                                   const double* __restrict__ pool,
-                                                   double* __restrict__ W, int nrhs, int backward) {
-  __shared__ double sL[64][65];
-  const SolveTask t = tasks[blockIdx.x];
-  const int sn = t.sn, ld = S.nrow[sn], n = t.rn;
-  const double* F = pool + S.foff[sn];
-  double* Wf = W + S.woff[sn] * (int64_t)nrhs;
-  const int lane = threadIdx.x;
-  for (int idx = lane; idx < n * n; idx += 64) {
-    int i = idx % n, j = idx / n;
-    sL[i][j] = F[(int64_t)(t.r0 + i) + (int64_t)(t.r0 + j) * ld];
+  // T = acc (rows i, cols j) -> LDS as sB[c][m] = T[m][c] for the second product
+  __syncthreads();
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) sB[wc + 16 * y + lk + 4 * r][wr + 16 * x + lr] = acc[x][y][r];
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    int r = idx & 63, m = idx >> 6;
+    sA[r][m] = (r < ni && m < ni && r >= m) ? X[(int64_t)(ri + r) + (int64_t)(ri + m) * nc] : 0.0;
   }
   __syncthreads();
-  const int j = t.j0 + lane;
-  if (j >= nrhs) return;
-  double w[64];
+  zero_acc(acc);
+  mfma_tile(sA, sB, 64, acc, lane, wr, wc);
 #pragma unroll
-  for (int i = 0; i < 64; i++) w[i] = (i < n) ? Wf[(int64_t)(t.r0 + i) * nrhs + j] : 0.0;
-  if (!backward) {
+  for (int x = 0; x < 2; x++)
 #pragma unroll
-    for (int i = 1; i < 64; i++) {
-      double s = w[i];
+    for (int y = 0; y < 2; y++)
 #pragma unroll
-      for (int m = 0; m < 64; m++)
-        if (m < i) s -= sL[i][m] * w[m];
-      w[i] = s;
-    }
-  } else {
-#pragma unroll
-    for (int i = 62; i >= 0; i--) {
-      double s = w[i];
-#pragma unroll
-      for (int m = 1; m < 64; m++)
-        if (m > i && m < n) s -= sL[m][i] * w[m];
-      w[i] = s;
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 64; i++)
-    if (i < n) Wf[(int64_t)(t.r0 + i) * nrhs + j] = w[i];
+      for (int r = 0; r < 4; r++) {
+        const int i = wr + 16 * x + lr, c = wc + 16 * y + lk + 4 * r;
+        if (i < ni && c < nj) X[(int64_t)(ri + i) + (int64_t)(rj + c) * nc] = -acc[x][y][r];
+      }
 }
 
-// Forward-solve front initialisation (one workgroup per front): pivot rows
-// from the global right-hand side R (row-major, perm rows), update rows
-// zero, then the children's update parts added in list order.
-__global__ __launch_bounds__(256) void k_fw_init(const int* __restrict__ sns, DevSym S,
-                                                 const double* __restrict__ R,
-                                                 double* __restrict__ W, int nrhs, int ndof1) {
-  const int sn = sns[blockIdx.x];
-  const int nc = S.ncol[sn], nr = S.nrow[sn], c0 = S.col0[sn];
+// ---------------------------------------------------------------- solves
+// Multi-vectors are row-major (rows x nrhs).  Work buffer W holds, for front
+// s, rows woff[s] .. woff[s]+nrow[s]-1; V holds the solved pivot rows before
+// D-scaling.  A k_mv task multiplies target rows r0..r0+rn-1 (rn <= 64) and
+// right-hand sides j0..j0+63 of front sn over the source slice k0..k0+kn-1:
+//   mode 0 FW_INV : V[r]      =  sum_{k<=r}   Linv[r][k]   W[k]
+//   mode 1 FW_UPD : W[nc+r]  -=  sum_{k<nc}   L[nc+r][k]   V[k]
+//   mode 2 BW_UPD : W[r]     -=  sum_{k<nu}   L[nc+k][r]   W[nc+k]
+//   mode 3 BW_INV : X[c0+r]   =  sum_{k>=r}   Linv[k][r]   W[k]
+// Long K ranges are split over several tasks (slot >= 0) whose 64x64
+// partials k_mv_red then sums in slot order, so the result does not depend
+// on scheduling.
+__device__ __forceinline__ void mv_apply(int mode, const DevSym& S, int sn, int nc, int r, int j, int nrhs,
+                                         double v, double* Wf, double* Vf, double* X) {
+  switch (mode) {
+    case 0: Vf[(int64_t)r * nrhs + j] = v; break;
+    case 1: Wf[(int64_t)(nc + r) * nrhs + j] -= v; break;
+    case 2: Wf[(int64_t)r * nrhs + j] -= v; break;
+    default: X[(int64_t)(S.col0[sn] + r) * nrhs + j] = v; break;
+  }
+}
+
+template <int MODE, int NJ>
+__global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks, DevSym S,
+                                            const double* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                            const double* __restrict__ ipool, double* __restrict__ W,
+                                            double* __restrict__ V, double* __restrict__ X,
+                                            double* __restrict__ part, int nrhs) {
+  __shared__ double sA[64][65];
+  __shared__ double sB[64][64];
+  const SolveTask t = tasks[blockIdx.x];
+  const int sn = t.sn, ld = S.nrow[sn], nc = S.ncol[sn];
+  const double* F = pool + S.foff[sn];
+  const double* Xi = ipool + ioff[sn];
+  double* Wf = W + S.woff[sn] * (int64_t)nrhs;
+  double* Vf = V + S.woff[sn] * (int64_t)nrhs;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nj = min(NJ, nrhs - t.j0);
+  const int r0 = t.r0, rn = t.rn;
+  const double* Bsrc = MODE == 1 ? Vf : (MODE == 2 ? Wf + (int64_t)nc * nrhs : Wf);
+  constexpr bool rowmajorA = (MODE == 0 || MODE == 1);
+  constexpr int CPT = NJ / 4;  // right-hand sides per thread: wv*CPT .. wv*CPT+CPT-1
+  double acc[CPT];
+#pragma unroll
+  for (int u = 0; u < CPT; u++) acc[u] = 0.0;
+  const int kend = t.k0 + t.kn;
+  for (int kc = t.k0; kc < kend; kc += 64) {
+    const int kn = min(64, kend - kc);
+    double ra[16], rb[16];
+    // issue every global load of the chunk before the first LDS store
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q;
+      const int i = rowmajorA ? (idx & 63) : (idx >> 6);
+      const int k = rowmajorA ? (idx >> 6) : (idx & 63);
+      const int r = r0 + i, kk = kc + k;
+      double v = 0.0;
+      if (i < rn && k < kn) {
+        if (MODE == 0) v = kk <= r ? Xi[(int64_t)r + (int64_t)kk * nc] : 0.0;
+        else if (MODE == 1) v = F[(int64_t)(nc + r) + (int64_t)kk * ld];
+        else if (MODE == 2) v = F[(int64_t)(nc + kk) + (int64_t)r * ld];
+        else v = kk >= r ? Xi[(int64_t)kk + (int64_t)r * nc] : 0.0;
+      }
+      ra[q] = v;
+      const int kb = idx >> 6, jb = idx & 63;
+      rb[q] = (kb < kn && jb < nj) ? Bsrc[(int64_t)(kc + kb) * nrhs + t.j0 + jb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q;
+      const int i = rowmajorA ? (idx & 63) : (idx >> 6);
+      const int k = rowmajorA ? (idx >> 6) : (idx & 63);
+      sA[i][k] = ra[q];
+      sB[idx >> 6][idx & 63] = rb[q];
+    }
+    __syncthreads();
+    // sB is zero beyond nj, so every thread runs the same unguarded loop
+#pragma unroll 4
+    for (int k = 0; k < kn; k++) {
+      const double a = sA[lane][k];
+#pragma unroll
+      for (int u = 0; u < CPT; u++) acc[u] += a * sB[k][wv * CPT + u];
+    }
+  }
+  if (t.slot >= 0) {
+    double* P = part + (int64_t)t.slot * 4096;
+#pragma unroll
+    for (int u = 0; u < CPT; u++) P[lane * 64 + wv * CPT + u] = acc[u];
+    return;
+  }
+  if (lane >= rn) return;
+#pragma unroll
+  for (int u = 0; u < CPT; u++) {
+    const int jj = wv * CPT + u;
+    if (jj < nj) mv_apply(MODE, S, sn, nc, r0 + lane, t.j0 + jj, nrhs, acc[u], Wf, Vf, X);
+  }
+}
+
+// Reduce task: slots k0 .. k0+kn-1 hold the partials of rows r0.. (rn) and
+// right-hand sides j0.. of front sn; sum them in slot order and apply.
+__global__ __launch_bounds__(256) void k_mv_red(const SolveTask* __restrict__ tasks, DevSym S,
+                                                double* __restrict__ W, double* __restrict__ V,
+                                                double* __restrict__ X, const double* __restrict__ part,
+                                                int nrhs, int mode) {
+  const SolveTask t = tasks[blockIdx.x];
+  const int sn = t.sn, nc = S.ncol[sn];
+  double* Wf = W + S.woff[sn] * (int64_t)nrhs;
+  double* Vf = V + S.woff[sn] * (int64_t)nrhs;
+  const int nj = min(64, nrhs - t.j0);
+  for (int idx = threadIdx.x; idx < t.rn * 64; idx += 256) {
+    const int i = idx >> 6, jj = idx & 63;
+    if (jj >= nj) continue;
+    double v = 0.0;
+    for (int q = 0; q < t.kn; q++) v += part[(int64_t)(t.k0 + q) * 4096 + i * 64 + jj];
+    mv_apply(mode, S, sn, nc, t.r0 + i, t.j0 + jj, nrhs, v, Wf, Vf, X);
+  }
+}
+
+// Forward-solve front initialisation, task = (front, 64-row chunk r0): pivot
+// rows from the global right-hand side R (row-major, perm rows; the Schur
+// front takes the boundary rows), update rows zero, then the children's
+// update rows falling in the chunk added child after child.
+__global__ __launch_bounds__(256) void k_fw_init(const SolveTask* __restrict__ tasks, DevSym S,
+                                                 const double* __restrict__ R, double* __restrict__ W, int nrhs,
+                                                 int ndof1) {
+  const SolveTask t = tasks[blockIdx.x];
+  const int sn = t.sn, nc = S.ncol[sn], c0 = S.col0[sn];
+  const int r0 = t.r0, r1 = t.r0 + t.rn;
   double* Wf = W + S.woff[sn] * (int64_t)nrhs;
   const int tid = threadIdx.x;
-  for (int64_t idx = tid; idx < (int64_t)nr * nrhs; idx += 256) {
-    int i = (int)(idx / nrhs), j = (int)(idx % nrhs);
+  const int base = nc > 0 ? c0 : ndof1;
+  for (int idx = tid; idx < t.rn * nrhs; idx += 256) {
+    const int i = r0 + idx / nrhs, j = idx % nrhs;
     double v = 0.0;
-    if (i < nc) v = R[(int64_t)(c0 + i) * nrhs + j];
-    else if (nc == 0) v = R[(int64_t)(ndof1 + i) * nrhs + j];  // Schur front: boundary rows
-    Wf[idx] = v;
+    if (i < nc || nc == 0) v = R[(int64_t)(base + i) * nrhs + j];
+    Wf[(int64_t)i * nrhs + j] = v;
   }
   for (int q = S.chptr[sn]; q < S.chptr[sn + 1]; q++) {
-    __syncthreads();
     const int c = S.chlist[q];
     const int ncc = S.ncol[c], nu = S.nrow[c] - ncc;
     const int* rel = S.rel + S.relptr[c];
+    int lo = 0, hi = nu;
+    while (lo < hi) { int m = (lo + hi) >> 1; if (rel[m] < r0) lo = m + 1; else hi = m; }
+    const int ka = lo;
+    hi = nu;
+    while (lo < hi) { int m = (lo + hi) >> 1; if (rel[m] < r1) lo = m + 1; else hi = m; }
+    const int kb = lo;
     const double* Wc = W + (S.woff[c] + ncc) * (int64_t)nrhs;
-    for (int64_t idx = tid; idx < (int64_t)nu * nrhs; idx += 256) {
-      int i = (int)(idx / nrhs), j = (int)(idx % nrhs);
-      Wf[(int64_t)rel[i] * nrhs + j] += Wc[idx];
+    __syncthreads();
+    for (int idx = tid; idx < (kb - ka) * nrhs; idx += 256) {
+      const int k = ka + idx / nrhs, j = idx % nrhs;
+      Wf[(int64_t)rel[k] * nrhs + j] += Wc[(int64_t)k * nrhs + j];
     }
   }
 }
 
-// Copy the solved pivot rows of each front out to Y (optionally scaled by
-// 1/D, the diagonal of the factor), and for the Schur front (ncol == 0) its
-// rows to the boundary rows of Y.
-__global__ __launch_bounds__(256) void k_fw_out(const int* __restrict__ sns, DevSym S,
+// Copy solved rows out to Y, task = (front, 64-row chunk): internal fronts
+// give their pivot rows (V, optionally scaled by 1/D), the Schur front its
+// rows (W) to the boundary rows of Y.
+__global__ __launch_bounds__(256) void k_fw_out(const SolveTask* __restrict__ tasks, DevSym S,
                                                 const double* __restrict__ pool,
-                                                const double* __restrict__ W,
+                                                const double* __restrict__ W, const double* __restrict__ V,
                                                 double* __restrict__ Y, int nrhs, int scale, int ndof1) {
-  const int sn = sns[blockIdx.x];
-  const int nc = S.ncol[sn], nr = S.nrow[sn], c0 = S.col0[sn];
+  const SolveTask t = tasks[blockIdx.x];
+  const int sn = t.sn, nc = S.ncol[sn], nr = S.nrow[sn], c0 = S.col0[sn];
   const double* F = pool + S.foff[sn];
-  const double* Wf = W + S.woff[sn] * (int64_t)nrhs;
-  const int n = nc > 0 ? nc : nr;
+  const double* Wf = (nc > 0 ? V : W) + S.woff[sn] * (int64_t)nrhs;
   const int base = nc > 0 ? c0 : ndof1;
-  for (int64_t idx = threadIdx.x; idx < (int64_t)n * nrhs; idx += 256) {
-    int i = (int)(idx / nrhs);
-    double v = Wf[idx];
+  for (int idx = threadIdx.x; idx < t.rn * nrhs; idx += 256) {
+    const int i = t.r0 + idx / nrhs, j = idx % nrhs;
+    double v = Wf[(int64_t)i * nrhs + j];
     if (scale && nc > 0) v /= F[(int64_t)i * (nr + 1)];
-    Y[(int64_t)(base + i) * nrhs + idx % nrhs] = v;
+    Y[(int64_t)(base + i) * nrhs + j] = v;
   }
 }
 
-// Backward-solve front initialisation: pivot rows from Y, update rows
-// gathered from the already final rows of X (ancestors and boundary).
-__global__ __launch_bounds__(256) void k_bw_init(const int* __restrict__ sns, DevSym S,
+// Backward-solve front initialisation, task = (front, 64-row chunk): pivot
+// rows from Y (zero when Y is null), update rows gathered from the already
+// final rows of X (ancestors and boundary).
+__global__ __launch_bounds__(256) void k_bw_init(const SolveTask* __restrict__ tasks, DevSym S,
                                                  const double* __restrict__ Y,
                                                  const double* __restrict__ X,
                                                  double* __restrict__ W, int nrhs) {
-  const int sn = sns[blockIdx.x];
-  const int nc = S.ncol[sn], nr = S.nrow[sn], c0 = S.col0[sn];
+  const SolveTask t = tasks[blockIdx.x];
+  const int sn = t.sn, nc = S.ncol[sn], c0 = S.col0[sn];
   const int* rows = S.rows + S.rptr[sn];
   double* Wf = W + S.woff[sn] * (int64_t)nrhs;
-  for (int64_t idx = threadIdx.x; idx < (int64_t)nr * nrhs; idx += 256) {
-    int i = (int)(idx / nrhs), j = (int)(idx % nrhs);
-    Wf[idx] = (i < nc) ? Y[(int64_t)(c0 + i) * nrhs + j] : X[(int64_t)rows[i] * nrhs + j];
+  for (int idx = threadIdx.x; idx < t.rn * nrhs; idx += 256) {
+    const int i = t.r0 + idx / nrhs, j = idx % nrhs;
+    double v;
+    if (i < nc) v = Y ? Y[(int64_t)(c0 + i) * nrhs + j] : 0.0;
+    else v = X[(int64_t)rows[i] * nrhs + j];
+    Wf[(int64_t)i * nrhs + j] = v;
   }
-}
-
-__global__ __launch_bounds__(256) void k_bw_out(const int* __restrict__ sns, DevSym S,
-                                                const double* __restrict__ W,
-                                                double* __restrict__ X, int nrhs) {
-  const int sn = sns[blockIdx.x];
-  const int nc = S.ncol[sn], c0 = S.col0[sn];
-  const double* Wf = W + S.woff[sn] * (int64_t)nrhs;
-  for (int64_t idx = threadIdx.x; idx < (int64_t)nc * nrhs; idx += 256)
-    X[(int64_t)c0 * nrhs + idx] = Wf[idx];
 }
 
 // ---------------------------------------------------------------- products
@@ -523,58 +694,83 @@ __global__ __launch_bounds__(256) void k_spmm(int n, int nrhs, const int* __rest
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int r = (int)(g / nrhs), j = (int)(g % nrhs);
   if (r >= n) return;
-  double s = 0.0;
-  for (int k = rowptr[r]; k < rowptr[r + 1]; k++) s += val[k] * X[(int64_t)colind[k] * ldx + j];
-  Z[(int64_t)r * ldz + j] = s;
+  const int kb = rowptr[r], ke = rowptr[r + 1];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int k = kb;
+  for (; k + 4 <= ke; k += 4) {
+    const int c0 = colind[k], c1 = colind[k + 1], c2 = colind[k + 2], c3 = colind[k + 3];
+    const double v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
+    s0 += v0 * X[(int64_t)c0 * ldx + j];
+    s1 += v1 * X[(int64_t)c1 * ldx + j];
+    s2 += v2 * X[(int64_t)c2 * ldx + j];
+    s3 += v3 * X[(int64_t)c3 * ldx + j];
+  }
+  for (; k < ke; k++) s0 += val[k] * X[(int64_t)colind[k] * ldx + j];
+  Z[(int64_t)r * ldz + j] = (s0 + s1) + (s2 + s3);
 }
 
-// Partial C = X^T Y: blockIdx.x = chunk of XTY_ROWS rows, blockIdx.y/z = 64x64
-// tile of C.  part[chunk][a][b] (na x nb row-major per chunk).
-#define XTY_ROWS 1024
+// Partial C = X^T Y: blockIdx.x = chunk of XTY_ROWS rows, blockIdx.y = 64
+// columns of X (a), blockIdx.z = 64 columns of Y (b); lane -> a, wave -> b
+// (b = 16*wave + u).  part[chunk][a][b] (na x nb row-major per chunk).
+#define XTY_ROWS 128
 __global__ __launch_bounds__(256) void k_xty_part(int n, int na, int nb, const double* __restrict__ X,
                                                   int ldx, const double* __restrict__ Y, int ldy,
                                                   double* __restrict__ part) {
   __shared__ double sX[32][64];
   __shared__ double sY[32][64];
   const int r0 = blockIdx.x * XTY_ROWS, a0 = blockIdx.y * 64, b0 = blockIdx.z * 64;
-  const int tid = threadIdx.x, lane = tid & 63, bg = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nbt = min(64, nb - b0), nat = min(64, na - a0);
   double acc[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) acc[q] = 0.0;
   const int rend = min(n, r0 + XTY_ROWS);
   for (int rc = r0; rc < rend; rc += 32) {
-    __syncthreads();
-    for (int idx = tid; idx < 32 * 64; idx += 256) {
-      int rr = idx >> 6, c = idx & 63, r = rc + rr;
-      bool ok = r < rend;
-      sX[rr][c] = (ok && a0 + c < na) ? X[(int64_t)r * ldx + a0 + c] : 0.0;
-      sY[rr][c] = (ok && b0 + c < nb) ? Y[(int64_t)r * ldy + b0 + c] : 0.0;
+    double rx[8], ry[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int idx = tid + 256 * q, rr = idx >> 6, c = idx & 63, r = rc + rr;
+      const bool ok = r < rend;
+      rx[q] = (ok && c < nat) ? X[(int64_t)r * ldx + a0 + c] : 0.0;
+      ry[q] = (ok && c < nbt) ? Y[(int64_t)r * ldy + b0 + c] : 0.0;
     }
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int idx = tid + 256 * q;
+      sX[idx >> 6][idx & 63] = rx[q];
+      sY[idx >> 6][idx & 63] = ry[q];
+    }
+    __syncthreads();
+    // sY is zero beyond nbt: same unguarded loop in every thread
 #pragma unroll 4
     for (int rr = 0; rr < 32; rr++) {
       const double xv = sX[rr][lane];
 #pragma unroll
-      for (int q = 0; q < 16; q++) acc[q] += xv * sY[rr][bg + 4 * q];
+      for (int q = 0; q < 16; q++) acc[q] += xv * sY[rr][wv * 16 + q];
     }
   }
   double* P = part + (int64_t)blockIdx.x * na * nb;
-  const int a = a0 + lane;
-  if (a < na) {
+  if (lane < nat) {
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-      int b = b0 + bg + 4 * q;
-      if (b < nb) P[(int64_t)a * nb + b] = acc[q];
+      int b = wv * 16 + q;
+      if (b < nbt) P[(int64_t)(a0 + lane) * nb + b0 + b] = acc[q];
     }
   }
 }
 
-__global__ void k_xty_sum(int nchunk, int nab, const double* __restrict__ part, double* __restrict__ C) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per output element: lane l sums chunks l, l+64, ... then a fixed
+// butterfly combines the lanes (same order on every run).
+__global__ __launch_bounds__(256) void k_xty_sum(int nchunk, int nab, const double* __restrict__ part,
+                                                 double* __restrict__ C) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (e >= nab) return;
   double s = 0.0;
-  for (int c = 0; c < nchunk; c++) s += part[(int64_t)c * nab + e];
-  C[e] = s;
+  for (int c = lane; c < nchunk; c += 64) s += part[(int64_t)c * nab + e];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) C[e] = s;
 }
 
 // Y[r][j] = alpha*Y[r][j] + beta * sum_a X[r][a] S[a][j]; S is (na x nb) row-major.
@@ -621,35 +817,50 @@ void launch_fill(hipStream_t st, double* p, int64_t n, double v) {
 void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool) {
   if (n > 0) k_extend_add<<<n, 256, 0, st>>>(t, S, pool);
 }
-void launch_panel(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, int* piv) {
-  if (n > 0) k_panel<<<n, 256, 0, st>>>(t, S, pool, piv);
+void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, const int64_t* ioff,
+                 double* ipool, int* piv) {
+  if (n > 0) k_diag<<<n, 64, 0, st>>>(t, S, pool, ioff, ipool, piv);
+}
+void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, const int64_t* ioff,
+                 const double* ipool) {
+  if (n > 0) k_trsm<<<n, 256, 0, st>>>(t, S, pool, ioff, ipool);
 }
 void launch_update(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool) {
   if (n > 0) k_update<<<n, 256, 0, st>>>(t, S, pool);
 }
-void launch_solve_gemm(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
-                       double* W, int nrhs, int backward) {
-  if (n > 0) k_solve_gemm<<<n, 256, 0, st>>>(t, S, pool, W, nrhs, backward);
+void launch_inv_off(hipStream_t st, const FTask* t, int n, const DevSym& S, const double* pool,
+                    const int64_t* ioff, double* ipool) {
+  if (n > 0) k_inv_off<<<n, 256, 0, st>>>(t, S, pool, ioff, ipool);
 }
-void launch_solve_diag(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
-                       double* W, int nrhs, int backward) {
-  if (n > 0) k_solve_diag<<<n, 64, 0, st>>>(t, S, pool, W, nrhs, backward);
+void launch_mv(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool, const int64_t* ioff,
+               const double* ipool, double* W, double* V, double* X, double* part, int nrhs, int mode) {
+  if (n <= 0) return;
+#define FGPU_MV(M)                                                                   \
+  if (nrhs <= 16) k_mv<M, 16><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); \
+  else k_mv<M, 64><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs);
+  switch (mode) {
+    case 0: FGPU_MV(0) break;
+    case 1: FGPU_MV(1) break;
+    case 2: FGPU_MV(2) break;
+    default: FGPU_MV(3) break;
+  }
+#undef FGPU_MV
 }
-void launch_fw_init(hipStream_t st, const int* sns, int n, const DevSym& S, const double* R, double* W,
+void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, double* W, double* V, double* X,
+                   const double* part, int nrhs, int mode) {
+  if (n > 0) k_mv_red<<<n, 256, 0, st>>>(t, S, W, V, X, part, nrhs, mode);
+}
+void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* R, double* W,
                     int nrhs, int ndof1) {
-  if (n > 0) k_fw_init<<<n, 256, 0, st>>>(sns, S, R, W, nrhs, ndof1);
+  if (n > 0) k_fw_init<<<n, 256, 0, st>>>(t, S, R, W, nrhs, ndof1);
 }
-void launch_fw_out(hipStream_t st, const int* sns, int n, const DevSym& S, const double* pool,
-                   const double* W, double* Y, int nrhs, int scale, int ndof1) {
-  if (n > 0) k_fw_out<<<n, 256, 0, st>>>(sns, S, pool, W, Y, nrhs, scale, ndof1);
+void launch_fw_out(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
+                   const double* W, const double* V, double* Y, int nrhs, int scale, int ndof1) {
+  if (n > 0) k_fw_out<<<n, 256, 0, st>>>(t, S, pool, W, V, Y, nrhs, scale, ndof1);
 }
-void launch_bw_init(hipStream_t st, const int* sns, int n, const DevSym& S, const double* Y,
+void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y,
                     const double* X, double* W, int nrhs) {
-  if (n > 0) k_bw_init<<<n, 256, 0, st>>>(sns, S, Y, X, W, nrhs);
-}
-void launch_bw_out(hipStream_t st, const int* sns, int n, const DevSym& S, const double* W, double* X,
-                   int nrhs) {
-  if (n > 0) k_bw_out<<<n, 256, 0, st>>>(sns, S, W, X, nrhs);
+  if (n > 0) k_bw_init<<<n, 256, 0, st>>>(t, S, Y, X, W, nrhs);
 }
 void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* colind, const double* val,
                  const double* X, int ldx, double* Z, int ldz) {
@@ -664,7 +875,7 @@ void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx,
   if (nch == 0) { launch_fill(st, C, (int64_t)na * nb, 0.0); return; }
   dim3 g(nch, (na + 63) / 64, (nb + 63) / 64);
   k_xty_part<<<g, 256, 0, st>>>(n, na, nb, X, ldx, Y, ldy, part);
-  k_xty_sum<<<nblk((int64_t)na * nb, 256), 256, 0, st>>>(nch, na * nb, part, C);
+  k_xty_sum<<<nblk((int64_t)na * nb, 4), 256, 0, st>>>(nch, na * nb, part, C);
 }
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta) {

@@ -12,20 +12,24 @@ void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t
                      const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool);
 void launch_fill(hipStream_t st, double* p, int64_t n, double v);
 void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool);
-void launch_panel(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, int* piv);
+void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, const int64_t* ioff,
+                 double* ipool, int* piv);
+void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, const int64_t* ioff,
+                 const double* ipool);
 void launch_update(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool);
-void launch_solve_gemm(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
-                       double* W, int nrhs, int backward);
-void launch_solve_diag(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
-                       double* W, int nrhs, int backward);
-void launch_fw_init(hipStream_t st, const int* sns, int n, const DevSym& S, const double* R, double* W,
+void launch_inv_off(hipStream_t st, const FTask* t, int n, const DevSym& S, const double* pool,
+                    const int64_t* ioff, double* ipool);
+// solve GEMM tasks, mode 0 FW_INV, 1 FW_UPD, 2 BW_UPD, 3 BW_INV
+void launch_mv(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool, const int64_t* ioff,
+               const double* ipool, double* W, double* V, double* X, double* part, int nrhs, int mode);
+void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, double* W, double* V, double* X,
+                   const double* part, int nrhs, int mode);
+void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* R, double* W,
                     int nrhs, int ndof1);
-void launch_fw_out(hipStream_t st, const int* sns, int n, const DevSym& S, const double* pool,
-                   const double* W, double* Y, int nrhs, int scale, int ndof1);
-void launch_bw_init(hipStream_t st, const int* sns, int n, const DevSym& S, const double* Y,
+void launch_fw_out(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
+                   const double* W, const double* V, double* Y, int nrhs, int scale, int ndof1);
+void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y,
                     const double* X, double* W, int nrhs);
-void launch_bw_out(hipStream_t st, const int* sns, int n, const DevSym& S, const double* W, double* X,
-                   int nrhs);
 void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* colind, const double* val,
                  const double* X, int ldx, double* Z, int ldz);
 int xty_chunks(int n);

@@ -55,16 +55,22 @@ struct DBuf {
   }
 };
 
-enum OpKind { EXTADD, PANEL, UPDATE, FW_INIT, FW_DIAG, FW_GEMM, FW_OUT, BW_INIT, BW_DIAG, BW_GEMM, BW_OUT };
+enum OpKind {
+  EXTADD, DIAG, TRSM, UPDATE, INV_OFF,
+  FW_INIT, FW_INV, FW_UPD, FW_OUT, BW_INIT, BW_UPD, BW_INV,
+  FW_INV_R, FW_UPD_R, BW_UPD_R, BW_INV_R  // ordered reductions of split-K partials
+};
 struct Op {
   int kind, off, n;
 };
 
 struct SolvePlan {
   int nrhs = 0;
+  int nslots = 0;  // 64x64 partial buffers of the split-K solve tasks
   std::vector<Op> fw_int, fw_all, bw;
   DBuf<SolveTask> tasks;
   DBuf<int> sns;
+  DBuf<double> part;
 };
 
 struct DeviceReducer::Impl {
@@ -74,13 +80,13 @@ struct DeviceReducer::Impl {
   DBuf<int64_t> eoff;
   // structure
   DBuf<int> col0, ncol, nrow, rptr, rows, chptr, chlist, relptr, rel, rowptr, colind, cptr;
-  DBuf<int64_t> foff, woff, cidx, fmap;
+  DBuf<int64_t> foff, woff, cidx, fmap, ioff;
   DevSym S{};
   DBuf<FTask> ftasks;
   std::vector<Op> fops;
   std::map<int, std::unique_ptr<SolvePlan>> plans;
   // values
-  DBuf<double> Kp, Mp, Kv, Mv, pool, W;
+  DBuf<double> Kp, Mp, Kv, Mv, pool, W, V, ipool;
   DBuf<int> ebad, piv;
   DBuf<double> X, Z, Y, part, small, acc, grav;
   // component modes
@@ -88,6 +94,7 @@ struct DeviceReducer::Impl {
   std::vector<int> hexl;
   int64_t wrows = 0;
   hipEvent_t ev[12]{};
+  std::vector<hipEvent_t> upd_ev;  // start/stop pairs of the trailing-update launches
 };
 
 DeviceReducer::DeviceReducer() : d_(new Impl) {}
@@ -96,6 +103,7 @@ DeviceReducer::~DeviceReducer() {
     (void)hipStreamSynchronize(st_);
     for (auto& e : d_->ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : d_->upd_ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(st_);
   }
 }
@@ -110,7 +118,7 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
     for (int q = 0; q < ns; q++) {
       int s = sups[q];
       if (y.chptr[s + 1] == y.chptr[s]) continue;
-      for (int j0 = 0; j0 < y.nrow[s]; j0 += 64) tasks.push_back({s, j0, 0, 0});
+      for (int j0 = 0; j0 < y.nrow[s]; j0 += 16) tasks.push_back({s, j0, 0, 0});
     }
     if ((int)tasks.size() > off) ops.push_back({EXTADD, off, (int)tasks.size() - off});
     int maxnc = 0;
@@ -119,12 +127,17 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
       off = (int)tasks.size();
       for (int q = 0; q < ns; q++) {
         int s = sups[q];
+        if (y.ncol[s] > k0) tasks.push_back({s, 0, 0, k0});
+      }
+      if ((int)tasks.size() > off) ops.push_back({DIAG, off, (int)tasks.size() - off});
+      off = (int)tasks.size();
+      for (int q = 0; q < ns; q++) {
+        int s = sups[q];
         if (y.ncol[s] <= k0) continue;
         int kb = std::min(64, y.ncol[s] - k0), rem = y.nrow[s] - k0 - kb;
-        int nch = std::max(1, (rem + 255) / 256);
-        for (int a = 0; a < nch; a++) tasks.push_back({s, a, 0, k0});
+        for (int a = 0; 64 * a < rem; a++) tasks.push_back({s, a, 0, k0});
       }
-      if ((int)tasks.size() > off) ops.push_back({PANEL, off, (int)tasks.size() - off});
+      if ((int)tasks.size() > off) ops.push_back({TRSM, off, (int)tasks.size() - off});
       off = (int)tasks.size();
       for (int q = 0; q < ns; q++) {
         int s = sups[q];
@@ -138,100 +151,99 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
       }
       if ((int)tasks.size() > off) ops.push_back({UPDATE, off, (int)tasks.size() - off});
     }
+    // explicit inverse of every pivot block: the diagonal tiles come from
+    // k_diag, the tiles at growing distance from the diagonal follow
+    int maxT = 0;
+    for (int q = 0; q < ns; q++) maxT = std::max(maxT, (y.ncol[sups[q]] + 63) / 64);
+    for (int dist = 1; dist < maxT; dist++) {
+      off = (int)tasks.size();
+      for (int q = 0; q < ns; q++) {
+        int s = sups[q], T = (y.ncol[s] + 63) / 64;
+        for (int j = 0; j + dist < T; j++) tasks.push_back({s, j, j + dist, 0});
+      }
+      if ((int)tasks.size() > off) ops.push_back({INV_OFF, off, (int)tasks.size() - off});
+    }
   }
 }
 
 static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<SolveTask>& tasks,
                              std::vector<int>& sns) {
   const int nrhs = sp.nrhs, schur = y.nsup - 1;
-  auto jtiles = [&](auto f) {
-    for (int j0 = 0; j0 < nrhs; j0 += 64) f(j0);
-  };
+  // (front, 64-row chunk) tasks over all rows (init) or the solved rows (out)
   auto add_sns = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list) {
-    if (list.empty()) return;
-    int off = (int)sns.size();
-    sns.insert(sns.end(), list.begin(), list.end());
-    ops.push_back({kind, off, (int)list.size()});
-  };
-  auto close = [&](std::vector<Op>& ops, int kind, int off) {
+    int off = (int)tasks.size();
+    for (int s : list) {
+      int nr = (kind == FW_OUT && s != schur) ? y.ncol[s] : y.nrow[s];
+      for (int r0 = 0; r0 < nr; r0 += 64) tasks.push_back({s, r0, std::min(64, nr - r0), 0, 0, 0, -1, 0});
+    }
     if ((int)tasks.size() > off) ops.push_back({kind, off, (int)tasks.size() - off});
   };
-  // forward: levels bottom-up
+  (void)sns;
+  // GEMM tasks over the row tiles of a front and all right-hand-side tiles;
+  // K ranges longer than KS are split and reduced in slot order
+  const int KS = 512;
+  auto add_tiles = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list, bool upd_rows) {
+    const int mode = kind == FW_INV ? 0 : kind == FW_UPD ? 1 : kind == BW_UPD ? 2 : 3;
+    const int rkind = kind == FW_INV ? FW_INV_R : kind == FW_UPD ? FW_UPD_R : kind == BW_UPD ? BW_UPD_R : BW_INV_R;
+    std::vector<SolveTask> red;
+    int off = (int)tasks.size();
+    for (int s : list) {
+      const int nc = y.ncol[s], nu = y.nrow[s] - nc;
+      int nr = upd_rows ? nu : nc;
+      if (kind == BW_UPD && nu == 0) continue;
+      for (int r0 = 0; r0 < nr; r0 += 64) {
+        const int rn = std::min(64, nr - r0);
+        int kb = 0, ke = 0;
+        switch (mode) {
+          case 0: kb = 0; ke = r0 + rn; break;
+          case 1: kb = 0; ke = nc; break;
+          case 2: kb = 0; ke = nu; break;
+          default: kb = r0; ke = nc; break;
+        }
+        for (int j0 = 0; j0 < nrhs; j0 += 64) {
+          if (ke - kb <= KS) {
+            tasks.push_back({s, r0, rn, kb, ke - kb, j0, -1, 0});
+            continue;
+          }
+          int first = sp.nslots;
+          for (int k = kb; k < ke; k += KS) tasks.push_back({s, r0, rn, k, std::min(KS, ke - k), j0, sp.nslots++, 0});
+          red.push_back({s, r0, rn, first, sp.nslots - first, j0, -2, 0});
+        }
+      }
+    }
+    if ((int)tasks.size() > off) ops.push_back({kind, off, (int)tasks.size() - off});
+    if (!red.empty()) {
+      off = (int)tasks.size();
+      tasks.insert(tasks.end(), red.begin(), red.end());
+      ops.push_back({rkind, off, (int)red.size()});
+    }
+  };
+  // forward: levels bottom-up; fw_all also carries the Schur front
   for (int pass = 0; pass < 2; pass++) {
     std::vector<Op>& ops = pass == 0 ? sp.fw_int : sp.fw_all;
     for (int l = 0; l < y.nlevels; l++) {
-      std::vector<int> list;
-      int maxT = 0;
+      std::vector<int> list, ilist;
       for (int q = y.lvptr[l]; q < y.lvptr[l + 1]; q++) {
         int s = y.lvsup[q];
         if (s == schur && pass == 0) continue;
         list.push_back(s);
-        maxT = std::max(maxT, (y.ncol[s] + 63) / 64);
+        if (s != schur) ilist.push_back(s);
       }
       add_sns(ops, FW_INIT, list);
-      for (int p = 0; p < maxT; p++) {
-        int off = (int)tasks.size();
-        for (int s : list) {
-          if (y.ncol[s] <= 64 * p) continue;
-          int r0 = 64 * p, rn = std::min(64, y.ncol[s] - r0);
-          jtiles([&](int j0) { tasks.push_back({s, r0, rn, 0, 0, j0}); });
-        }
-        close(ops, FW_DIAG, off);
-        off = (int)tasks.size();
-        for (int s : list) {
-          if (y.ncol[s] <= 64 * p) continue;
-          int k0 = 64 * p, kn = std::min(64, y.ncol[s] - k0);
-          for (int r0 = k0 + kn; r0 < y.nrow[s]; r0 += 64) {
-            int rn = std::min(64, y.nrow[s] - r0);
-            jtiles([&](int j0) { tasks.push_back({s, r0, rn, k0, kn, j0}); });
-          }
-        }
-        close(ops, FW_GEMM, off);
-      }
+      add_tiles(ops, FW_INV, ilist, false);
+      add_tiles(ops, FW_UPD, ilist, true);
       add_sns(ops, FW_OUT, list);
     }
   }
   // backward: levels top-down, internal fronts only
   for (int l = y.nlevels - 1; l >= 0; l--) {
     std::vector<int> list;
-    int maxT = 0;
-    for (int q = y.lvptr[l]; q < y.lvptr[l + 1]; q++) {
-      int s = y.lvsup[q];
-      if (s == schur) continue;
-      list.push_back(s);
-      maxT = std::max(maxT, (y.ncol[s] + 63) / 64);
-    }
+    for (int q = y.lvptr[l]; q < y.lvptr[l + 1]; q++)
+      if (y.lvsup[q] != schur) list.push_back(y.lvsup[q]);
     if (list.empty()) continue;
     add_sns(sp.bw, BW_INIT, list);
-    int off = (int)tasks.size();
-    for (int s : list) {
-      int nu = y.nrow[s] - y.ncol[s];
-      if (nu <= 0) continue;
-      for (int r0 = 0; r0 < y.ncol[s]; r0 += 64) {
-        int rn = std::min(64, y.ncol[s] - r0);
-        jtiles([&](int j0) { tasks.push_back({s, r0, rn, y.ncol[s], nu, j0}); });
-      }
-    }
-    close(sp.bw, BW_GEMM, off);
-    for (int p = 0; p < maxT; p++) {
-      off = (int)tasks.size();
-      for (int s : list) {
-        int T = (y.ncol[s] + 63) / 64, t = T - 1 - p;
-        if (t < 0) continue;
-        int r0 = 64 * t, rn = std::min(64, y.ncol[s] - r0);
-        jtiles([&](int j0) { tasks.push_back({s, r0, rn, 0, 0, j0}); });
-      }
-      close(sp.bw, BW_DIAG, off);
-      off = (int)tasks.size();
-      for (int s : list) {
-        int T = (y.ncol[s] + 63) / 64, t = T - 1 - p;
-        if (t <= 0) continue;
-        int k0 = 64 * t, kn = std::min(64, y.ncol[s] - k0);
-        for (int r0 = 0; r0 < k0; r0 += 64) jtiles([&](int j0) { tasks.push_back({s, r0, 64, k0, kn, j0}); });
-      }
-      close(sp.bw, BW_GEMM, off);
-    }
-    add_sns(sp.bw, BW_OUT, list);
+    add_tiles(sp.bw, BW_UPD, list, false);
+    add_tiles(sp.bw, BW_INV, list, false);
   }
 }
 
@@ -280,6 +292,10 @@ int DeviceReducer::upload(std::string& err) {
   for (int s = 0; s < y.nsup; s++) woff[s + 1] = woff[s] + y.nrow[s];
   d.wrows = woff[y.nsup];
   HIPCHK(d.woff.upload(woff));
+  std::vector<int64_t> ioff(y.nsup + 1, 0);
+  for (int s = 0; s < y.nsup; s++) ioff[s + 1] = ioff[s] + (int64_t)y.ncol[s] * y.ncol[s];
+  HIPCHK(d.ioff.upload(ioff));
+  HIPCHK(d.ipool.alloc(ioff[y.nsup]));
   HIPCHK(d.rowptr.upload(y.rowptr));
   HIPCHK(d.colind.upload(y.colind));
   HIPCHK(d.cptr.upload(y.cptr));
@@ -326,9 +342,33 @@ static int get_plan(const Symbolic& y, DeviceReducer::Impl& d, int nrhs, SolvePl
   build_solve_plan(y, *sp, tasks, sns);
   HIPCHK(sp->tasks.upload(tasks));
   HIPCHK(sp->sns.upload(sns));
+  HIPCHK(sp->part.alloc((size_t)std::max(1, sp->nslots) * 4096));
   out = sp.get();
   d.plans[nrhs] = std::move(sp);
   return 0;
+}
+
+static int ensure_work(DeviceReducer::Impl& d, int nrhs, std::string& err) {
+  HIPCHK(d.W.alloc((size_t)d.wrows * nrhs));
+  HIPCHK(d.V.alloc((size_t)d.wrows * nrhs));
+  return 0;
+}
+
+static void run_mv_op(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan& sp, const Op& op, double* X) {
+  static const int mode_of[] = {0, 1, 2, 3};
+  const SolveTask* t = sp.tasks.p + op.off;
+  switch (op.kind) {
+    case FW_INV: case FW_UPD: case BW_UPD: case BW_INV: {
+      int mode = op.kind == FW_INV ? 0 : op.kind == FW_UPD ? 1 : op.kind == BW_UPD ? 2 : 3;
+      launch_mv(st, t, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p, d.W.p, d.V.p, X, sp.part.p, sp.nrhs, mode_of[mode]);
+      break;
+    }
+    case FW_INV_R: case FW_UPD_R: case BW_UPD_R: case BW_INV_R: {
+      int mode = op.kind == FW_INV_R ? 0 : op.kind == FW_UPD_R ? 1 : op.kind == BW_UPD_R ? 2 : 3;
+      launch_mv_red(st, t, op.n, d.S, d.W.p, d.V.p, X, sp.part.p, sp.nrhs, mode);
+      break;
+    }
+  }
 }
 
 // Y = L^{-1} R (scale: D^{-1} L^{-1} R); all: also the Schur front (reduced RHS rows)
@@ -337,10 +377,11 @@ static void run_forward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan&
   const std::vector<Op>& ops = all ? sp.fw_all : sp.fw_int;
   for (const Op& op : ops) {
     switch (op.kind) {
-      case FW_INIT: launch_fw_init(st, sp.sns.p + op.off, op.n, d.S, R, d.W.p, sp.nrhs, ndof1); break;
-      case FW_DIAG: launch_solve_diag(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, sp.nrhs, 0); break;
-      case FW_GEMM: launch_solve_gemm(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, sp.nrhs, 0); break;
-      case FW_OUT: launch_fw_out(st, sp.sns.p + op.off, op.n, d.S, d.pool.p, d.W.p, Yout, sp.nrhs, scale, ndof1); break;
+      case FW_INIT: launch_fw_init(st, sp.tasks.p + op.off, op.n, d.S, R, d.W.p, sp.nrhs, ndof1); break;
+      case FW_OUT:
+        launch_fw_out(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, d.V.p, Yout, sp.nrhs, scale, ndof1);
+        break;
+      default: run_mv_op(st, d, sp, op, nullptr); break;
     }
   }
 }
@@ -348,12 +389,8 @@ static void run_forward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan&
 // X = L^{-T} Y on the internal rows; X's boundary rows must be set by the caller
 static void run_backward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan& sp, const double* Yin, double* X) {
   for (const Op& op : sp.bw) {
-    switch (op.kind) {
-      case BW_INIT: launch_bw_init(st, sp.sns.p + op.off, op.n, d.S, Yin, X, d.W.p, sp.nrhs); break;
-      case BW_DIAG: launch_solve_diag(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, sp.nrhs, 1); break;
-      case BW_GEMM: launch_solve_gemm(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, sp.nrhs, 1); break;
-      case BW_OUT: launch_bw_out(st, sp.sns.p + op.off, op.n, d.S, d.W.p, X, sp.nrhs); break;
-    }
+    if (op.kind == BW_INIT) launch_bw_init(st, sp.tasks.p + op.off, op.n, d.S, Yin, X, d.W.p, sp.nrhs);
+    else run_mv_op(st, d, sp, op, X);
   }
 }
 
@@ -398,11 +435,23 @@ int DeviceReducer::reduce(std::string& err) {
     int init[2] = {0, INT_MAX};
     HIPCHK(hipMemcpyAsync(d.piv.p, init, sizeof init, hipMemcpyHostToDevice, st));
   }
+  if (opt_.time_updates && d.upd_ev.empty()) {
+    d.upd_ev.resize(2 * upd_launches_);
+    for (auto& e : d.upd_ev) HIPCHK(hipEventCreate(&e));
+  }
+  int64_t ui = 0;
   for (const Op& op : d.fops) {
     switch (op.kind) {
       case EXTADD: launch_extend_add(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p); break;
-      case PANEL: launch_panel(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.piv.p); break;
-      case UPDATE: launch_update(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p); break;
+      case DIAG: launch_diag(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p, d.piv.p); break;
+      case TRSM: launch_trsm(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p); break;
+      case UPDATE:
+        if (opt_.time_updates) HIPCHK(hipEventRecord(d.upd_ev[2 * ui], st));
+        launch_update(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p);
+        if (opt_.time_updates) HIPCHK(hipEventRecord(d.upd_ev[2 * ui + 1], st));
+        ui++;
+        break;
+      case INV_OFF: launch_inv_off(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p); break;
     }
   }
   HIPCHK(hipGetLastError());
@@ -411,7 +460,7 @@ int DeviceReducer::reduce(std::string& err) {
   SolvePlan* pb = nullptr;
   if (n2 > 0) {
     if (get_plan(y, d, n2, pb, err)) return -3;
-    HIPCHK(d.W.alloc((size_t)d.wrows * n2));
+    if (ensure_work(d, n2, err)) return -3;
     HIPCHK(d.X.alloc((size_t)neq * n2));
     HIPCHK(d.Z.alloc((size_t)neq * n2));
     HIPCHK(d.Y.alloc((size_t)neq * n2));
@@ -442,12 +491,12 @@ int DeviceReducer::reduce(std::string& err) {
   if (nev > 0) {
     int rc = 0;
     const int bs = std::max(1, std::min(opt_.lanczos_block, n1));
-    int mcap = opt_.lanczos_maxvec > 0 ? opt_.lanczos_maxvec : std::max(6 * nev + 24, 3 * nev + 12);
+    int mcap = opt_.lanczos_maxvec > 0 ? opt_.lanczos_maxvec : std::max(10 * nev + 40, 16 * bs);
     mcap = std::min(mcap, n1);
     mcap = std::max(bs, (mcap / bs) * bs);
     SolvePlan* pl = nullptr;
     if (get_plan(y, d, bs, pl, err)) return -3;
-    HIPCHK(d.W.alloc((size_t)d.wrows * std::max(bs, n2)));
+    if (ensure_work(d, std::max(bs, n2), err)) return -3;
     HIPCHK(d.Q.alloc((size_t)neq * mcap));
     HIPCHK(d.P.alloc((size_t)neq * mcap));
     HIPCHK(d.Wl.alloc((size_t)neq * bs));
@@ -541,7 +590,7 @@ int DeviceReducer::reduce(std::string& err) {
       for (int a = 0; a < bs; a++)
         for (int b = 0; b < bs; b++) H[(size_t)(mj + a) * mcap + j * bs + b] = Rn[(size_t)a * bs + b];
       // convergence test on the projected problem once enough vectors exist
-      if (mj >= nev + bs) {
+      if (mj >= std::max(2 * nev, 3 * bs) && (j % 2 == 1 || mj + 2 * bs > mcap)) {
         std::vector<double> Hs((size_t)mj * mj);
         for (int a = 0; a < mj; a++)
           for (int b = 0; b < mj; b++) {
@@ -549,19 +598,21 @@ int DeviceReducer::reduce(std::string& err) {
             Hs[(size_t)a * mj + b] = (a <= b + bs && b <= a + bs) ? 0.5 * (h1 + h2) : 0.0;
           }
         std::vector<double> w, V;
-        sym_eig(mj, Hs, w, V);
-        // largest theta first
+        sym_eig_top(mj, Hs, nev, w, V);  // V column i: i-th largest theta
         bool ok = true;
-        for (int i = 0; i < nev && ok; i++) {
-          int c = mj - 1 - i;
-          double th = w[c], r2 = 0.0;
+        double worst = 0.0;
+        for (int i = 0; i < nev; i++) {
+          double th = w[mj - 1 - i], r2 = 0.0;
           for (int a = 0; a < bs; a++) {
             double s = 0;
-            for (int b = 0; b < bs; b++) s += Rn[(size_t)a * bs + b] * V[(size_t)(mj - bs + b) * mj + c];
+            for (int b = 0; b < bs; b++) s += Rn[(size_t)a * bs + b] * V[(size_t)(mj - bs + b) * nev + i];
             r2 += s * s;
           }
-          if (!(std::sqrt(r2) <= opt_.tol_eig * std::fabs(th))) ok = false;
+          double rel = std::sqrt(r2) / std::fabs(th);
+          worst = std::max(worst, rel);
+          if (!(rel <= opt_.tol_eig)) ok = false;
         }
+        res_.lanczos_resid = worst;
         if (ok) { conv = true; break; }
       }
     }
@@ -573,7 +624,7 @@ int DeviceReducer::reduce(std::string& err) {
         Hs[(size_t)a * m + b] = (a <= b + bs && b <= a + bs) ? 0.5 * (h1 + h2) : 0.0;
       }
     std::vector<double> w, V;
-    sym_eig(m, Hs, w, V);
+    sym_eig_top(m, Hs, nev, w, V);
     (void)conv;
     res_.lanczos_vectors = m;
     // Ritz vectors for the nev largest theta (lowest lambda), lambda ascending
@@ -581,11 +632,10 @@ int DeviceReducer::reduce(std::string& err) {
     res_.lambda.resize(nev);
     res_.eval.resize(nev);
     for (int i = 0; i < nev; i++) {
-      int c = m - 1 - i;
-      double lam = 1.0 / w[c] + opt_.eig_shift;
+      double lam = 1.0 / w[m - 1 - i] + opt_.eig_shift;
       res_.lambda[i] = lam;
       res_.eval[i] = lam < 0 ? -std::sqrt(-lam) : std::sqrt(lam);
-      for (int a = 0; a < m; a++) S[(size_t)a * nev + i] = V[(size_t)a * m + c];
+      for (int a = 0; a < m; a++) S[(size_t)a * nev + i] = V[(size_t)a * nev + i];
     }
     HIPCHK(hipMemcpyAsync(d.small.p, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice, st));
     HIPCHK(d.Phi.alloc((size_t)neq * nev));
@@ -647,7 +697,7 @@ int DeviceReducer::reduce(std::string& err) {
     if (opt_.want_vgi && n1 > 0) {
       SolvePlan* p3 = nullptr;
       if (get_plan(y, d, 3, p3, err)) return -3;
-      HIPCHK(d.W.alloc((size_t)d.wrows * std::max(std::max(n2, 3), opt_.lanczos_block)));
+      if (ensure_work(d, std::max(std::max(n2, 3), opt_.lanczos_block), err)) return -3;
       HIPCHK(d.Zl.alloc((size_t)neq * std::max(3, opt_.lanczos_block)));
       HIPCHK(d.Wl.alloc((size_t)neq * std::max(3, opt_.lanczos_block)));
       run_forward(st, d, *p3, d.Tmp.p, d.Zl.p, n1, true, false);
@@ -677,6 +727,12 @@ int DeviceReducer::reduce(std::string& err) {
   times_.mass = ev_ms(d.ev[5], d.ev[6]);
   times_.gravity = ev_ms(d.ev[6], d.ev[7]);
   times_.total = ev_ms(d.ev[0], d.ev[7]);
+  times_.update_kernel_ms = 0;
+  times_.update_launches = 0;
+  if (opt_.time_updates) {
+    for (int64_t q = 0; q < ui; q++) times_.update_kernel_ms += ev_ms(d.upd_ev[2 * q], d.upd_ev[2 * q + 1]);
+    times_.update_launches = ui;
+  }
   // ---- reduced matrices in the reference layout (CMSTRS, GRAV): first ndof2
   // rows in DOF order through dofPosIn2, then the ngen component modes, then
   // zero padding up to ndim
@@ -780,7 +836,7 @@ int DeviceReducer::solve_kii(int nrhs, const double* R_eq, double* X_eq, std::st
   const int neq = y.neq, n1 = y.ndof1;
   SolvePlan* p = nullptr;
   if (get_plan(y, d, nrhs, p, err)) return -3;
-  HIPCHK(d.W.alloc((size_t)d.wrows * std::max<int64_t>(nrhs, std::max(y.ndof2, opt_.lanczos_block))));
+  if (ensure_work(d, std::max(nrhs, std::max(y.ndof2, opt_.lanczos_block)), err)) return -3;
   DBuf<double> r, t, x;
   std::vector<double> h((size_t)neq * nrhs, 0.0);
   for (int q = 0; q < n1; q++) {

@@ -12,18 +12,20 @@ struct ReduceOptions {
   int ngen = 0;            // component modes kept in the reduced matrices
   int neval = 0;           // eigenpairs computed (max(neval, ngen) when ngen > 0)
   double eig_shift = 0.0;  // eigenshift
-  double tol_eig = 1e-12;  // relative residual for Lanczos convergence
-  int lanczos_block = 8;
-  int lanczos_maxvec = 0;  // 0: min(ndof1, 6*nev + 24)
+  double tol_eig = 1e-11;  // relative residual ||Op phi - theta phi||_M / theta
+  int lanczos_block = 16;
+  int lanczos_maxvec = 0;  // 0: min(ndof1, 10*nev + 40)
   bool want_vgi = false;   // K_ii^{-1} g_i (dispfile)
   bool no_mass = false;
+  bool time_updates = false;  // bracket every trailing-update launch with HIP events
   uint64_t seed = 0x5eed;  // Lanczos start block
   SymbolicOptions sym;
 };
 
 struct PhaseTimes {  // milliseconds, measured with HIP events on the work stream
   double elements = 0, assembly = 0, factor = 0, bsolve = 0, modes = 0, mass = 0, gravity = 0, total = 0;
-  double factor_kernel_ms = 0;  // sum of update-kernel (MFMA) time inside factor
+  double update_kernel_ms = 0;   // sum of trailing-update (MFMA) launches, when timed
+  int64_t update_launches = 0;
 };
 
 struct Results {
@@ -33,6 +35,7 @@ struct Results {
   std::vector<double> lambda;      // nev eigenvalues
   std::vector<double> vgi;         // ndof1 x 3 (meqn1 order)
   int lanczos_vectors = 0;
+  double lanczos_resid = 0;        // worst relative Ritz residual at the last check
   double mass_fe = 0;              // sum of element masses
 };
 

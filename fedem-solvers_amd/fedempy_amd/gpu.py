@@ -35,6 +35,7 @@ def load_library(path=None):
     lib.fgpu_last_error.restype = ctypes.c_char_p
     lib.fgpu_set_part.argtypes = [P, I, P, P, I, P, P, P, P]
     lib.fgpu_analyze.argtypes = [P, I, I, I]
+    lib.fgpu_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_double]
     for name in ("fgpu_get_sizes",):
         getattr(lib, name).argtypes = [P, P]
     lib.fgpu_get_sam.argtypes = [P] + [P] * 6
@@ -51,6 +52,7 @@ def load_library(path=None):
     lib.fgpu_get_element_matrices.argtypes = [P, P, P, P]
     lib.fgpu_get_supernodes.argtypes = [P, P, P, P, P, P]
     lib.fgpu_get_perm.argtypes = [P, P]
+    lib.fgpu_get_lanczos.argtypes = [P, P, P]
     _lib = lib
     return lib
 
@@ -100,6 +102,9 @@ class GpuReducer:
                                            _ptr(melcon), _ptr(mpmnpc), _ptr(mmnpc), _ptr(eprop)),
                     "fgpu_set_part")
 
+    def set_option(self, name, value):
+        self._check(self.lib.fgpu_set_option(self.h, name.encode(), float(value)), "fgpu_set_option")
+
     def analyze(self, ngen=0, neval=0, vgi=False, nomass=False):
         flags = (1 if vgi else 0) | (2 if nomass else 0)
         self._check(self.lib.fgpu_analyze(self.h, ngen, neval, flags), "fgpu_analyze")
@@ -120,6 +125,12 @@ class GpuReducer:
         self._check(self.lib.fgpu_get_sam(self.h, _ptr(madof), _ptr(msc), _ptr(meqn), _ptr(meqn1),
                                           _ptr(meqn2), _ptr(pos2)), "fgpu_get_sam")
         return dict(madof=madof, msc=msc, meqn=meqn, meqn1=meqn1, meqn2=meqn2, dofPosIn2=pos2)
+
+    def lanczos(self):
+        n = ctypes.c_int(0)
+        r = ctypes.c_double(0)
+        self._check(self.lib.fgpu_get_lanczos(self.h, ctypes.byref(n), ctypes.byref(r)), "fgpu_get_lanczos")
+        return n.value, r.value
 
     def supernodes(self):
         inf = self.info()
@@ -179,9 +190,10 @@ class GpuReducer:
         return X
 
     def times(self):
-        t = np.zeros(8)
+        t = np.zeros(10)
         self._check(self.lib.fgpu_get_times(self.h, _ptr(t)), "fgpu_get_times")
-        keys = ("elements", "assembly", "factor", "bsolve", "modes", "mass", "gravity", "total")
+        keys = ("elements", "assembly", "factor", "bsolve", "modes", "mass", "gravity", "total",
+                "update_kernel_ms", "update_launches")
         return dict(zip(keys, t.tolist()))
 
     def info(self):

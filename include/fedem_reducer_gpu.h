@@ -78,6 +78,12 @@ int fgpu_set_part(fgpu_reducer* h, int nnod, const double* xyz, const int* statu
  * 678) and the symbolic analysis, and uploads the part. */
 int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags);
 
+/* Numeric options applied at the next fgpu_analyze: "lanczosBlock",
+ * "lanczosMaxVectors", "lanczosTol" (relative Ritz residual), "eigenshift",
+ * "seed" (Lanczos start block), "leafNodes" (nested-dissection leaf size),
+ * "timeUpdates" (1: time every trailing-update launch with HIP events). */
+int fgpu_set_option(fgpu_reducer* h, const char* name, double value);
+
 /* sizes[0..7] = nnod, ndof, neq, ndof1, ndof2, ndim, nenod, nel */
 int fgpu_get_sizes(const fgpu_reducer* h, int* sizes);
 /* SAM arrays (samModule.f90:27-64): madof(nnod+1), msc(ndof), meqn(ndof),
@@ -105,8 +111,9 @@ int fgpu_get_vgi(const fgpu_reducer* h, double* vgi);
 /* X = K_ii^-1 R on the resident factor, R and X ndof1 x nrhs (MEQN1 rows). */
 int fgpu_solve_internal(fgpu_reducer* h, int nrhs, const double* R, double* X);
 
-/* Diagnostics: times[0..7] (ms) = elements, assembly, factor, B, modes,
- * mass, gravity, total; info[0..9] = nsup, nlevels, front pool doubles,
+/* Diagnostics: times[0..9] (ms) = elements, assembly, factor, B, modes,
+ * mass, gravity, total, summed duration of the trailing-update (MFMA)
+ * launches and their count (option "timeUpdates"); info[0..9] = nsup, nlevels, front pool doubles,
  * nnz(L), factor flops, max front, nnz(K), Lanczos vectors, packed element
  * pool doubles, trailing-update (MFMA) flops. */
 int fgpu_get_times(const fgpu_reducer* h, double* times);
@@ -119,6 +126,9 @@ int fgpu_get_element_matrices(fgpu_reducer* h, double* Kp, double* Mp, int64_t* 
 /* Symbolic structure for host-side verification: per supernode col0, ncol,
  * nrow, parent; rows (nrow each, perm indices). */
 int fgpu_get_supernodes(const fgpu_reducer* h, int* col0, int* ncol, int* nrow, int* parent, int* rows);
+/* Lanczos statistics of the last reduction: basis vectors used and the worst
+ * relative Ritz residual ||K^-1 M phi - theta phi||_M / theta at the end. */
+int fgpu_get_lanczos(const fgpu_reducer* h, int* nvec, double* resid);
 /* perm_to_eq[p] = 0-based equation number at device position p (neq). */
 int fgpu_get_perm(const fgpu_reducer* h, int* perm_to_eq);
 

@@ -1,0 +1,131 @@
+"""ORACLE — test infrastructure only (see oracle/__init__.py).
+
+Nastran bulk-data subset reader for the reference's FE part fixtures
+(src/vpmReducer/vpmReducerTests/FEparts/*.nas, read in the reference by FFl,
+which is absent from the reference tree).  Cards: GRID, CQUAD4, CHEXA, CBAR,
+MAT1, PSHELL, PSOLID, PBAR, ASET, RBE2, in free (comma), small (8-column) and
+large (16-column, '*') field formats.  Returns a part dict (see
+fedempy_amd.parts) plus the RBE2 constraint list.
+"""
+import re
+
+import numpy as np
+
+NPROP = 16
+
+
+def _num(s):
+    s = s.strip()
+    if not s:
+        return None
+    try:
+        return int(s)
+    except ValueError:
+        pass
+    t = s.upper().replace("D", "E")
+    m = re.match(r"^([+-]?[0-9.]+)([+-][0-9]+)$", t)  # Nastran implied exponent 1.5-3
+    if m and "E" not in t:
+        t = m.group(1) + "E" + m.group(2)
+    return float(t)
+
+
+def _cards(text):
+    """Yield (name, [fields]) with continuations merged."""
+    lines = [ln.rstrip("\n") for ln in text.splitlines()]
+    cur = None
+    for ln in lines:
+        if not ln.strip() or ln.startswith("$"):
+            continue
+        head = ln[:8].strip()
+        if ln.startswith("*") or ln.startswith("+"):
+            if cur is None:
+                continue
+            if "," in ln:
+                cur[1].extend(f.strip() for f in ln.split(",")[1:])
+            elif cur[2]:  # large field continuation
+                body = ln[8:72]
+                cur[1].extend(body[i:i + 16] for i in range(0, len(body), 16))
+            else:
+                body = ln[8:72]
+                cur[1].extend(body[i:i + 8] for i in range(0, len(body), 8))
+            continue
+        if cur is not None:
+            yield cur[0], cur[1]
+        if "," in ln:
+            parts = ln.split(",")
+            name = parts[0].strip().upper()
+            cur = [name.rstrip("*"), [p.strip() for p in parts[1:]], name.endswith("*")]
+        elif head.endswith("*"):
+            body = ln[8:72]
+            cur = [head[:-1].upper(), [body[i:i + 16] for i in range(0, len(body), 16)], True]
+        else:
+            body = ln[8:72]
+            cur = [head.upper(), [body[i:i + 8] for i in range(0, len(body), 8)], False]
+    if cur is not None:
+        yield cur[0], cur[1]
+
+
+def read(path):
+    with open(path) as f:
+        text = f.read()
+    grids, elems, mats, props, aset, rbe2 = {}, [], {}, {}, [], []
+    known = {"GRID", "CHEXA", "CQUAD4", "CBAR", "MAT1", "PSHELL", "PSOLID", "PBAR", "ASET", "RBE2"}
+    for name, fl in _cards(text):
+        if name not in known:
+            continue
+        v = [_num(x) for x in fl]
+        if name == "GRID":
+            grids[int(v[0])] = (v[2] or 0.0, v[3] or 0.0, v[4] or 0.0)
+        elif name == "CHEXA":
+            elems.append((44, int(v[0]), int(v[1]), [int(x) for x in v[2:10]], None))
+        elif name == "CQUAD4":
+            elems.append((24, int(v[0]), int(v[1]), [int(x) for x in v[2:6]], None))
+        elif name == "CBAR":
+            elems.append((11, int(v[0]), int(v[1]), [int(x) for x in v[2:4]], (v[4], v[5], v[6])))
+        elif name == "MAT1":
+            mats[int(v[0])] = dict(E=v[1], G=v[2], nu=v[3], rho=v[4] or 0.0)
+        elif name == "PSHELL":
+            props[int(v[0])] = dict(kind="shell", mid=int(v[1]), t=v[2])
+        elif name == "PSOLID":
+            props[int(v[0])] = dict(kind="solid", mid=int(v[1]))
+        elif name == "PBAR":
+            vv = v + [None] * 20
+            props[int(v[0])] = dict(kind="bar", mid=int(v[1]), A=vv[2], I1=vv[3], I2=vv[4], J=vv[5],
+                                    nsm=vv[6] or 0.0, K1=vv[16], K2=vv[17])
+        elif name == "ASET":
+            vals = [x for x in v if x is not None]
+            aset.extend(int(vals[i]) for i in range(0, len(vals), 2))
+        elif name == "RBE2":
+            vals = [x for x in v if x is not None]
+            rbe2.append((int(vals[1]), str(int(vals[2])), [int(x) for x in vals[3:]]))
+    ids = sorted(grids)
+    idx = {g: i for i, g in enumerate(ids)}
+    xyz = np.array([grids[g] for g in ids], float)
+    status = np.zeros(len(ids), np.int32)
+    for g in aset:
+        status[idx[g]] = 1
+    melcon, conn, eprop = [], [], []
+    for t, eid, pid, nodes, orient in elems:
+        p = props[pid]
+        m = mats[p["mid"]]
+        pr = np.zeros(NPROP)
+        pr[0], pr[1], pr[2] = m["E"], m["nu"], m["rho"]
+        pr[3] = m["G"] if m["G"] is not None else m["E"] / (2 * (1 + m["nu"]))
+        if t == 24:
+            pr[4:8] = p["t"]
+        elif t == 11:
+            pr[4], pr[5], pr[6], pr[7] = p["A"], p["I1"], p["I2"], p["J"]
+            pr[8:11] = orient
+            pr[11] = p["K1"] if p["K1"] is not None else 0.0
+            pr[12] = p["K2"] if p["K2"] is not None else 0.0
+            pr[13] = p["nsm"]
+        melcon.append(t)
+        conn.append([idx[n] for n in nodes])
+        eprop.append(pr)
+    sizes = [len(c) for c in conn]
+    part = dict(xyz=xyz, status=status, melcon=np.array(melcon, np.int32),
+                mpmnpc=np.concatenate([[1], 1 + np.cumsum(sizes)]).astype(np.int32),
+                mmnpc=(np.concatenate([np.asarray(c) for c in conn]) + 1).astype(np.int32),
+                eprop=np.array(eprop), node_ids=np.array(ids),
+                rbe2=[(idx[gn], cm, [idx[d] for d in dep]) for gn, cm, dep in rbe2])
+    return part

@@ -79,6 +79,13 @@ def sam_numbering(part):
     madof = np.concatenate([[1], 1 + np.cumsum(nndof)])
     ndof = int(madof[-1] - 1)
     msc = np.zeros(ndof, int)
+    dep = {}
+    for gn, cm, deps in part.get("rbe2", []):
+        for dn in deps:
+            for c in cm:
+                j = int(c) - 1
+                if j < nndof[dn]:
+                    dep[(dn, j)] = gn
     for i in range(nnod):
         for j in range(nndof[i]):
             st = int(status[i])
@@ -88,8 +95,10 @@ def sam_numbering(part):
                 m = 2
             else:
                 m = 1
-            if m == 1 and j >= maxdn[i]:
+            if m == 1 and j >= maxdn[i] and not any(i == g for g, _, _ in part.get("rbe2", [])):
                 m = 0
+            if (i, j) in dep:
+                m = -1  # dependent DOF of a rigid element: no equation
             msc[madof[i] - 1 + j] = m
     meqn = np.zeros(ndof, int)
     n1 = int(np.sum(msc == 1))
@@ -105,7 +114,36 @@ def sam_numbering(part):
     dofpos2 = pos2[meqn[msc == 2]]
     ndim_ext = int(np.sum(nndof[status == 1]))
     return dict(madof=madof, msc=msc, meqn=meqn, meqn1=meqn1, meqn2=meqn2, dofPosIn2=dofpos2,
-                nndof=nndof, neq=neq, ndof1=len(meqn1), ndof2=len(meqn2), ndim_ext=ndim_ext)
+                nndof=nndof, neq=neq, ndof1=len(meqn1), ndof2=len(meqn2), ndim_ext=ndim_ext, dep=dep)
+
+
+def constraint_T(part, s):
+    """T (ndof x neq): u_dof = T u_eq.  Free DOFs map to their equation,
+    fixed DOFs to zero, dependent DOFs of RBE2 elements to the rigid-body
+    motion of the independent node: u_d = u_i + theta_i x (x_d - x_i) for
+    translations, theta_d = theta_i for rotations."""
+    rows, cols, vals = [], [], []
+    madof, meqn = s["madof"], s["meqn"]
+    xyz = part["xyz"]
+    for d in range(len(s["msc"])):
+        if s["msc"][d] > 0:
+            rows.append(d); cols.append(meqn[d] - 1); vals.append(1.0)
+    for (dn, j), gn in s["dep"].items():
+        d = madof[dn] - 1 + j
+        eqi = [meqn[madof[gn] - 1 + k] - 1 for k in range(6)]
+        r = xyz[dn] - xyz[gn]
+        if j < 3:
+            coef = {j: 1.0}
+            # theta x r: component j
+            if j == 0: coef.update({4: r[2], 5: -r[1]})
+            if j == 1: coef.update({5: r[0], 3: -r[2]})
+            if j == 2: coef.update({3: r[1], 4: -r[0]})
+        else:
+            coef = {j: 1.0}
+        for k, c in coef.items():
+            if eqi[k] >= 0 and c != 0.0:
+                rows.append(d); cols.append(eqi[k]); vals.append(c)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(len(s["msc"]), s["neq"]))
 
 
 def element_matrices(part, e):
@@ -121,28 +159,39 @@ def element_matrices(part, e):
 
 
 def assemble(part, s):
-    """Global K, M (CSR, 0-based equation order) and the element masses."""
+    """Global K, M (CSR, 0-based equation order) and the element masses:
+    element matrices are added in DOF space, then K = T^T K_dof T with the
+    constraint transformation T (identity on the free DOFs)."""
     rows, cols, kv, mv = [], [], [], []
     emass = 0.0
     for e in range(len(part["melcon"])):
         nodes, ndn, K, M = element_matrices(part, e)
-        eq = np.full(len(nodes) * ndn, -1)
+        dof = np.full(len(nodes) * ndn, -1)
         for a, n in enumerate(nodes):
             for j in range(ndn):
                 if j < s["nndof"][n]:
-                    eq[a * ndn + j] = s["meqn"][s["madof"][n] - 1 + j] - 1
-        ok = eq >= 0
-        ii, jj = np.meshgrid(eq[ok], eq[ok], indexing="ij")
+                    dof[a * ndn + j] = s["madof"][n] - 1 + j
+        ok = dof >= 0
+        ii, jj = np.meshgrid(dof[ok], dof[ok], indexing="ij")
         rows.append(ii.ravel())
         cols.append(jj.ravel())
         kv.append(K[np.ix_(ok, ok)].ravel())
         mv.append(M[np.ix_(ok, ok)].ravel())
-        emass += M[0::3, :].sum() if ndn == 3 else 0.0
-    n = s["neq"]
+        emass += element_mass(part, e, M)
+    nd = len(s["msc"])
     r, c = np.concatenate(rows), np.concatenate(cols)
-    Kg = sp.csr_matrix((np.concatenate(kv), (r, c)), shape=(n, n))
-    Mg = sp.csr_matrix((np.concatenate(mv), (r, c)), shape=(n, n))
-    return Kg, Mg, emass
+    Kd = sp.csr_matrix((np.concatenate(kv), (r, c)), shape=(nd, nd))
+    Md = sp.csr_matrix((np.concatenate(mv), (r, c)), shape=(nd, nd))
+    T = constraint_T(part, s)
+    return (T.T @ Kd @ T).tocsr(), (T.T @ Md @ T).tocsr(), emass
+
+
+def element_mass(part, e, M):
+    """EMASS of the element (inadd.f90: lumpSolidMass for solids)."""
+    t = int(part["melcon"][e])
+    if t == 44:
+        return float(M[:, 0::3].sum())
+    return 0.0
 
 
 def gravity_dirs(part, s):
@@ -166,13 +215,17 @@ def fix_signs(phi, eq_order=None):
     return phi
 
 
-def modes(Kii, Mii, nev, dense_limit=3000):
-    """Lowest nev eigenpairs of Kii phi = lambda Mii phi, M-orthonormal."""
+def modes(Kii, Mii, nev, dense_limit=3000, solve=None):
+    """Lowest nev eigenpairs of Kii phi = lambda Mii phi, M-orthonormal
+    (shift-invert about 0, reusing the factor of Kii when given)."""
     n = Kii.shape[0]
     if n <= dense_limit:
         w, V = sla.eigh(Kii.toarray(), Mii.toarray(), subset_by_index=[0, nev - 1])
     else:
-        w, V = spla.eigsh(Kii.tocsc(), k=nev, M=Mii.tocsc(), sigma=0.0, which="LM", tol=0)
+        opinv = None
+        if solve is not None:
+            opinv = spla.LinearOperator(Kii.shape, matvec=lambda x: solve(x), dtype=np.float64)
+        w, V = spla.eigsh(Kii.tocsc(), k=nev, M=Mii.tocsc(), sigma=0.0, which="LM", tol=0, OPinv=opinv)
         o = np.argsort(w)
         w, V = w[o], V[:, o]
         nrm = np.sqrt(np.einsum("ij,ij->j", V, Mii @ V))
@@ -201,7 +254,7 @@ def reduce(part, ngen=0, neval=0, want_vgi=False, dense_limit=3000):
     nev = max(neval, ngen) if ngen > 0 else 0
     lam = phi = None
     if nev > 0:
-        lam, phi = modes(Kii, Mii, nev, dense_limit)
+        lam, phi = modes(Kii, Mii, nev, dense_limit, solve)
     MB = Mii @ B
     M22 = B.T @ MB + Mib.T @ B + B.T @ Mib + Mbb
     M22 = 0.5 * (M22 + M22.T)

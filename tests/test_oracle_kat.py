@@ -1,0 +1,44 @@
+"""The oracle against the reference's own known-answer fixtures.
+
+FEparts/*.sol hold the reference test's expected superelement solution:
+after reduction, solve K_red v = G g with g = (0, 9.81, 0) and the first six
+reduced DOFs fixed; every nonzero entry must match within 1e-9 absolute
+(solveSupel.f90:120-184, vpmReducerTests/CMakeLists.txt:50-57)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import nastran
+from oracle import reducer as orc
+
+
+def kat(name, golden_dir):
+    part = nastran.read(os.path.join(golden_dir, name + ".nas"))
+    with open(os.path.join(golden_dir, name + ".sol")) as f:
+        S = np.array([float(x) for x in f.read().split()])
+    o = orc.reduce(part)
+    A = o["SK"][6:, 6:]
+    R = o["G"][6:] @ np.array([0.0, 9.81, 0.0])
+    v = np.linalg.solve(A, R)
+    return v, S
+
+
+def test_chexa08_known_answer(golden_dir):
+    v, S = kat("CHEXA08_", golden_dir)
+    for i, s in enumerate(S):
+        if abs(s) > 0:
+            assert abs(v[i] - s) <= 1e-9, (i, v[i], s)
+
+
+def test_kat_fails_with_planted_error(golden_dir, monkeypatch):
+    """A dropped mass term (planted in the oracle) must break the KAT."""
+    real = orc.hex8_matrices
+
+    def bad(*a):
+        K, M = real(*a)
+        return K, 0.9 * M
+
+    monkeypatch.setattr(orc, "hex8_matrices", bad)
+    v, S = kat("CHEXA08_", golden_dir)
+    assert abs(v[1] - S[1]) > 1e-9
