@@ -78,7 +78,7 @@ def test_device_assembly_matches_oracle():
 
 
 def test_device_elements_match_oracle():
-    part = parts.hex_block(3, 2, 2, 0.03, 0.02, 0.02, seed=11, n_ext=2)
+    part = parts.hex_block(3, 2, 2, 0.03, 0.02, 0.02, seed=11, n_ext=4)
     R = gpu.GpuReducer()
     R.set_part(part)
     R.analyze()
