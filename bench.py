@@ -49,6 +49,19 @@ print(json.dumps({"n": n, "s": time.perf_counter() - t}))
     return r["n"] / r["s"]
 
 
+def measured_traffic():
+    """HBM bytes per k_update launch (FETCH_SIZE + WRITE_SIZE, copy-kernel
+    calibrated) from the committed PMC summary, or None without one."""
+    path = os.path.join(ROOT, "profiles", "k_update_hex8_48x24x24_traffic.md")
+    try:
+        for ln in open(path):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,11 +69,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one plain device copy (torch.neg over N MiB of FP64) whose HBM "
+                         "counters calibrate FETCH_SIZE/WRITE_SIZE of the profiled kernels")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this reducer runs on one GPU; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         sys.exit(2)
     from fedempy_amd import gpu, parts
+
+    if a.calib_copy_mib > 0:
+        import torch
+        src = torch.ones(a.calib_copy_mib * 2**20 // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     part = parts.config1(seed=a.seed)
     R = gpu.GpuReducer()
@@ -105,7 +129,7 @@ def main():
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "mfma", "kernel": "k_update (trailing LDL^T update, FP64 MFMA)",
                      "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                     "frac": achieved / PEAK_FP64_MFMA, "traffic": measured_traffic(),
                      "launches": upd_n, "avg_launch_us": 1e3 * upd_ms / upd_n,
                      "flops_per_launch": upd_flops / upd_n},
         "phases_ms": {k: round(v, 3) for k, v in phases.items() if k not in ("update_kernel_ms", "update_launches")},

@@ -1,0 +1,230 @@
+// Nastran bulk-data subset reader for the part input (-linkfile).  The
+// reference parses parts with FFl (absent fedem-foundation); this reader
+// covers the cards of the element types on the device path: GRID, CHEXA,
+// CQUAD4, CBAR, MAT1, PSHELL, PSOLID, PBAR, ASET/ASET1, SPC1 and RBE2, in
+// free (comma), small (8 column) and large (16 column, '*') field formats.
+// Nodes are numbered in ascending GRID id order, as SAM numbers them.
+#include "nastran.hpp"
+#include <algorithm>
+#include <cctype>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <sstream>
+
+namespace fgpu {
+namespace {
+
+std::string trim(const std::string& s) {
+  size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+// Nastran real: 1.5E-3, 1.5D-3 or the implied-exponent form 1.5-3
+bool to_double(std::string s, double& v) {
+  s = trim(s);
+  if (s.empty()) return false;
+  for (char& c : s) c = (char)std::toupper(c);
+  std::replace(s.begin(), s.end(), 'D', 'E');
+  if (s.find('E') == std::string::npos) {
+    size_t p = s.find_last_of("+-");
+    if (p != std::string::npos && p > 0) s.insert(p, "E");
+  }
+  char* end = nullptr;
+  v = std::strtod(s.c_str(), &end);
+  return end && *end == 0;
+}
+
+struct Card {
+  std::string name;
+  std::vector<std::string> f;
+};
+
+std::vector<Card> read_cards(std::istream& in) {
+  std::vector<Card> cards;
+  std::string ln;
+  bool large = false;
+  while (std::getline(in, ln)) {
+    if (!ln.empty() && ln.back() == '\r') ln.pop_back();
+    if (trim(ln).empty() || ln[0] == '$') continue;
+    auto split_fixed = [](const std::string& l, int w) {
+      std::vector<std::string> r;
+      std::string body = l.size() > 8 ? l.substr(8, std::min<size_t>(l.size(), 72) - 8) : "";
+      for (size_t i = 0; i < body.size(); i += w) r.push_back(trim(body.substr(i, w)));
+      return r;
+    };
+    if (ln[0] == '*' || ln[0] == '+') {
+      if (cards.empty()) continue;
+      std::vector<std::string> add;
+      if (ln.find(',') != std::string::npos) {
+        std::stringstream ss(ln);
+        std::string t;
+        std::getline(ss, t, ',');
+        while (std::getline(ss, t, ',')) add.push_back(trim(t));
+      } else {
+        add = split_fixed(ln, large ? 16 : 8);
+      }
+      cards.back().f.insert(cards.back().f.end(), add.begin(), add.end());
+      continue;
+    }
+    Card c;
+    if (ln.find(',') != std::string::npos) {
+      std::stringstream ss(ln);
+      std::string t;
+      std::getline(ss, t, ',');
+      c.name = trim(t);
+      while (std::getline(ss, t, ',')) c.f.push_back(trim(t));
+      large = !c.name.empty() && c.name.back() == '*';
+    } else {
+      c.name = trim(ln.substr(0, std::min<size_t>(8, ln.size())));
+      large = !c.name.empty() && c.name.back() == '*';
+      c.f = split_fixed(ln, large ? 16 : 8);
+    }
+    if (!c.name.empty() && c.name.back() == '*') c.name.pop_back();
+    for (char& ch : c.name) ch = (char)std::toupper(ch);
+    cards.push_back(c);
+  }
+  return cards;
+}
+
+}  // namespace
+
+int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
+  std::ifstream in(path);
+  if (!in) { err = "cannot open link file " + path; return -1; }
+  std::vector<Card> cards = read_cards(in);
+  struct Grid { double x[3]; };
+  struct Elem { int type, eid, pid; std::vector<int> nodes; double orient[3]; };
+  struct Mat { double E = 0, G = 0, nu = 0, rho = 0; bool hasG = false; };
+  struct Prop { int kind = 0, mid = 0; double t = 0, A = 0, I1 = 0, I2 = 0, J = 0, nsm = 0, K1 = 0, K2 = 0; };
+  std::map<int, Grid> grids;
+  std::vector<Elem> elems;
+  std::map<int, Mat> mats;
+  std::map<int, Prop> props;
+  std::vector<int> aset;
+  std::map<int, int> spc;  // grid -> fixed-DOF bitmask
+  std::vector<std::pair<int, std::pair<std::string, std::vector<int>>>> rbe2;
+  auto I = [](const std::vector<std::string>& f, size_t k) -> int {
+    return k < f.size() && !f[k].empty() ? std::atoi(f[k].c_str()) : 0;
+  };
+  auto D = [](const std::vector<std::string>& f, size_t k, double def = 0.0) -> double {
+    double v;
+    return (k < f.size() && to_double(f[k], v)) ? v : def;
+  };
+  for (const Card& c : cards) {
+    const auto& f = c.f;
+    if (c.name == "GRID") {
+      Grid g{{D(f, 2), D(f, 3), D(f, 4)}};
+      grids[I(f, 0)] = g;
+    } else if (c.name == "CHEXA" || c.name == "CQUAD4" || c.name == "CBAR") {
+      Elem e;
+      e.type = c.name == "CHEXA" ? HEX8 : c.name == "CQUAD4" ? ANDES4 : BEAM2;
+      e.eid = I(f, 0);
+      e.pid = I(f, 1);
+      int nn = elem_nodes(e.type);
+      for (int k = 0; k < nn; k++) e.nodes.push_back(I(f, 2 + k));
+      e.orient[0] = e.orient[1] = e.orient[2] = 0.0;
+      if (e.type == BEAM2)
+        for (int k = 0; k < 3; k++) e.orient[k] = D(f, 4 + k);
+      elems.push_back(e);
+    } else if (c.name == "MAT1") {
+      Mat m;
+      m.E = D(f, 1);
+      double g;
+      m.hasG = f.size() > 2 && to_double(f[2], g);
+      m.G = m.hasG ? g : 0.0;
+      m.nu = D(f, 3);
+      m.rho = D(f, 4);
+      mats[I(f, 0)] = m;
+    } else if (c.name == "PSHELL") {
+      Prop p;
+      p.kind = 1; p.mid = I(f, 1); p.t = D(f, 2);
+      props[I(f, 0)] = p;
+    } else if (c.name == "PSOLID") {
+      Prop p;
+      p.kind = 2; p.mid = I(f, 1);
+      props[I(f, 0)] = p;
+    } else if (c.name == "PBAR") {
+      Prop p;
+      p.kind = 3; p.mid = I(f, 1); p.A = D(f, 2); p.I1 = D(f, 3); p.I2 = D(f, 4); p.J = D(f, 5); p.nsm = D(f, 6);
+      p.K1 = D(f, 16); p.K2 = D(f, 17);
+      props[I(f, 0)] = p;
+    } else if (c.name == "ASET") {
+      for (size_t k = 0; k + 1 < f.size() + 1; k += 2)
+        if (I(f, k)) aset.push_back(I(f, k));
+    } else if (c.name == "ASET1") {
+      for (size_t k = 1; k < f.size(); k++)
+        if (I(f, k)) aset.push_back(I(f, k));
+    } else if (c.name == "SPC1") {
+      int mask = 0;
+      for (char ch : f.size() > 1 ? f[1] : std::string())
+        if (ch >= '1' && ch <= '6') mask |= 1 << (ch - '1');
+      for (size_t k = 2; k < f.size(); k++)
+        if (I(f, k)) spc[I(f, k)] |= mask;
+    } else if (c.name == "RBE2") {
+      std::vector<int> dep;
+      for (size_t k = 3; k < f.size(); k++)
+        if (I(f, k)) dep.push_back(I(f, k));
+      rbe2.push_back({I(f, 1), {f.size() > 2 ? f[2] : std::string(), dep}});
+    }
+  }
+  NastranPart& np = out;
+  np = NastranPart();
+  Part& p = np.part;
+  std::map<int, int> idx;
+  for (auto& g : grids) {
+    idx[g.first] = p.nnod++;
+    np.node_ids.push_back(g.first);
+    for (int k = 0; k < 3; k++) p.xyz.push_back(g.second.x[k]);
+  }
+  p.status.assign(p.nnod, 0);
+  for (auto& s : spc)
+    if (idx.count(s.first)) p.status[idx[s.first]] = -s.second;
+  for (int g : aset) {
+    if (!idx.count(g)) { err = "ASET grid " + std::to_string(g) + " not defined"; return -2; }
+    p.status[idx[g]] = 1;
+  }
+  p.eptr.push_back(0);
+  for (auto& e : elems) {
+    if (!props.count(e.pid)) { err = "element " + std::to_string(e.eid) + ": missing property"; return -2; }
+    const Prop& pr = props[e.pid];
+    if (!mats.count(pr.mid)) { err = "element " + std::to_string(e.eid) + ": missing material"; return -2; }
+    const Mat& m = mats[pr.mid];
+    p.etype.push_back(e.type);
+    for (int n : e.nodes) {
+      if (!idx.count(n)) { err = "element " + std::to_string(e.eid) + ": undefined grid"; return -2; }
+      p.enodes.push_back(idx[n]);
+    }
+    p.eptr.push_back((int)p.enodes.size());
+    std::vector<double> ep(NPROP, 0.0);
+    ep[P_E] = m.E;
+    ep[P_NU] = m.nu;
+    ep[P_RHO] = m.rho;
+    ep[P_G] = m.hasG ? m.G : m.E / (2.0 * (1.0 + m.nu));
+    if (e.type == ANDES4)
+      for (int k = 0; k < 4; k++) ep[P_T1 + k] = pr.t;
+    if (e.type == BEAM2) {
+      ep[P_A] = pr.A; ep[P_IY] = pr.I1; ep[P_IZ] = pr.I2; ep[P_IT] = pr.J;
+      ep[P_DX] = e.orient[0]; ep[P_DY] = e.orient[1]; ep[P_DZ] = e.orient[2];
+      ep[P_KY] = pr.K1; ep[P_KZ] = pr.K2; ep[P_NSM] = pr.nsm;
+    }
+    p.eprop.insert(p.eprop.end(), ep.begin(), ep.end());
+    p.nel++;
+  }
+  for (auto& r : rbe2) {
+    Rbe2 c;
+    if (!idx.count(r.first)) { err = "RBE2: undefined independent grid"; return -2; }
+    c.indep = idx[r.first];
+    c.dofs = 0;
+    for (char ch : r.second.first)
+      if (ch >= '1' && ch <= '6') c.dofs |= 1 << (ch - '1');
+    for (int g : r.second.second) {
+      if (!idx.count(g)) { err = "RBE2: undefined dependent grid"; return -2; }
+      c.dep.push_back(idx[g]);
+    }
+    np.rbe2.push_back(c);
+  }
+  return 0;
+}
+
+}  // namespace fgpu
