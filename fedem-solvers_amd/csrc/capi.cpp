@@ -1,5 +1,7 @@
 // C ABI of the plug-in seam (include/fedem_reducer_gpu.h, part 2).
+#include <algorithm>
 #include <cstring>
+#include <vector>
 #include <exception>
 #include <new>
 #include "../../include/fedem_reducer_gpu.h"
@@ -186,6 +188,26 @@ int fgpu_solve_internal(fgpu_reducer* h, int nrhs, const double* R, double* X) {
   GUARD(h)
   if (!h->reduced || nrhs <= 0 || !R || !X) return -1;
   return h->red.solve_kii(nrhs, R, X, h->err);
+  END_GUARD(h)
+}
+
+int fgpu_get_gravity_internal(fgpu_reducer* h, double* gi) {
+  GUARD(h)
+  if (!h->reduced || !gi) return -1;
+  std::vector<double> g;
+  int rc = h->red.gravity_internal(g, h->err);
+  if (rc == 0) std::copy(g.begin(), g.end(), gi);
+  return rc;
+  END_GUARD(h)
+}
+
+int fgpu_reduce_loads(fgpu_reducer* h, int nrhs, const double* F, double* Fred) {
+  GUARD(h)
+  if (!h->reduced || nrhs <= 0 || !F || !Fred) return -1;
+  std::vector<double> r;
+  int rc = h->red.reduce_loads(nrhs, F, r, h->err);
+  if (rc == 0) std::copy(r.begin(), r.end(), Fred);
+  return rc;
   END_GUARD(h)
 }
 

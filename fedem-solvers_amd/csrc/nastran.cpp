@@ -1,7 +1,8 @@
 // Nastran bulk-data subset reader for the part input (-linkfile).  The
 // reference parses parts with FFl (absent fedem-foundation); this reader
 // covers the cards of the element types on the device path: GRID, CHEXA,
-// CQUAD4, CBAR, MAT1, PSHELL, PSOLID, PBAR, ASET/ASET1, SPC1 and RBE2, in
+// CQUAD4, CBAR, MAT1, PSHELL, PSOLID, PBAR, ASET/ASET1, SPC1, RBE2 and the
+// FORCE/MOMENT nodal loads (basic coordinate system only), in
 // free (comma), small (8 column) and large (16 column, '*') field formats.
 // Nodes are numbered in ascending GRID id order, as SAM numbers them.
 #include "nastran.hpp"
@@ -104,6 +105,8 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
   std::vector<int> aset;
   std::map<int, int> spc;  // grid -> fixed-DOF bitmask
   std::vector<std::pair<int, std::pair<std::string, std::vector<int>>>> rbe2;
+  struct Load { int sid, grid, dof0; double v[3]; };
+  std::vector<Load> loads;
   auto I = [](const std::vector<std::string>& f, size_t k) -> int {
     return k < f.size() && !f[k].empty() ? std::atoi(f[k].c_str()) : 0;
   };
@@ -161,6 +164,10 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
         if (ch >= '1' && ch <= '6') mask |= 1 << (ch - '1');
       for (size_t k = 2; k < f.size(); k++)
         if (I(f, k)) spc[I(f, k)] |= mask;
+    } else if (c.name == "FORCE" || c.name == "MOMENT") {
+      if (I(f, 2) != 0) { err = c.name + ": only the basic coordinate system (CID 0) is supported"; return -2; }
+      double F = D(f, 3);
+      loads.push_back({I(f, 0), I(f, 1), c.name == "FORCE" ? 0 : 3, {F * D(f, 4), F * D(f, 5), F * D(f, 6)}});
     } else if (c.name == "RBE2") {
       std::vector<int> dep;
       for (size_t k = 3; k < f.size(); k++)
@@ -223,6 +230,11 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
       c.dep.push_back(idx[g]);
     }
     np.rbe2.push_back(c);
+  }
+  for (auto& l : loads) {
+    if (!idx.count(l.grid)) { err = "load on undefined grid " + std::to_string(l.grid); return -2; }
+    for (int k = 0; k < 3; k++)
+      if (l.v[k] != 0.0) np.loads.push_back({l.sid, idx[l.grid], l.dof0 + k, l.v[k]});
   }
   return 0;
 }

@@ -13,10 +13,19 @@ struct Rbe2 {
   std::vector<int> dep;
 };
 
+// Nodal load of a load case (FORCE: DOFs 0-2, MOMENT: DOFs 3-5)
+struct NodalLoad {
+  int lc = 0;    // load set id
+  int node = 0;  // 0-based node index
+  int dof = 0;   // 0..5
+  double value = 0.0;
+};
+
 struct NastranPart {
   Part part;
   std::vector<int> node_ids;
   std::vector<Rbe2> rbe2;
+  std::vector<NodalLoad> loads;
 };
 
 int read_nastran(const std::string& path, NastranPart& out, std::string& err);

@@ -858,4 +858,59 @@ int DeviceReducer::solve_kii(int nrhs, const double* R_eq, double* X_eq, std::st
   return 0;
 }
 
+int DeviceReducer::gravity_internal(std::vector<double>& gi, std::string& err) {
+  Impl& d = *d_;
+  const Symbolic& y = sym_;
+  const int neq = y.neq, n1 = y.ndof1;
+  if (opt_.no_mass) { err = "gravity forces need the mass matrix"; return -1; }
+  DBuf<double> g;
+  HIPCHK(g.alloc((size_t)neq * 3));
+  launch_spmm(st_, neq, 3, d.rowptr.p, d.colind.p, d.Mv.p, d.grav.p, 3, g.p, 3);
+  std::vector<double> h((size_t)neq * 3);
+  HIPCHK(hipMemcpyAsync(h.data(), g.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st_));
+  HIPCHK(hipStreamSynchronize(st_));
+  gi.assign((size_t)n1 * 3, 0.0);
+  for (int q = 0; q < n1; q++) {
+    int rr = y.eq_to_perm[sam_.meqn1[q] - 1];
+    for (int c = 0; c < 3; c++) gi[q + (size_t)c * n1] = h[(size_t)rr * 3 + c];
+  }
+  return 0;
+}
+
+int DeviceReducer::reduce_loads(int nrhs, const double* F_eq, std::vector<double>& out, std::string& err) {
+  Impl& d = *d_;
+  const Symbolic& y = sym_;
+  const int neq = y.neq, n2 = y.ndof2, ngen = res_.ngen, nev = res_.nev, nd = res_.ndim;
+  out.assign((size_t)nd * std::max(nrhs, 0), 0.0);
+  if (nrhs <= 0) return 0;
+  std::vector<double> h((size_t)neq * nrhs);
+  for (int e = 0; e < neq; e++) {
+    int rr = y.eq_to_perm[e];
+    for (int c = 0; c < nrhs; c++) h[(size_t)rr * nrhs + c] = F_eq[e + (size_t)c * neq];
+  }
+  DBuf<double> f;
+  HIPCHK(f.upload(h));
+  const int maxab = std::max(std::max(n2, ngen), nrhs);
+  HIPCHK(d.part.alloc((size_t)xty_chunks(neq) * maxab * maxab));
+  HIPCHK(d.acc.alloc((size_t)maxab * maxab));
+  std::vector<double> r2((size_t)n2 * nrhs), r1((size_t)ngen * nrhs);
+  if (n2 > 0) {  // rhs2 = f_e + B^T f_i = [B; I]^T f
+    launch_xty(st_, neq, n2, nrhs, d.X.p, n2, f.p, nrhs, d.part.p, d.acc.p);
+    HIPCHK(hipMemcpyAsync(r2.data(), d.acc.p, sizeof(double) * r2.size(), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+  }
+  if (ngen > 0) {  // rhs1 = Phi^T f_i
+    launch_xty(st_, neq, ngen, nrhs, d.Phi.p, nev, f.p, nrhs, d.part.p, d.acc.p);
+    HIPCHK(hipMemcpyAsync(r1.data(), d.acc.p, sizeof(double) * r1.size(), hipMemcpyDeviceToHost, st_));
+    HIPCHK(hipStreamSynchronize(st_));
+  }
+  const std::vector<int>& pos2 = sam_.dofPosIn2;
+  for (int c = 0; c < nrhs; c++) {
+    for (int i = 0; i < n2; i++) out[i + (size_t)c * nd] = r2[(size_t)(pos2[i] - 1) * nrhs + c];
+    for (int a = 0; a < ngen; a++) out[(n2 + a) + (size_t)c * nd] = r1[(size_t)a * nrhs + c];
+  }
+  (void)err;
+  return 0;
+}
+
 }  // namespace fgpu

@@ -59,6 +59,11 @@ class DeviceReducer {
   int get_element_matrices(std::vector<double>& Kp, std::vector<double>& Mp);
   // Extra solves on the resident factor (load cases / recovery)
   int solve_kii(int nrhs, const double* R_eq, double* X_eq, std::string& err);  // MEQN1 rows
+  // g_i = (M a) on the internal equations for unit accelerations along x, y, z
+  int gravity_internal(std::vector<double>& gi, std::string& err);  // ndof1 x 3, MEQN1 rows
+  // reduced load vectors [f_e + B^T f_i (dofPosIn2 order); Phi^T f_i], ndim x nrhs,
+  // from full load vectors F_eq (neq x nrhs, equation order)
+  int reduce_loads(int nrhs, const double* F_eq, std::vector<double>& out, std::string& err);
 
   const Sam& sam() const { return sam_; }
   const Symbolic& sym() const { return sym_; }

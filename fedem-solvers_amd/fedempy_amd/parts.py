@@ -67,3 +67,48 @@ def config1(seed=0):
     """48x24x24 HEX8 steel block, 0.48x0.24x0.24 m, +-5 % jitter, 12 external
     nodes on the x=0 and x=max faces (72 boundary DOFs), 20 component modes."""
     return hex_block(48, 24, 24, 0.48, 0.24, 0.24, jitter=0.05, seed=seed, n_ext=12)
+
+
+def write_nastran(part, path, loads=None):
+    """Writes a part as Nastran bulk data in free-field format with every
+    real at full double precision (GRID, CHEXA/CQUAD4/CBAR, MAT1,
+    PSOLID/PSHELL/PBAR, ASET1 for the external nodes, SPC1 for fixed DOFs,
+    FORCE for nodal loads), so that the part read back is bit-identical.
+    Node ids are 1-based node numbers; one material and one property card
+    per element.
+    loads: iterable of (load set id, node number 1-based, (fx, fy, fz))."""
+    xyz, status = part["xyz"], part["status"]
+    mel, mp, mm, pr = part["melcon"], part["mpmnpc"], part["mmnpc"], part["eprop"]
+    out = ["BEGIN BULK"]
+    r = lambda v: repr(float(v))
+    for n, (x, y, z) in enumerate(xyz):
+        out.append("GRID,%d,,%s,%s,%s" % (n + 1, r(x), r(y), r(z)))
+    name = {44: "CHEXA", 24: "CQUAD4", 11: "CBAR"}
+    for e in range(len(mel)):
+        nodes = [str(int(v)) for v in mm[mp[e] - 1:mp[e + 1] - 1]]
+        t = int(mel[e])
+        fields = [str(e + 1), str(e + 1)] + nodes
+        if t == 11:
+            fields += [r(v) for v in pr[e, 8:11]]
+        out.append(name[t] + "," + ",".join(fields))
+        E, nu, rho = pr[e, 0], pr[e, 1], pr[e, 2]
+        G = pr[e, 3] if pr[e, 3] else E / (2 * (1 + nu))
+        out.append("MAT1,%d,%s,%s,%s,%s" % (e + 1, r(E), r(G), r(nu), r(rho)))
+        if t == 44:
+            out.append("PSOLID,%d,%d" % (e + 1, e + 1))
+        elif t == 24:
+            out.append("PSHELL,%d,%d,%s" % (e + 1, e + 1, r(pr[e, 4])))
+        else:
+            out.append("PBAR,%d,%d,%s" % (e + 1, e + 1, ",".join(r(v) for v in (*pr[e, 4:8], pr[e, 13]))))
+    ext = [i + 1 for i in range(len(status)) if status[i] == 1]
+    for k in range(0, len(ext), 6):
+        out.append("ASET1,123456," + ",".join(str(v) for v in ext[k:k + 6]))
+    for i in range(len(status)):
+        if status[i] < 0:
+            comp = "".join(str(j + 1) for j in range(6) if (-int(status[i]) >> j) & 1)
+            out.append("SPC1,1,%s,%d" % (comp, i + 1))
+    for sid, node, f in loads or []:
+        out.append("FORCE,%d,%d,0,1.0,%s,%s,%s" % (sid, node, *(r(v) for v in f)))
+    out.append("ENDDATA")
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")

