@@ -8,6 +8,9 @@ namespace fgpu {
 void launch_hex8(hipStream_t st, int nhex, const int* elist, const int* eptr, const int* enodes,
                  const double* xyz, const double* eprop, const int64_t* eoff, double* Kp, double* Mp,
                  int* ebad);
+// ANDES4 shells (elem_shell.hip): P^T K P stiffness and FQS35 lumped mass
+void launch_andes4(hipStream_t st, int nel, const int* elist, const int* eptr, const int* enodes, const double* xyz,
+                   const double* eprop, const int64_t* eoff, double* Kp, double* Mp, int* ebad);
 void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* Kp,
                      const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool);
 void launch_fill(hipStream_t st, double* p, int64_t n, double v);

@@ -76,7 +76,7 @@ struct SolvePlan {
 struct DeviceReducer::Impl {
   // part
   DBuf<double> xyz, eprop;
-  DBuf<int> eptr, enodes, hexlist;
+  DBuf<int> eptr, enodes, hexlist, shlist, bmlist;
   DBuf<int64_t> eoff;
   // structure
   DBuf<int> col0, ncol, nrow, rptr, rows, chptr, chlist, relptr, rel, rowptr, colind, cptr;
@@ -91,7 +91,7 @@ struct DeviceReducer::Impl {
   DBuf<double> X, Z, Y, part, small, acc, grav;
   // component modes
   DBuf<double> Q, P, Wl, Zl, Yl, Cc, Phi, Tmp;
-  std::vector<int> hexl;
+  std::vector<int> hexl, shl, bml;  // elements by type: HEX8, ANDES4, BEAM2
   int64_t wrows = 0;
   hipEvent_t ev[12]{};
   std::vector<hipEvent_t> upd_ev;  // start/stop pairs of the trailing-update launches
@@ -253,8 +253,11 @@ int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& e
   if (build_sam(part_, sam_, err)) return -1;
   if (build_symbolic(part_, sam_, opt_.sym, sym_, err)) return -1;
   d_->hexl.clear();
+  d_->shl.clear();
+  d_->bml.clear();
   for (int e = 0; e < part_.nel; e++) {
     if (part_.etype[e] == HEX8) d_->hexl.push_back(e);
+    else if (part_.etype[e] == ANDES4) d_->shl.push_back(e);
     else {
       err = "element type " + std::to_string(part_.etype[e]) + " not supported on the device path yet";
       return -2;
@@ -276,6 +279,8 @@ int DeviceReducer::upload(std::string& err) {
   HIPCHK(d.eptr.upload(part_.eptr));
   HIPCHK(d.enodes.upload(part_.enodes));
   HIPCHK(d.hexlist.upload(d.hexl));
+  HIPCHK(d.shlist.upload(d.shl));
+  HIPCHK(d.bmlist.upload(d.bml));
   HIPCHK(d.eoff.upload(sym_.elem_off));
   const Symbolic& y = sym_;
   HIPCHK(d.col0.upload(y.col0));
@@ -423,6 +428,8 @@ int DeviceReducer::reduce(std::string& err) {
   HIPCHK(hipMemsetAsync(d.ebad.p, 0, sizeof(int) * part_.nel, st));
   launch_hex8(st, (int)d.hexl.size(), d.hexlist.p, d.eptr.p, d.enodes.p, d.xyz.p, d.eprop.p, d.eoff.p, d.Kp.p,
               d.Mp.p, d.ebad.p);
+  launch_andes4(st, (int)d.shl.size(), d.shlist.p, d.eptr.p, d.enodes.p, d.xyz.p, d.eprop.p, d.eoff.p, d.Kp.p,
+                d.Mp.p, d.ebad.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[1], st));
   // ---- assembly (CSR K, M and the K entries of the fronts)
@@ -478,7 +485,9 @@ int DeviceReducer::reduce(std::string& err) {
   HIPCHK(hipStreamSynchronize(st));
   for (int e = 0; e < part_.nel; e++)
     if (eb[e]) {
-      err = "element " + std::to_string(e + 1) + " has a non-positive Jacobian determinant";
+      err = "element " + std::to_string(e + 1) +
+            (eb[e] == -3 ? " has a degenerate shell geometry (zero area or length)"
+                         : " has a non-positive Jacobian determinant");
       return -4;
     }
   if (hbad[0] > 0) {

@@ -63,6 +63,61 @@ def hex_block(nx, ny, nz, lx, ly, lz, E=2.1e11, nu=0.3, rho=7850.0, jitter=0.05,
     return _finish(xyz, status, np.full(nel, 44, np.int32), list(conn), props)
 
 
+def plate(nx, ny, lx, ly, t=0.005, E=2.1e11, nu=0.3, rho=7850.0, thickness=None, ext_nodes=(), clamp_x0=False,
+          warp=0.0, seed=0):
+    """nx x ny ANDES4 shell plate on [0,lx]x[0,ly] in the xy plane.
+
+    Node (i, j) (i along x, j along y) has index i*(ny+1) + j, so its id is
+    (ny+1)*i + j + 1 (the testModels.C:397-405 numbering).  thickness: per
+    element thickness array (nx*ny, element (i, j) at i*ny + j), else t.
+    ext_nodes: node indices made external; clamp_x0 fixes all six DOFs of the
+    x=0 edge.  warp: out-of-plane jitter amplitude (times the element size)
+    drawn with the seed, for non-planar test elements."""
+    i, j = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    hx, hy = lx / nx, ly / ny
+    xyz = np.stack([i * hx, j * hy, np.zeros_like(i, float)], axis=1).astype(np.float64)
+    if warp > 0:
+        rng = np.random.default_rng(seed)
+        xyz[:, 2] = rng.uniform(-warp, warp, len(xyz)) * min(hx, hy)
+    nid = lambda a, b: a * (ny + 1) + b
+    ei, ej = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ei, ej = ei.ravel(), ej.ravel()
+    conn = np.stack([nid(ei, ej), nid(ei + 1, ej), nid(ei + 1, ej + 1), nid(ei, ej + 1)], axis=1)
+    status = np.zeros(len(xyz), np.int32)
+    if clamp_x0:
+        status[i == 0] = FIXED_ALL
+    status[np.asarray(ext_nodes, int)] = 1
+    nel = len(conn)
+    th = np.full(nel, t) if thickness is None else np.asarray(thickness, float)
+    props = np.zeros((nel, NPROP))
+    props[:, 0], props[:, 1], props[:, 2] = E, nu, rho
+    props[:, 3] = E / (2 * (1 + nu))
+    props[:, 4:8] = th[:, None]
+    return _finish(xyz, status, np.full(nel, 24, np.int32), list(conn), props)
+
+
+def config0():
+    """Cantilever plate 1.0 x 0.5 m, 20 x 10 ANDES4, t = 5 mm, steel, x = 0
+    edge clamped, external nodes at the free corners (1.0, 0) and (1.0, 0.5)
+    (12 boundary DOFs), no component modes."""
+    nx, ny = 20, 10
+    return plate(nx, ny, 1.0, 0.5, 0.005, ext_nodes=[nx * (ny + 1), nx * (ny + 1) + ny], clamp_x0=True)
+
+
+def config2(seed=0, n=200, n_ext=64):
+    """n x n ANDES4 plate, 1 x 1 m, per-element thickness lognormal(ln 5 mm,
+    0.3) from the seed, n_ext external nodes drawn from the edge nodes with
+    the same seed (6 DOFs each), no fixed DOFs."""
+    rng = np.random.default_rng(seed)
+    th = np.exp(rng.normal(np.log(0.005), 0.3, n * n))
+    i, j = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    edge = np.nonzero((i == 0) | (i == n) | (j == 0) | (j == n))[0]
+    ext = rng.choice(edge, size=n_ext, replace=False)
+    return plate(n, n, 1.0, 1.0, thickness=th, ext_nodes=ext)
+
+
 def config1(seed=0):
     """48x24x24 HEX8 steel block, 0.48x0.24x0.24 m, +-5 % jitter, 12 external
     nodes on the x=0 and x=max faces (72 boundary DOFs), 20 component modes."""

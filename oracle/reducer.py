@@ -61,6 +61,24 @@ def hex8_matrices(X, Y, Z, E, nu, rho):
     return K.reshape(24, 24, order="F"), M.reshape(24, 24, order="F")
 
 
+def andes4_matrices(xyz, E, nu, th, rho):
+    """(K, M) 24x24 of one ANDES4 shell (ADD_ANDES4, inadd.f90:1407-1552):
+    P^T K P of the rotated local ANDES stiffness, FQS35 lumped diagonal
+    mass.  th: the four nodal thicknesses (their mean is the element's)."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(4, 3)
+    th = np.ascontiguousarray(th, np.float64)
+    t = (th[0] + th[1] + th[2] + th[3]) / 4.0
+    K = np.zeros(576)
+    rc = lib().andes4_stiff(_dp(xyz.ravel().copy()), ctypes.c_double(E), ctypes.c_double(nu), ctypes.c_double(t),
+                            _dp(K))
+    if rc != 0:
+        raise ValueError(f"andes4_stiff error {rc}")
+    em = np.zeros(24)
+    X, Y, Z = (np.ascontiguousarray(xyz[:, c]) for c in range(3))
+    lib().fqs35(_dp(X), _dp(Y), _dp(Z), _dp(th), ctypes.c_double(rho), _dp(em))
+    return K.reshape(24, 24, order="F"), np.diag(em)
+
+
 ELEM_NODES = {44: 8, 24: 4, 11: 2}
 ELEM_NDN = {44: 3, 24: 6, 11: 6}
 
@@ -155,6 +173,9 @@ def element_matrices(part, e):
     if t == 44:
         K, M = hex8_matrices(x[:, 0], x[:, 1], x[:, 2], pr[0], pr[1], pr[2])
         return nodes, 3, K, M
+    if t == 24:
+        K, M = andes4_matrices(x, pr[0], pr[1], pr[4:8], pr[2])
+        return nodes, 6, K, M
     raise NotImplementedError(f"element type {t}")
 
 
@@ -191,6 +212,8 @@ def element_mass(part, e, M):
     t = int(part["melcon"][e])
     if t == 44:
         return float(M[:, 0::3].sum())
+    if t == 24:  # dsum(NEDOF, EMV, 6): x-translation masses of the lumped diagonal
+        return float(np.diag(M)[0::6].sum())
     return 0.0
 
 

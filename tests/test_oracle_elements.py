@@ -86,3 +86,100 @@ def test_hex8_brick_mass_closed_form():
     assert np.allclose(Mx, m, rtol=2e-7, atol=0)
     assert abs(Mx.sum() - rho * a * b * c) <= 2e-7 * rho * a * b * c
     assert np.abs(M[0::3, 1::3]).max() == 0.0
+
+
+def _andes_local(g):
+    """Local frame, local coordinates and constitutive matrix of a golden
+    ANDES4 case, as ADD_ANDES4 builds them (inadd.f90:1470-1510)."""
+    import ctypes
+    L = orc.lib()
+    xyz = np.array(g["xyz"])
+    tr = np.array(g["trel"])
+    Tel = tr[:9].reshape(3, 3, order="F").T
+    loc = Tel @ (xyz - xyz[0]).T
+    E, nu, t = g["E"], g["nu"], g["t"]
+    c11 = E / (1 - nu * nu)
+    C3 = np.array([[c11, nu * c11, 0], [nu * c11, c11, 0], [0, 0, 0.5 * E / (1 + nu)]])
+    Cm = np.zeros((6, 6))
+    Cm[:3, :3] = C3 * t
+    Cm[3:, 3:] = C3 * t * t * t / 12.0
+    return L, xyz, loc, Cm
+
+
+def test_andes4_matches_reference_golden(golden):
+    """Frame (trans3P), local ANDES stiffness, projector P and FQS35 mass of
+    the restatement against the reference's own Femlib build."""
+    import ctypes
+    for g in golden["andes4"]:
+        L, xyz, loc, Cm = _andes_local(g)
+        tr = np.zeros(12)
+        mid = 0.5 * (xyz[2] + xyz[3])
+        L.andes4_trans3p(orc._dp(np.ascontiguousarray(xyz[0])), orc._dp(np.ascontiguousarray(xyz[1])),
+                         orc._dp(mid), orc._dp(tr))
+        assert np.array_equal(tr, np.array(g["trel"]))
+        K = np.zeros(576)
+        X, Y, Z = (np.ascontiguousarray(loc[i]) for i in range(3))
+        L.andes4_local_stiff(orc._dp(X), orc._dp(Y), orc._dp(Z), orc._dp(Cm.ravel(order="F").copy()),
+                             ctypes.c_double(1.5), ctypes.c_double(0.9), orc._dp(K))
+        Kr = np.array(g["Kloc"])
+        assert np.linalg.norm(K - Kr) <= 1e-14 * np.linalg.norm(Kr)
+        P = np.zeros(576)
+        L.pmat_stiff(4, orc._dp(np.ascontiguousarray((xyz - xyz[0]).ravel())), orc._dp(P))
+        assert np.max(np.abs(P - np.array(g["P"]))) <= 1e-15
+    for g in golden["fqs35"]:
+        _, M = orc.andes4_matrices(np.array(g["xyz"]), 2.1e11, 0.3, g["th"], g["rho"])
+        assert np.array_equal(np.diag(M), np.array(g["EM"]))
+
+
+def shell_rigid_modes(xyz):
+    R = np.zeros((24, 6))
+    for a in range(4):
+        x = xyz[a]
+        R[6 * a:6 * a + 3, :3] = np.eye(3)
+        R[6 * a:6 * a + 3, 3:] = np.array([[0, x[2], -x[1]], [-x[2], 0, x[0]], [x[1], -x[0], 0]])
+        R[6 * a + 3:6 * a + 6, 3:] = np.eye(3)
+    return R
+
+
+def test_andes4_invariants():
+    """Projected stiffness: symmetric, the six rigid-body motions of the
+    (warped, rotated) element in its null space, no other zero modes beyond
+    the drilling-free ones; lumped mass: translational masses sum to
+    rho t A."""
+    rng = np.random.default_rng(11)
+    base = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0.]]) * 0.05
+    xyz = base + rng.uniform(-4e-3, 4e-3, (4, 3)) * np.array([1, 1, 0.3])
+    Q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    xyz = xyz @ Q.T + np.array([0.3, -0.1, 2.0])
+    th = np.full(4, 0.004)
+    K, M = orc.andes4_matrices(xyz, 2.1e11, 0.3, th, 7850.0)
+    assert np.allclose(K, K.T, rtol=0, atol=1e-10 * np.abs(K).max())
+    R = shell_rigid_modes(xyz)
+    assert np.abs(K @ R).max() <= 1e-8 * np.abs(K).max() * np.abs(R).max()
+    w = np.linalg.eigvalsh(K)
+    assert np.sum(w < 1e-10 * w.max()) >= 6 and w.min() > -1e-9 * w.max()
+    # planar element in the xy plane: translational lumped mass = rho t A
+    flat = base + np.array([[0, 0, 0], [0.01, 0.002, 0], [0.0, 0.01, 0], [-0.004, 0, 0.]])
+    _, Mf = orc.andes4_matrices(flat, 2.1e11, 0.3, th, 7850.0)
+    x, y = flat[:, 0], flat[:, 1]
+    A = 0.5 * abs((x[0] - x[2]) * (y[1] - y[3]) - (x[1] - x[3]) * (y[0] - y[2]))
+    m = np.diag(Mf)
+    for d in range(3):
+        assert abs(m[d::6].sum() - 7850.0 * 0.004 * A) <= 1e-12 * 7850.0 * 0.004 * A
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built here")
+def test_andes4_matches_live_reference_build():
+    rng = np.random.default_rng(5)
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0.]]) * 0.03 + rng.uniform(-2e-3, 2e-3, (4, 3))
+    inp = "andes4\n" + " ".join("%.17e" % v for v in [2.1e11, 0.3, 0.003, *xyz[:, 0], *xyz[:, 1], *xyz[:, 2]])
+    out = subprocess.run([REF], input=inp + "\nquit\n", capture_output=True, text=True, check=True).stdout.split()
+    g = {"xyz": xyz.tolist(), "E": 2.1e11, "nu": 0.3, "t": 0.003, "trel": [float(v) for v in out[1:13]]}
+    Kr = np.array(out[13:13 + 576], float).reshape(24, 24, order="F")
+    Pr = np.array(out[13 + 576:13 + 1152], float).reshape(24, 24, order="F")
+    # full ADD_ANDES4 chain: rotate the reference's local K, project with its P
+    tr = np.array(g["trel"])[:9].reshape(3, 3, order="F")
+    T = np.kron(np.eye(8), tr)
+    EKr = Pr.T @ (T @ Kr @ T.T) @ Pr
+    K, _ = orc.andes4_matrices(xyz, 2.1e11, 0.3, np.full(4, 0.003), 7850.0)
+    assert np.linalg.norm(K - EKr) <= 1e-12 * np.linalg.norm(EKr)

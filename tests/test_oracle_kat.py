@@ -31,6 +31,26 @@ def test_chexa08_known_answer(golden_dir):
             assert abs(v[i] - s) <= 1e-9, (i, v[i], s)
 
 
+def test_cquad04_known_answer(golden_dir):
+    v, S = kat("CQUAD04_", golden_dir)
+    for i, s in enumerate(S):
+        if abs(s) > 0:
+            assert abs(v[i] - s) <= 1e-9, (i, v[i], s)
+
+
+def test_cquad04_kat_fails_with_planted_error(golden_dir, monkeypatch):
+    """A dropped bending stiffness term (planted in the oracle) must break
+    the shell KAT."""
+    real = orc.andes4_matrices
+
+    def bad(xyz, E, nu, th, rho):
+        return real(xyz, E, nu, np.asarray(th) * 0.99, rho)
+
+    monkeypatch.setattr(orc, "andes4_matrices", bad)
+    v, S = kat("CQUAD04_", golden_dir)
+    assert abs(v[1] - S[1]) > 1e-9
+
+
 def test_kat_fails_with_planted_error(golden_dir, monkeypatch):
     """A dropped mass term (planted in the oracle) must break the KAT."""
     real = orc.hex8_matrices
