@@ -23,7 +23,7 @@ enum PropSlot : int {
   P_E = 0, P_NU = 1, P_RHO = 2, P_G = 3,
   P_T1 = 4, P_T2 = 5, P_T3 = 6, P_T4 = 7,        // shell thickness per node
   P_A = 4, P_IY = 5, P_IZ = 6, P_IT = 7,         // beam section
-  P_DX = 8, P_DY = 9, P_DZ = 10,                 // beam local-z direction
+  P_DX = 8, P_DY = 9, P_DZ = 10,                 // beam orientation vector v (local x-y plane, CBAR)
   P_KY = 11, P_KZ = 12,                          // beam shear reduction factors
   P_NSM = 13,                                    // non-structural mass
   NPROP = 16

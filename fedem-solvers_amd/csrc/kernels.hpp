@@ -11,6 +11,9 @@ void launch_hex8(hipStream_t st, int nhex, const int* elist, const int* eptr, co
 // ANDES4 shells (elem_shell.hip): P^T K P stiffness and FQS35 lumped mass
 void launch_andes4(hipStream_t st, int nel, const int* elist, const int* eptr, const int* enodes, const double* xyz,
                    const double* eprop, const int64_t* eoff, double* Kp, double* Mp, int* ebad);
+// 2-node beams (elem_beam.hip): BEAM31 stiffness and BEAM35 consistent mass
+void launch_beam(hipStream_t st, int nbeam, const int* elist, const int* eptr, const int* enodes, const double* xyz,
+                 const double* eprop, const int64_t* eoff, double* Kp, double* Mp, int* ebad);
 void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* Kp,
                      const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool);
 void launch_fill(hipStream_t st, double* p, int64_t n, double v);

@@ -258,6 +258,7 @@ int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& e
   for (int e = 0; e < part_.nel; e++) {
     if (part_.etype[e] == HEX8) d_->hexl.push_back(e);
     else if (part_.etype[e] == ANDES4) d_->shl.push_back(e);
+    else if (part_.etype[e] == BEAM2) d_->bml.push_back(e);
     else {
       err = "element type " + std::to_string(part_.etype[e]) + " not supported on the device path yet";
       return -2;
@@ -430,6 +431,8 @@ int DeviceReducer::reduce(std::string& err) {
               d.Mp.p, d.ebad.p);
   launch_andes4(st, (int)d.shl.size(), d.shlist.p, d.eptr.p, d.enodes.p, d.xyz.p, d.eprop.p, d.eoff.p, d.Kp.p,
                 d.Mp.p, d.ebad.p);
+  launch_beam(st, (int)d.bml.size(), d.bmlist.p, d.eptr.p, d.enodes.p, d.xyz.p, d.eprop.p, d.eoff.p, d.Kp.p,
+              d.Mp.p, d.ebad.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[1], st));
   // ---- assembly (CSR K, M and the K entries of the fronts)

@@ -118,6 +118,76 @@ def config2(seed=0, n=200, n_ext=64):
     return plate(n, n, 1.0, 1.0, thickness=th, ext_nodes=ext)
 
 
+TUBE = dict(ro=0.020, t=0.002)
+
+
+def tube_section(ro=0.020, t=0.002):
+    """Area, I1 = I2 and J = 2 I of a thin-walled tube."""
+    ri = ro - t
+    A = np.pi * (ro * ro - ri * ri)
+    I = np.pi * (ro ** 4 - ri ** 4) / 4.0
+    return A, I, 2.0 * I
+
+
+def beam_frame(n_el=100, seed=0, n_ext=4, bay=0.5, E=2.1e11, nu=0.3, rho=7850.0, shear_k=0.5):
+    """Space-frame tower of 2-node tube beams (r_o = 20 mm, t = 2 mm): square
+    bays of side `bay` stacked along z, each with four columns, four
+    horizontal members and two face diagonals, every member split into `seg`
+    elements; bays and segments are chosen so the element count is close to
+    n_el.  n_ext external nodes are drawn with the seed among the member
+    joints, always including one base corner so the internal problem is
+    constrained."""
+    rng = np.random.default_rng(seed)
+    per_bay = 10
+    seg = max(1, int(round(np.sqrt(n_el / (per_bay * 4.0)))))
+    nbay = max(1, int(round(n_el / (per_bay * seg))))
+    corners = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float) * bay
+    joints = []
+    for k in range(nbay + 1):
+        for c in corners:
+            joints.append([c[0], c[1], k * bay])
+    xyz = [list(j) for j in joints]
+    conn, vecs = [], []
+
+    def member(a, b):
+        pa, pb = np.array(xyz[a]), np.array(xyz[b])
+        d = pb - pa
+        v = [1.0, 0.0, 0.0] if abs(d[2]) > 0.9 * np.linalg.norm(d) else [0.0, 0.0, 1.0]
+        prev = a
+        for s_ in range(1, seg + 1):
+            if s_ < seg:
+                xyz.append(list(pa + d * s_ / seg))
+                cur = len(xyz) - 1
+            else:
+                cur = b
+            conn.append([prev, cur])
+            vecs.append(v)
+            prev = cur
+
+    J = lambda k, c: 4 * k + c
+    for k in range(nbay):
+        for c in range(4):
+            member(J(k, c), J(k + 1, c))                # columns
+            member(J(k + 1, c), J(k + 1, (c + 1) % 4))  # ring at the top of the bay
+        member(J(k, 0), J(k + 1, 1))                    # face diagonals
+        member(J(k, 2), J(k + 1, 3))
+    for c in range(4):
+        member(J(0, c), J(0, (c + 1) % 4))              # base ring
+    xyz = np.array(xyz, float)
+    status = np.zeros(len(xyz), np.int32)
+    cand = np.arange(1, 4 * (nbay + 1))
+    ext = [0] + list(rng.choice(cand, size=min(max(n_ext - 1, 0), len(cand)), replace=False))
+    status[ext] = 1
+    nel = len(conn)
+    A, I, Jt = tube_section()
+    props = np.zeros((nel, NPROP))
+    props[:, 0], props[:, 1], props[:, 2], props[:, 3] = E, nu, rho, E / (2 * (1 + nu))
+    props[:, 4], props[:, 5], props[:, 6], props[:, 7] = A, I, I, Jt
+    props[:, 8:11] = np.array(vecs)
+    props[:, 11] = props[:, 12] = shear_k
+    return _finish(xyz, status, np.full(nel, 11, np.int32), [np.array(c) for c in conn], props)
+
+
 def config1(seed=0):
     """48x24x24 HEX8 steel block, 0.48x0.24x0.24 m, +-5 % jitter, 12 external
     nodes on the x=0 and x=max faces (72 boundary DOFs), 20 component modes."""

@@ -59,7 +59,7 @@ enum {
   FGPU_P_E = 0, FGPU_P_NU = 1, FGPU_P_RHO = 2, FGPU_P_G = 3,
   FGPU_P_T1 = 4, /* shell thickness of nodes 1..4 at 4..7 */
   FGPU_P_A = 4, FGPU_P_IY = 5, FGPU_P_IZ = 6, FGPU_P_IT = 7, /* beam section */
-  FGPU_P_DX = 8, FGPU_P_DY = 9, FGPU_P_DZ = 10,              /* beam local z */
+  FGPU_P_DX = 8, FGPU_P_DY = 9, FGPU_P_DZ = 10,              /* beam orientation v (local x-y plane) */
   FGPU_P_KY = 11, FGPU_P_KZ = 12, FGPU_P_NSM = 13,
   FGPU_NPROP = 16
 };

@@ -79,6 +79,33 @@ def andes4_matrices(xyz, E, nu, th, rho):
     return K.reshape(24, 24, order="F"), np.diag(em)
 
 
+def beam_matrices(xyz, pr):
+    """(K, M) 12x12 of one beam (ADD11 -> BEAM31/BEAM35, inadd.f90:829-930).
+    pr: property record (E, nu, rho, G, A, I1, I2, J, v(3), K1, K2, NSM)
+    with the Nastran PBAR/CBAR meaning; I1 bends in the local x-y plane
+    (the plane of v), the shear factors kappa = 1/K (0: no shear
+    deformation), the density carries the non-structural mass.  The
+    section mapping (I1 -> I_y of BELS31, kappa = 1/K) is the one the
+    CBAR02_ known answer selects among the alternatives."""
+    x1 = np.ascontiguousarray(xyz[0], np.float64)
+    x2 = np.ascontiguousarray(xyz[1], np.float64)
+    v = np.ascontiguousarray(pr[8:11], np.float64)
+    A = pr[4]
+    Iy, Iz = pr[5], pr[6]
+    kap = lambda k: 1.0 / k if k > 0 else 0.0
+    rho = pr[2] + (pr[13] / A if A > 1e-16 else 0.0)
+    K = np.zeros(144)
+    M = np.zeros(144)
+    c = ctypes.c_double
+    rc = lib().beam_stiff(_dp(x1), _dp(x2), _dp(v), c(pr[0]), c(pr[3]), c(A), c(Iy), c(Iz), c(pr[7]),
+                          c(kap(pr[11])), c(kap(pr[12])), _dp(K))
+    if rc != 0:
+        raise ValueError(f"beam_stiff error {rc}")
+    em = np.zeros(1)
+    lib().beam_mass(_dp(x1), _dp(x2), _dp(v), c(A), c(rho), c(pr[5] + pr[6]), _dp(M), _dp(em))
+    return K.reshape(12, 12, order="F"), M.reshape(12, 12, order="F"), float(em[0])
+
+
 ELEM_NODES = {44: 8, 24: 4, 11: 2}
 ELEM_NDN = {44: 3, 24: 6, 11: 6}
 
@@ -176,6 +203,9 @@ def element_matrices(part, e):
     if t == 24:
         K, M = andes4_matrices(x, pr[0], pr[1], pr[4:8], pr[2])
         return nodes, 6, K, M
+    if t == 11:
+        K, M, _ = beam_matrices(x, pr)
+        return nodes, 6, K, M
     raise NotImplementedError(f"element type {t}")
 
 
@@ -214,6 +244,10 @@ def element_mass(part, e, M):
         return float(M[:, 0::3].sum())
     if t == 24:  # dsum(NEDOF, EMV, 6): x-translation masses of the lumped diagonal
         return float(np.diag(M)[0::6].sum())
+    if t == 11:  # EM(1,1) + EM(7,1) of the diagonal form: rho A L
+        mp = part["mpmnpc"]
+        nodes = part["mmnpc"][mp[e] - 1:mp[e + 1] - 1] - 1
+        return beam_matrices(part["xyz"][nodes], part["eprop"][e])[2]
     return 0.0
 
 

@@ -203,3 +203,50 @@ def test_shell_reduction_matches_oracle(name, make, ngen):
         lam, _ = R.eigen()
         assert np.max(np.abs(lam - o["lam"]) / o["lam"]) <= 1e-9
         assert rel(align(R.phi(), o["phi"]), o["phi"]) <= TOL
+
+
+def test_device_beam_elements_match_oracle():
+    part = parts.beam_frame(n_el=60, seed=4, n_ext=3)
+    rng = np.random.default_rng(0)
+    part["eprop"][:, 8:11] += rng.uniform(-0.2, 0.2, (len(part["melcon"]), 3))  # skew orientations
+    part["eprop"][:, 13] = 0.1  # non-structural mass
+    part["eprop"][::3, 11] = 0.0  # some members without shear deformation
+    R = gpu.GpuReducer()
+    R.set_part(part)
+    R.analyze()
+    R.reduce()
+    Kp, Mp, off = R.element_matrices()
+    for e in range(len(part["melcon"])):
+        _, _, Ke, Me = orc.element_matrices(part, e)
+        blk = Kp[off[e]:off[e + 1]].reshape(3, 6, 6)
+        mbl = Mp[off[e]:off[e + 1]].reshape(3, 6, 6)
+        for t, (a, b) in enumerate([(0, 0), (1, 0), (1, 1)]):
+            assert np.allclose(blk[t], Ke[6 * a:6 * a + 6, 6 * b:6 * b + 6], rtol=0, atol=1e-12 * np.abs(Ke).max())
+            assert np.allclose(mbl[t], Me[6 * a:6 * a + 6, 6 * b:6 * b + 6], rtol=0, atol=1e-12 * np.abs(Me).max())
+
+
+BEAM_CASES = [
+    ("frame60_static", lambda: parts.beam_frame(n_el=60, seed=1, n_ext=2), 0),
+    ("frame200_modes", lambda: parts.beam_frame(n_el=200, seed=2, n_ext=5), 6),
+    ("frame500_modes", lambda: parts.beam_frame(n_el=500, seed=3, n_ext=16), 10),
+]
+
+
+@pytest.mark.parametrize("name,make,ngen", BEAM_CASES)
+def test_beam_reduction_matches_oracle(name, make, ngen):
+    part = make()
+    R = gpu.GpuReducer()
+    R.set_part(part)
+    R.analyze(ngen=ngen, vgi=True)
+    R.reduce()
+    SK, SM, G = R.reduced()
+    o = orc.reduce(part, ngen=ngen, want_vgi=True)
+    assert rel(SK, o["SK"]) <= TOL
+    assert rel(SM, o["SM"]) <= TOL
+    assert rel(G, o["G"]) <= TOL
+    assert rel(R.B(), o["B"]) <= TOL
+    assert rel(R.vgi(), o["vgi"]) <= TOL
+    if ngen:
+        lam, _ = R.eigen()
+        assert np.max(np.abs(lam - o["lam"]) / o["lam"]) <= 1e-9
+        assert rel(align(R.phi(), o["phi"]), o["phi"]) <= TOL

@@ -183,3 +183,44 @@ def test_andes4_matches_live_reference_build():
     EKr = Pr.T @ (T @ Kr @ T.T) @ Pr
     K, _ = orc.andes4_matrices(xyz, 2.1e11, 0.3, np.full(4, 0.003), 7850.0)
     assert np.linalg.norm(K - EKr) <= 1e-12 * np.linalg.norm(EKr)
+
+
+def test_beam_closed_forms():
+    """Beam restatement against the mathematics: a cantilever's tip
+    deflections P L^3/(3 E I) (1 + 3 phi / (1 + ... )) in Timoshenko form,
+    the axial and torsional stiffness, six rigid-body modes of an arbitrarily
+    oriented element and the consistent-mass totals (rho A L per direction)."""
+    E, G, A, I1, I2, J, L = 2.1e11, 8.1e10, 3e-3, 2e-6, 5e-6, 4e-6, 0.7
+    K1 = 0.8
+    pr = np.zeros(16)
+    pr[0], pr[1], pr[2], pr[3] = E, 0.3, 7800.0, G
+    pr[4:8] = A, I1, I2, J
+    pr[8:11] = 0.0, 1.0, 0.0  # local y = global y for a beam along x
+    pr[11] = pr[12] = K1
+    K, M, em = orc.beam_matrices(np.array([[0, 0, 0], [L, 0, 0.]]), pr)
+    assert np.allclose(K, K.T, rtol=0, atol=1e-9 * np.abs(K).max())
+    f = K[6:, 6:]
+    u = np.linalg.solve(f, np.eye(6))  # node 1 clamped: flexibility at node 2
+    phi1 = 12 * E * I1 / (K1 * A * G * L * L)
+    phi2 = 12 * E * I2 / (K1 * A * G * L * L)
+    assert np.isclose(u[0, 0], L / (E * A), rtol=1e-12)
+    assert np.isclose(u[3, 3], L / (G * J), rtol=1e-12)
+    assert np.isclose(u[1, 1], L ** 3 / (3 * E * I1) * (1 + phi1 / 4), rtol=1e-12)  # y: I1
+    assert np.isclose(u[2, 2], L ** 3 / (3 * E * I2) * (1 + phi2 / 4), rtol=1e-12)  # z: I2
+    # rotated, skew element: rigid-body modes in the null space
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(2, 3))
+    pr[8:11] = rng.normal(size=3)
+    K, M, em = orc.beam_matrices(x, pr)
+    R = np.zeros((12, 6))
+    for a in range(2):
+        R[6 * a:6 * a + 3, :3] = np.eye(3)
+        p = x[a]
+        R[6 * a:6 * a + 3, 3:] = np.array([[0, p[2], -p[1]], [-p[2], 0, p[0]], [p[1], -p[0], 0]])
+        R[6 * a + 3:6 * a + 6, 3:] = np.eye(3)
+    assert np.abs(K @ R).max() <= 1e-8 * np.abs(K).max() * np.abs(R).max()
+    Lx = np.linalg.norm(x[1] - x[0])
+    m = 7800.0 * A * Lx
+    assert np.isclose(em, m, rtol=1e-14)
+    for d in range(3):
+        assert np.isclose(R[:, d] @ M @ R[:, d], m, rtol=1e-12)

@@ -51,6 +51,28 @@ def test_cquad04_kat_fails_with_planted_error(golden_dir, monkeypatch):
     assert abs(v[1] - S[1]) > 1e-9
 
 
+def test_cbar02_known_answer(golden_dir):
+    v, S = kat("CBAR02_", golden_dir)
+    for i, s in enumerate(S):
+        if abs(s) > 0:
+            assert abs(v[i] - s) <= 1e-9, (i, v[i], s)
+
+
+def test_cbar02_kat_fails_with_swapped_section(golden_dir, monkeypatch):
+    """Swapping the two bending inertias (planted in the oracle) must break
+    the beam KAT."""
+    real = orc.beam_matrices
+
+    def bad(xyz, pr):
+        q = np.array(pr, float).copy()
+        q[5], q[6] = pr[6], pr[5]
+        return real(xyz, q)
+
+    monkeypatch.setattr(orc, "beam_matrices", bad)
+    v, S = kat("CBAR02_", golden_dir)
+    assert abs(v[1] - S[1]) > 1e-9
+
+
 def test_kat_fails_with_planted_error(golden_dir, monkeypatch):
     """A dropped mass term (planted in the oracle) must break the KAT."""
     real = orc.hex8_matrices
