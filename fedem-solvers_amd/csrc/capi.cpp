@@ -81,6 +81,29 @@ int fgpu_set_part(fgpu_reducer* h, int nnod, const double* xyz, const int* statu
   END_GUARD(h)
 }
 
+int fgpu_set_rigid(fgpu_reducer* h, int nrgd, const int* indep, const int* dofmask, const int* dptr,
+                   const int* dep) {
+  GUARD(h)
+  if (!h->have_part || nrgd < 0 || (nrgd > 0 && (!indep || !dofmask || !dptr || !dep))) {
+    h->err = "fgpu_set_rigid: invalid arguments (set the part first)";
+    return -1;
+  }
+  fgpu::Part& p = h->part;
+  p.rgd_indep.resize(nrgd);
+  p.rgd_mask.assign(dofmask, dofmask + nrgd);
+  p.rgd_ptr.resize(nrgd + 1);
+  for (int r = 0; r <= nrgd; r++) p.rgd_ptr[r] = dptr[r] - 1;
+  for (int r = 0; r < nrgd; r++) {
+    p.rgd_indep[r] = indep[r] - 1;
+    if (p.rgd_ptr[r + 1] < p.rgd_ptr[r]) { h->err = "fgpu_set_rigid: bad pointers"; return -2; }
+  }
+  p.rgd_dep.resize(nrgd ? p.rgd_ptr[nrgd] : 0);
+  for (size_t k = 0; k < p.rgd_dep.size(); k++) p.rgd_dep[k] = dep[k] - 1;
+  h->analyzed = h->reduced = false;
+  return 0;
+  END_GUARD(h)
+}
+
 int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags) {
   GUARD(h)
   if (!h->have_part) { h->err = "fgpu_analyze: no part"; return -1; }

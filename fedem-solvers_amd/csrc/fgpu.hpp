@@ -46,6 +46,12 @@ struct Part {
   std::vector<int> eptr;       // nel+1
   std::vector<int> enodes;     // 0-based node indices
   std::vector<double> eprop;   // NPROP*nel
+  // rigid elements (RBE2): the DOFs in rgd_mask of every dependent node
+  // follow the rigid-body motion of the independent node
+  std::vector<int> rgd_indep;  // per rigid element, 0-based node
+  std::vector<int> rgd_mask;   // bitmask over the six local DOFs
+  std::vector<int> rgd_ptr;    // nrgd+1 pointers into rgd_dep (0-based)
+  std::vector<int> rgd_dep;    // 0-based dependent nodes
 };
 
 // SAM arrays (samModule.f90:27-64), 0-based storage of 1-based values where
@@ -53,10 +59,11 @@ struct Part {
 struct Sam {
   int nnod = 0, ndof = 0, neq = 0, ndof1 = 0, ndof2 = 0;
   std::vector<int> madof;      // nnod+1, 1-based accumulated DOF pointers
-  std::vector<int> msc;        // ndof, 0 fixed / 1 internal / 2 external
+  std::vector<int> msc;        // ndof, 0 fixed / 1 internal / 2 external / -1 dependent (rigid element)
   std::vector<int> meqn;       // ndof, 0 for fixed, else 1-based equation
   std::vector<int> meqn1, meqn2, dofPosIn2;
   std::vector<int> nndof;      // per node
+  std::vector<int> dep_indep;  // ndof: independent node of a dependent DOF, else -1
   int nenod = 0;               // number of external nodes
   int ndim_ext = 0;            // sum of DOFs of external nodes
 };
@@ -79,6 +86,7 @@ struct Symbolic {
   // pull-assembly contributions of every CSR entry: packed element entries
   std::vector<int> cptr;              // nnz+1
   std::vector<int64_t> cidx;          // index into the packed element pool
+  std::vector<double> ccoef;          // constraint coefficient of each contribution (empty: all 1)
   // lower CSR entry -> offset into the front pool (-1 for upper entries)
   std::vector<int64_t> fmap;
 

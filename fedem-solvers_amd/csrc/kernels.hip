@@ -151,8 +151,10 @@ __global__ __launch_bounds__(64) void k_hex8(int nhex, const int* __restrict__ e
 // contributions in a fixed order, so the result is deterministic and needs no
 // atomics or colouring passes.  Lower-triangle stiffness entries are also
 // written straight into their front position.
+template <bool COEF>
 __global__ __launch_bounds__(256) void k_assemble(int64_t nnz, const int* __restrict__ cptr,
                                                   const int64_t* __restrict__ cidx,
+                                                  const double* __restrict__ ccoef,
                                                   const double* __restrict__ Kp,
                                                   const double* __restrict__ Mp,
                                                   double* __restrict__ Kv, double* __restrict__ Mv,
@@ -163,8 +165,14 @@ __global__ __launch_bounds__(256) void k_assemble(int64_t nnz, const int* __rest
   double kv = 0.0, mv = 0.0;
   for (int q = cptr[k]; q < cptr[k + 1]; q++) {
     int64_t c = cidx[q];
-    kv += Kp[c];
-    mv += Mp[c];
+    if (COEF) {  // rigid-element constraint coefficient of this contribution
+      const double w = ccoef[q];
+      kv += w * Kp[c];
+      mv += w * Mp[c];
+    } else {
+      kv += Kp[c];
+      mv += Mp[c];
+    }
   }
   Kv[k] = kv;
   Mv[k] = mv;
@@ -807,9 +815,11 @@ void launch_hex8(hipStream_t st, int nhex, const int* elist, const int* eptr, co
                  int* ebad) {
   if (nhex > 0) k_hex8<<<nhex, 64, 0, st>>>(nhex, elist, eptr, enodes, xyz, eprop, eoff, Kp, Mp, ebad);
 }
-void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* Kp,
-                     const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool) {
-  if (nnz > 0) k_assemble<<<nblk(nnz, 256), 256, 0, st>>>(nnz, cptr, cidx, Kp, Mp, Kv, Mv, fmap, pool);
+void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* ccoef,
+                     const double* Kp, const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool) {
+  if (nnz <= 0) return;
+  if (ccoef) k_assemble<true><<<nblk(nnz, 256), 256, 0, st>>>(nnz, cptr, cidx, ccoef, Kp, Mp, Kv, Mv, fmap, pool);
+  else k_assemble<false><<<nblk(nnz, 256), 256, 0, st>>>(nnz, cptr, cidx, ccoef, Kp, Mp, Kv, Mv, fmap, pool);
 }
 void launch_fill(hipStream_t st, double* p, int64_t n, double v) {
   if (n > 0) k_fill<<<(unsigned)std::min<int64_t>(nblk(n, 256), 16384), 256, 0, st>>>(p, n, v);

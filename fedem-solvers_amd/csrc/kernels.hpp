@@ -14,8 +14,9 @@ void launch_andes4(hipStream_t st, int nel, const int* elist, const int* eptr, c
 // 2-node beams (elem_beam.hip): BEAM31 stiffness and BEAM35 consistent mass
 void launch_beam(hipStream_t st, int nbeam, const int* elist, const int* eptr, const int* enodes, const double* xyz,
                  const double* eprop, const int64_t* eoff, double* Kp, double* Mp, int* ebad);
-void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* Kp,
-                     const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool);
+// ccoef: per-contribution constraint coefficients, or nullptr (all 1)
+void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* ccoef,
+                     const double* Kp, const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool);
 void launch_fill(hipStream_t st, double* p, int64_t n, double v);
 void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool);
 void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, const int64_t* ioff,

@@ -1,7 +1,7 @@
 // Nastran bulk-data subset reader for the part input (-linkfile).  The
 // reference parses parts with FFl (absent fedem-foundation); this reader
 // covers the cards of the element types on the device path: GRID, CHEXA,
-// CQUAD4, CBAR, MAT1, PSHELL, PSOLID, PBAR, ASET/ASET1, SPC1, RBE2 and the
+// CQUAD4, CBAR, MAT1, PSHELL, PSOLID, PBAR, PBARL (TUBE), ASET/ASET1, SPC1, RBE2 and the
 // FORCE/MOMENT nodal loads (basic coordinate system only), in
 // free (comma), small (8 column) and large (16 column, '*') field formats.
 // Nodes are numbered in ascending GRID id order, as SAM numbers them.
@@ -151,6 +151,19 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
       Prop p;
       p.kind = 3; p.mid = I(f, 1); p.A = D(f, 2); p.I1 = D(f, 3); p.I2 = D(f, 4); p.J = D(f, 5); p.nsm = D(f, 6);
       p.K1 = D(f, 16); p.K2 = D(f, 17);
+      props[I(f, 0)] = p;
+    } else if (c.name == "PBARL") {
+      // TUBE only: DIM1 outer radius, DIM2 inner radius; thin-tube shear area A/2
+      std::string type = f.size() > 3 ? trim(f[3]) : std::string();
+      for (char& ch : type) ch = (char)std::toupper(ch);
+      if (type != "TUBE") { err = "PBARL section type " + type + " is not supported"; return -2; }
+      const double R = D(f, 8), r = D(f, 9), pi = 3.14159265358979323846;
+      Prop p;
+      p.kind = 3; p.mid = I(f, 1);
+      p.A = pi * (R * R - r * r);
+      p.I1 = p.I2 = pi * (R * R * R * R - r * r * r * r) / 4.0;
+      p.J = 2.0 * p.I1;
+      p.K1 = p.K2 = 0.5;
       props[I(f, 0)] = p;
     } else if (c.name == "ASET") {
       for (size_t k = 0; k + 1 < f.size() + 1; k += 2)

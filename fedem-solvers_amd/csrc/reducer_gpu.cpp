@@ -88,7 +88,7 @@ struct DeviceReducer::Impl {
   // values
   DBuf<double> Kp, Mp, Kv, Mv, pool, W, V, ipool;
   DBuf<int> ebad, piv;
-  DBuf<double> X, Z, Y, part, small, acc, grav;
+  DBuf<double> X, Z, Y, part, small, acc, grav, ccoef;
   // component modes
   DBuf<double> Q, P, Wl, Zl, Yl, Cc, Phi, Tmp;
   std::vector<int> hexl, shl, bml;  // elements by type: HEX8, ANDES4, BEAM2
@@ -306,6 +306,7 @@ int DeviceReducer::upload(std::string& err) {
   HIPCHK(d.colind.upload(y.colind));
   HIPCHK(d.cptr.upload(y.cptr));
   HIPCHK(d.cidx.upload(y.cidx));
+  if (!y.ccoef.empty()) HIPCHK(d.ccoef.upload(y.ccoef));
   HIPCHK(d.fmap.upload(y.fmap));
   d.S = DevSym{d.col0.p, d.ncol.p, d.nrow.p, d.foff.p, d.rptr.p, d.rows.p, d.chptr.p,
                d.chlist.p, d.relptr.p, d.rel.p, d.woff.p};
@@ -437,7 +438,8 @@ int DeviceReducer::reduce(std::string& err) {
   HIPCHK(hipEventRecord(d.ev[1], st));
   // ---- assembly (CSR K, M and the K entries of the fronts)
   launch_fill(st, d.pool.p, y.fpool, 0.0);
-  launch_assemble(st, nnz, d.cptr.p, d.cidx.p, d.Kp.p, d.Mp.p, d.Kv.p, d.Mv.p, d.fmap.p, d.pool.p);
+  launch_assemble(st, nnz, d.cptr.p, d.cidx.p, y.ccoef.empty() ? nullptr : d.ccoef.p, d.Kp.p, d.Mp.p, d.Kv.p, d.Mv.p,
+                  d.fmap.p, d.pool.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[2], st));
   // ---- multifrontal LDL^T; the Schur front ends up holding K_bb - K_bi K_ii^-1 K_ib

@@ -286,16 +286,22 @@ int fgpu_link_node_ids(int* ids) {
 int fgpu_link_attach(fgpu_reducer* h) {
   Link& L = link();
   if (!L.loaded || !h) return -1;
-  if (!L.np.rbe2.empty()) {
-    L.err = "rigid elements (RBE2) are not supported by the device reducer";
-    return -3;
-  }
   const fgpu::Part& p = L.np.part;
   std::vector<int> mpmnpc(p.nel + 1), mmnpc(p.enodes.size());
   for (int e = 0; e <= p.nel; e++) mpmnpc[e] = p.eptr[e] + 1;
   for (size_t k = 0; k < p.enodes.size(); k++) mmnpc[k] = p.enodes[k] + 1;
   int rc = fgpu_set_part(h, p.nnod, p.xyz.data(), p.status.data(), p.nel, p.etype.data(), mpmnpc.data(),
                          mmnpc.data(), p.eprop.data());
+  if (rc == 0 && !L.np.rbe2.empty()) {
+    std::vector<int> indep, mask, dptr(1, 1), dep;
+    for (const auto& r : L.np.rbe2) {
+      indep.push_back(r.indep + 1);
+      mask.push_back(r.dofs);
+      for (int u : r.dep) dep.push_back(u + 1);
+      dptr.push_back((int)dep.size() + 1);
+    }
+    rc = fgpu_set_rigid(h, (int)indep.size(), indep.data(), mask.data(), dptr.data(), dep.data());
+  }
   if (rc) L.err = fgpu_last_error(h);
   return rc;
 }

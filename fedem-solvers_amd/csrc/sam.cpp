@@ -39,6 +39,30 @@ int build_sam(const Part& p, Sam& s, std::string& err) {
   }
   s.ndof = s.madof[p.nnod] - 1;
   s.msc.assign(s.ndof, 0);
+  // rigid elements: dependent DOFs carry no equation (RGD elements,
+  // samReducerModule.f90:385-459); their independent nodes keep all DOFs
+  s.dep_indep.assign(s.ndof, -1);
+  std::vector<char> indep(p.nnod, 0);
+  const int nrgd = (int)p.rgd_indep.size();
+  for (int r = 0; r < nrgd; r++) {
+    int g = p.rgd_indep[r];
+    if (g < 0 || g >= p.nnod) { err = "rigid element " + std::to_string(r + 1) + ": bad independent node"; return -1; }
+    indep[g] = 1;
+  }
+  for (int r = 0; r < nrgd; r++) {
+    int g = p.rgd_indep[r];
+    for (int k = p.rgd_ptr[r]; k < p.rgd_ptr[r + 1]; k++) {
+      int u = p.rgd_dep[k];
+      if (u < 0 || u >= p.nnod || u == g) { err = "rigid element " + std::to_string(r + 1) + ": bad dependent node"; return -1; }
+      if (indep[u]) { err = "rigid element chains (a dependent node that is independent elsewhere) are not supported"; return -1; }
+      for (int j = 0; j < s.nndof[u]; j++)
+        if ((p.rgd_mask[r] >> j) & 1) {
+          int d = s.madof[u] - 1 + j;
+          if (s.dep_indep[d] >= 0) { err = "DOF constrained by two rigid elements"; return -1; }
+          s.dep_indep[d] = g;
+        }
+    }
+  }
   for (int i = 0; i < p.nnod; i++) {
     int st = p.status[i];
     if (st == 1) { s.nenod++; s.ndim_ext += s.nndof[i]; }
@@ -48,8 +72,10 @@ int build_sam(const Part& p, Sam& s, std::string& err) {
       if (st < 0 && ((-st) >> j) & 1) m = 0;
       else if (st == 1) m = 2;
       else m = 1;
-      // unconnected internal DOF -> fixed (SPRPRP behaviour)
-      if (m == 1 && j >= maxdn[i]) m = 0;
+      // unconnected internal DOF -> fixed (SPRPRP behaviour); the independent
+      // node of a rigid element is connected through the constraint
+      if (m == 1 && j >= maxdn[i] && !indep[i]) m = 0;
+      if (s.dep_indep[idof] >= 0) m = -1;
       s.msc[idof] = m;
     }
   }

@@ -104,16 +104,59 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
       if (m > 0) { neqd[i]++; ncls[i] = m; }
     }
   }
+  // Equation terms of every DOF: a free DOF is its own equation; a dependent
+  // DOF of a rigid element is u_g + theta_g x (x_u - x_g) (translations) or
+  // theta_g (rotations) of its independent node g; a fixed DOF has none.
+  struct Term { int node, j; double c; };
+  auto dof_terms = [&](int u, int j, Term* t) -> int {
+    int d = s.madof[u] - 1 + j;
+    if (s.msc[d] > 0) { t[0] = {u, j, 1.0}; return 1; }
+    if (s.msc[d] != -1) return 0;
+    int g = s.dep_indep[d], n = 0;
+    double r[3];
+    for (int c = 0; c < 3; c++) r[c] = p.xyz[3 * u + c] - p.xyz[3 * g + c];
+    Term cand[3];
+    int nc = 1;
+    cand[0] = {g, j, 1.0};
+    if (j == 0) { cand[1] = {g, 4, r[2]}; cand[2] = {g, 5, -r[1]}; nc = 3; }
+    if (j == 1) { cand[1] = {g, 5, r[0]}; cand[2] = {g, 3, -r[2]}; nc = 3; }
+    if (j == 2) { cand[1] = {g, 3, r[1]}; cand[2] = {g, 4, -r[0]}; nc = 3; }
+    for (int k = 0; k < nc; k++) {
+      if (cand[k].j >= s.nndof[g] || cand[k].c == 0.0) continue;
+      if (s.msc[s.madof[g] - 1 + cand[k].j] <= 0) continue;
+      t[n++] = cand[k];
+    }
+    return n;
+  };
+  // equation nodes each element couples (its own nodes with equations and
+  // the independent nodes of its dependent DOFs)
+  std::vector<int> ecptr(p.nel + 1, 0), ecn;
+  {
+    std::vector<int> stamp(nnod, -1);
+    for (int e = 0; e < p.nel; e++) {
+      const int d = elem_ndn(p.etype[e]);
+      for (int q = p.eptr[e]; q < p.eptr[e + 1]; q++) {
+        int u = p.enodes[q];
+        for (int j = 0; j < d && j < s.nndof[u]; j++) {
+          Term t[3];
+          int nt = dof_terms(u, j, t);
+          for (int k = 0; k < nt; k++)
+            if (stamp[t[k].node] != e) { stamp[t[k].node] = e; ecn.push_back(t[k].node); }
+        }
+      }
+      ecptr[e + 1] = (int)ecn.size();
+    }
+  }
   // node adjacency through elements (equation nodes only, no self loops)
   std::vector<int> nel_of(nnod + 1, 0);
   for (int e = 0; e < p.nel; e++)
-    for (int k = p.eptr[e]; k < p.eptr[e + 1]; k++) nel_of[p.enodes[k] + 1]++;
+    for (int k = ecptr[e]; k < ecptr[e + 1]; k++) nel_of[ecn[k] + 1]++;
   for (int i = 0; i < nnod; i++) nel_of[i + 1] += nel_of[i];
   std::vector<int> elist(nel_of[nnod]);
   {
     std::vector<int> fill(nel_of.begin(), nel_of.end() - 1);
     for (int e = 0; e < p.nel; e++)
-      for (int k = p.eptr[e]; k < p.eptr[e + 1]; k++) elist[fill[p.enodes[k]]++] = e;
+      for (int k = ecptr[e]; k < ecptr[e + 1]; k++) elist[fill[ecn[k]]++] = e;
   }
   std::vector<int> adjptr(nnod + 1, 0), adj;
   {
@@ -124,8 +167,8 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
       if (ncls[u]) {
         for (int k = nel_of[u]; k < nel_of[u + 1]; k++) {
           int e = elist[k];
-          for (int q = p.eptr[e]; q < p.eptr[e + 1]; q++) {
-            int v = p.enodes[q];
+          for (int q = ecptr[e]; q < ecptr[e + 1]; q++) {
+            int v = ecn[q];
             if (v != u && ncls[v] && stamp[v] != u) { stamp[v] = u; tmp.push_back(v); }
           }
         }
@@ -350,7 +393,9 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
     for (int j = 0; j < s.nndof[i]; j++)
       if (s.msc[s.madof[i] - 1 + j] > 0) drank[s.madof[i] - 1 + j] = k++;
   }
-  // contributions: two passes over (element, node pair, dof pair)
+  // contributions: two passes over (element, node pair, dof pair, term pair);
+  // a coefficient is kept only when the part has rigid elements
+  const bool coef = !p.rgd_indep.empty();
   std::vector<int> ccount(nnz + 1, 0);
   auto find_nb = [&](int u, int v) -> int {
     const int* b = &nbl[nbptr[u]];
@@ -364,35 +409,43 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
       for (int64_t k = 0; k < nnz; k++) ccount[k + 1] += ccount[k];
       y.cptr.assign(ccount.begin(), ccount.end());
       y.cidx.resize(ccount[nnz]);
+      if (coef) y.ccoef.resize(ccount[nnz]);
       fill.assign(ccount.begin(), ccount.end() - 1);
     }
     for (int e = 0; e < p.nel; e++) {
       int t = p.etype[e], d = elem_ndn(t), nn = elem_nodes(t);
       const int* en = &p.enodes[p.eptr[e]];
-      for (int a = 0; a < nn; a++) {
-        int u = en[a];
-        if (!ncls[u]) continue;
+      // equation terms of the element's local DOFs
+      Term tl[8 * 6][3];
+      int nt[8 * 6];
+      for (int a = 0; a < nn; a++)
+        for (int i = 0; i < d; i++)
+          nt[a * d + i] = i < s.nndof[en[a]] ? dof_terms(en[a], i, tl[a * d + i]) : 0;
+      for (int a = 0; a < nn; a++)
         for (int b = 0; b < nn; b++) {
-          int v = en[b];
-          if (!ncls[v]) continue;
-          int qn = find_nb(u, v);
-          int co = coloff[qn];
           int blk = a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a;
-          for (int i = 0; i < d && i < s.nndof[u]; i++) {
-            int ri = drank[s.madof[u] - 1 + i];
-            if (ri < 0) continue;
-            int r = nfirst[u] + ri;
-            for (int j = 0; j < d && j < s.nndof[v]; j++) {
-              int cj = drank[s.madof[v] - 1 + j];
-              if (cj < 0) continue;
-              int64_t k = y.rowptr[r] + co + cj;
-              if (pass == 0) { ccount[k + 1]++; continue; }
+          for (int i = 0; i < d; i++)
+            for (int j = 0; j < d; j++) {
               int off = a >= b ? (blk * d * d + i * d + j) : (blk * d * d + j * d + i);
-              y.cidx[fill[k]++] = y.elem_off[e] + off;
+              for (int ti = 0; ti < nt[a * d + i]; ti++) {
+                const Term& P = tl[a * d + i][ti];
+                int r = nfirst[P.node] + drank[s.madof[P.node] - 1 + P.j];
+                int qn = -1, co = 0;
+                for (int tj = 0; tj < nt[b * d + j]; tj++) {
+                  const Term& Q = tl[b * d + j][tj];
+                  if (qn < 0 || nbl[qn] != Q.node) {
+                    qn = find_nb(P.node, Q.node);
+                    co = coloff[qn];
+                  }
+                  int64_t k = y.rowptr[r] + co + drank[s.madof[Q.node] - 1 + Q.j];
+                  if (pass == 0) { ccount[k + 1]++; continue; }
+                  int64_t slot = fill[k]++;
+                  y.cidx[slot] = y.elem_off[e] + off;
+                  if (coef) y.ccoef[slot] = P.c * Q.c;
+                }
+              }
             }
-          }
         }
-      }
     }
   }
   // lower CSR entries -> front positions

@@ -34,6 +34,7 @@ def load_library(path=None):
     lib.fgpu_last_error.argtypes = [P]
     lib.fgpu_last_error.restype = ctypes.c_char_p
     lib.fgpu_set_part.argtypes = [P, I, P, P, I, P, P, P, P]
+    lib.fgpu_set_rigid.argtypes = [P, I, P, P, P, P]
     lib.fgpu_analyze.argtypes = [P, I, I, I]
     lib.fgpu_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_double]
     for name in ("fgpu_get_sizes",):
@@ -101,6 +102,15 @@ class GpuReducer:
         self._check(self.lib.fgpu_set_part(self.h, len(status), _ptr(xyz), _ptr(status), len(melcon),
                                            _ptr(melcon), _ptr(mpmnpc), _ptr(mmnpc), _ptr(eprop)),
                     "fgpu_set_part")
+        rbe2 = part.get("rbe2") or []
+        if rbe2:  # (independent node index, DOF digits "123456", [dependent node indices])
+            indep = np.array([g + 1 for g, _, _ in rbe2], np.int32)
+            mask = np.array([sum(1 << (int(c) - 1) for c in cm if c in "123456") for _, cm, _ in rbe2], np.int32)
+            dep = np.array([u + 1 for _, _, ds in rbe2 for u in ds], np.int32)
+            dptr = np.concatenate([[1], 1 + np.cumsum([len(ds) for _, _, ds in rbe2])]).astype(np.int32)
+            self._keep_rgd = (indep, mask, dptr, dep)
+            self._check(self.lib.fgpu_set_rigid(self.h, len(indep), _ptr(indep), _ptr(mask), _ptr(dptr), _ptr(dep)),
+                        "fgpu_set_rigid")
 
     def set_option(self, name, value):
         self._check(self.lib.fgpu_set_option(self.h, name.encode(), float(value)), "fgpu_set_option")

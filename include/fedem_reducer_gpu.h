@@ -76,6 +76,14 @@ const char* fgpu_last_error(const fgpu_reducer* h);
 int fgpu_set_part(fgpu_reducer* h, int nnod, const double* xyz, const int* status, int nel,
                   const int* melcon, const int* mpmnpc, const int* mmnpc, const double* eprop);
 
+/* Rigid elements (RBE2 / RGD, samReducerModule.f90:385-459) of the part set
+ * last: element r has independent node indep[r], constrains the local DOFs
+ * in the bitmask dofmask[r] (bit i = DOF i+1) of its dependent nodes
+ * dep[dptr[r]-1 .. dptr[r+1]-2]; all node numbers and pointers 1-based.
+ * Dependent DOFs get MSC -1 and follow u_g + theta_g x (x - x_g). */
+int fgpu_set_rigid(fgpu_reducer* h, int nrgd, const int* indep, const int* dofmask, const int* dptr,
+                   const int* dep);
+
 /* Options of the reduction: ngen (component modes kept), neval (eigenpairs
  * computed), flags bit0 = dispfile (vgi), bit1 = nomass.  Performs SAM
  * numbering (csAllocPointerArrays/csPreassemble, asmExtensionModule.f90:227,

@@ -3,7 +3,8 @@
 Nastran bulk-data subset reader for the reference's FE part fixtures
 (src/vpmReducer/vpmReducerTests/FEparts/*.nas, read in the reference by FFl,
 which is absent from the reference tree).  Cards: GRID, CQUAD4, CHEXA, CBAR,
-MAT1, PSHELL, PSOLID, PBAR, ASET, RBE2, in free (comma), small (8-column) and
+MAT1, PSHELL, PSOLID, PBAR, PBARL (TUBE), ASET, ASET1, SPC1, RBE2, FORCE,
+MOMENT, in free (comma), small (8-column) and
 large (16-column, '*') field formats.  Returns a part dict (see
 fedempy_amd.parts) plus the RBE2 constraint list.
 """
@@ -26,7 +27,10 @@ def _num(s):
     m = re.match(r"^([+-]?[0-9.]+)([+-][0-9]+)$", t)  # Nastran implied exponent 1.5-3
     if m and "E" not in t:
         t = m.group(1) + "E" + m.group(2)
-    return float(t)
+    try:
+        return float(t)
+    except ValueError:
+        return s  # a name field (e.g. the PBARL section type)
 
 
 def _cards(text):
@@ -68,12 +72,13 @@ def _cards(text):
 def read(path):
     with open(path) as f:
         text = f.read()
-    grids, elems, mats, props, aset, rbe2 = {}, [], {}, {}, [], []
-    known = {"GRID", "CHEXA", "CQUAD4", "CBAR", "MAT1", "PSHELL", "PSOLID", "PBAR", "ASET", "RBE2"}
+    grids, elems, mats, props, aset, rbe2, spc, loads = {}, [], {}, {}, [], [], {}, []
+    known = {"GRID", "CHEXA", "CQUAD4", "CBAR", "MAT1", "PSHELL", "PSOLID", "PBAR", "PBARL", "ASET", "ASET1",
+             "SPC1", "RBE2", "FORCE", "MOMENT"}
     for name, fl in _cards(text):
         if name not in known:
             continue
-        v = [_num(x) for x in fl]
+        v = [_num(x) for x in fl] + [None] * 24
         if name == "GRID":
             grids[int(v[0])] = (v[2] or 0.0, v[3] or 0.0, v[4] or 0.0)
         elif name == "CHEXA":
@@ -81,7 +86,8 @@ def read(path):
         elif name == "CQUAD4":
             elems.append((24, int(v[0]), int(v[1]), [int(x) for x in v[2:6]], None))
         elif name == "CBAR":
-            elems.append((11, int(v[0]), int(v[1]), [int(x) for x in v[2:4]], (v[4], v[5], v[6])))
+            elems.append((11, int(v[0]), int(v[1]), [int(x) for x in v[2:4]],
+                          tuple(x or 0.0 for x in (v[4], v[5], v[6]))))
         elif name == "MAT1":
             mats[int(v[0])] = dict(E=v[1], G=v[2], nu=v[3], rho=v[4] or 0.0)
         elif name == "PSHELL":
@@ -89,12 +95,33 @@ def read(path):
         elif name == "PSOLID":
             props[int(v[0])] = dict(kind="solid", mid=int(v[1]))
         elif name == "PBAR":
-            vv = v + [None] * 20
+            vv = v
             props[int(v[0])] = dict(kind="bar", mid=int(v[1]), A=vv[2], I1=vv[3], I2=vv[4], J=vv[5],
                                     nsm=vv[6] or 0.0, K1=vv[16], K2=vv[17])
+        elif name == "PBARL":
+            # TUBE: DIM1 outer radius, DIM2 inner radius; thin-tube shear area A/2
+            if str(fl[3]).strip().upper() != "TUBE":
+                raise ValueError("PBARL type %s not supported" % fl[3])
+            R, r = v[8], v[9]
+            A = np.pi * (R * R - r * r)
+            I = np.pi * (R ** 4 - r ** 4) / 4.0
+            props[int(v[0])] = dict(kind="bar", mid=int(v[1]), A=A, I1=I, I2=I, J=2 * I, nsm=0.0, K1=0.5, K2=0.5)
         elif name == "ASET":
             vals = [x for x in v if x is not None]
             aset.extend(int(vals[i]) for i in range(0, len(vals), 2))
+        elif name == "ASET1":
+            aset.extend(int(x) for x in v[1:] if x is not None)
+        elif name == "SPC1":
+            mask = sum(1 << (int(c) - 1) for c in str(fl[1]).strip() if c in "123456")
+            for x in v[2:]:
+                if x is not None:
+                    spc[int(x)] = spc.get(int(x), 0) | mask
+        elif name in ("FORCE", "MOMENT"):
+            if v[2]:
+                raise ValueError("only the basic coordinate system is supported")
+            F = v[3] or 0.0
+            loads.append((int(v[0]), int(v[1]), 0 if name == "FORCE" else 3,
+                          tuple(F * (x or 0.0) for x in v[4:7])))
         elif name == "RBE2":
             vals = [x for x in v if x is not None]
             rbe2.append((int(vals[1]), str(int(vals[2])), [int(x) for x in vals[3:]]))
@@ -102,6 +129,8 @@ def read(path):
     idx = {g: i for i, g in enumerate(ids)}
     xyz = np.array([grids[g] for g in ids], float)
     status = np.zeros(len(ids), np.int32)
+    for g, m in spc.items():
+        status[idx[g]] = -m
     for g in aset:
         status[idx[g]] = 1
     melcon, conn, eprop = [], [], []
@@ -127,5 +156,6 @@ def read(path):
                 mpmnpc=np.concatenate([[1], 1 + np.cumsum(sizes)]).astype(np.int32),
                 mmnpc=(np.concatenate([np.asarray(c) for c in conn]) + 1).astype(np.int32),
                 eprop=np.array(eprop), node_ids=np.array(ids),
-                rbe2=[(idx[gn], cm, [idx[d] for d in dep]) for gn, cm, dep in rbe2])
+                rbe2=[(idx[gn], cm, [idx[d] for d in dep]) for gn, cm, dep in rbe2],
+                loads=[(sid, idx[g], d0, f) for sid, g, d0, f in loads])
     return part

@@ -366,3 +366,36 @@ def solve_static(part, F):
     K, _, _ = assemble(p, s)
     u = spla.spsolve(K.tocsc(), np.asarray(F, float))
     return u, s
+
+
+def direct_solve_displacements(part, node_ids, gvec=None):
+    """solver.f90:30-383 on a part read from bulk data: external nodes
+    clamped, nodal loads of the first load case (plus M a for a gravity
+    vector gvec), K u = F; returns the translations of the listed node ids
+    (len x 3)."""
+    st = np.asarray(part["status"]).copy()
+    st[st == 1] = -63
+    q = dict(part, status=st)
+    s = sam_numbering(q)
+    F = np.zeros(s["neq"])
+    loads = part.get("loads") or []
+    if loads:
+        first = min(l[0] for l in loads)
+        for sid, n, d0, f in loads:
+            for k in range(3):
+                if sid == first and d0 + k < s["nndof"][n]:
+                    eq = s["meqn"][s["madof"][n] - 1 + d0 + k]
+                    if eq > 0:
+                        F[eq - 1] += f[k]
+    if gvec is not None:
+        _, M, _ = assemble(q, s)
+        F = F + M @ gravity_dirs(q, s) @ np.asarray(gvec, float)
+    u, _ = solve_static(part, F)
+    ids = list(part["node_ids"]) if "node_ids" in part else list(range(1, len(st) + 1))
+    out = np.zeros((len(node_ids), 3))
+    for r, nid in enumerate(node_ids):
+        i = ids.index(nid)
+        for k in range(min(3, s["nndof"][i])):
+            eq = s["meqn"][s["madof"][i] - 1 + k]
+            out[r, k] = u[eq - 1] if eq > 0 else 0.0
+    return out

@@ -113,3 +113,64 @@ def test_solvepartdis_matches_direct_solve(tmp_path):
             eq = s["meqn"][s["madof"][n - 1] - 1 + k]
             want[j, k] = u[eq - 1] if eq > 0 else 0.0
     assert rel(d, want) <= TOL
+
+
+def _fmx(path):
+    raw = open(path, "rb").read()
+    head, body = raw.split(b"\n", 1)
+    return np.frombuffer(body, "<f8"), head.decode()
+
+
+@pytest.mark.parametrize("name", ["CHEXA08_", "CQUAD04_", "CBAR02_"])
+def test_reference_superelement_kats_through_reducepart(name, tmp_path, golden_dir):
+    """The reference's FEparts/*.sol known answers (solveSupel.f90:120-184:
+    fix the first six reduced DOFs, solve K_red v = G g with g = (0, 9.81, 0),
+    every nonzero entry within 1e-9 absolute), from the _S.fmx and _G.fmx the
+    drop-in writes; and the files against the oracle at 1e-10."""
+    import shutil
+    from oracle import nastran
+    nas = tmp_path / (name + ".nas")
+    shutil.copy(os.path.join(golden_dir, name + ".nas"), nas)
+    print("drop-in KAT:", name, flush=True)
+    red = FedemReducer(solver_options=["-linkfile=%s" % nas])
+    assert red.run() == 0, (tmp_path / (name + ".res")).read_text()
+    red.release()
+    sk, _ = _fmx(tmp_path / (name + "_S.fmx"))
+    g, _ = _fmx(tmp_path / (name + "_G.fmx"))
+    nd = int(round(np.sqrt(sk.size)))
+    SK = sk.reshape(nd, nd, order="F")
+    G = g.reshape(nd, 3, order="F")
+    S = np.array([float(x) for x in open(os.path.join(golden_dir, name + ".sol")).read().split()])
+    v = np.linalg.solve(SK[6:, 6:], G[6:] @ np.array([0.0, 9.81, 0.0]))
+    for i, s in enumerate(S):
+        if abs(s) > 0:
+            assert abs(v[i] - s) <= 1e-9, (i, v[i], s)
+    o = orc.reduce(nastran.read(str(nas)))
+    assert rel(SK, o["SK"]) <= TOL
+    assert rel(G, o["G"]) <= TOL
+
+
+FE_SOLVE = [("03_2D_ShellElements", 29358, (0.0, 0.0123, 0.0)),
+            ("CylinderBeam", 6, (0.0, 0.3450, 0.0)),
+            ("CylinderBeam-shearY", 6, (0.3434, 0.3450, 0.0))]
+
+
+@pytest.mark.parametrize("name,node,want", FE_SOLVE)
+def test_reference_fe_solve_kats_through_solvepartdis(name, node, want, tmp_path, golden_dir):
+    """test_FEsolve.C:124-133 node displacements (1e-4 absolute) from
+    solvePartDis on the device; and against the oracle's direct solve
+    (scipy SuperLU) at 1e-8 relative: the shell part's stiffness, with its
+    small drilling terms, is ill-conditioned enough that two FP64 direct
+    solvers differ by ~5e-10 in the displacement vector."""
+    import shutil
+    nas = tmp_path / (name + ".nas")
+    shutil.copy(os.path.join(golden_dir, name + ".nas"), nas)
+    print("drop-in FE solve:", name, flush=True)
+    red = FedemReducer(solver_options=["-linkfile=%s" % nas])
+    status, d = red.solve([node])
+    assert status == 0, (tmp_path / (name + ".res")).read_text()
+    red.release()
+    assert np.max(np.abs(d[0] - np.array(want))) <= 1e-4, d
+    from oracle import nastran
+    ref = orc.direct_solve_displacements(nastran.read(str(nas)), [node])[0]
+    assert np.linalg.norm(d[0] - ref) <= 1e-8 * np.linalg.norm(ref)

@@ -84,3 +84,42 @@ def test_kat_fails_with_planted_error(golden_dir, monkeypatch):
     monkeypatch.setattr(orc, "hex8_matrices", bad)
     v, S = kat("CHEXA08_", golden_dir)
     assert abs(v[1] - S[1]) > 1e-9
+
+
+# Node displacements of the reference's direct FE solve (solvePartDis):
+# src/vpmReducer/vpmReducerTests/test_FEsolve.C:124-133, 1e-4 absolute.
+# External (ASET) nodes are clamped in the direct solve; load case = first
+# FORCE set.  (01_1D/02_1D2D/L-beam use CBEAM/PBEAM(L) cards and
+# 04_3D_VolumeElements uses 20-node CHEXA, outside the element set here.)
+FE_SOLVE = [("03_2D_ShellElements", 29358, (0.0, 0.0123, 0.0)),
+            ("CylinderBeam", 6, (0.0, 0.3450, 0.0)),
+            ("CylinderBeam-generic", 6, (0.0, 0.3450, 0.0)),
+            ("CylinderBeam-shearY", 6, (0.3434, 0.3450, 0.0)),
+            ("CylinderBeam-noshear", 6, (0.0, 0.3434, 0.0))]
+
+
+def fe_solve(name, node, golden_dir):
+    part = nastran.read(os.path.join(golden_dir, name + ".nas"))
+    return orc.direct_solve_displacements(part, [node])[0]
+
+
+@pytest.mark.parametrize("name,node,want", FE_SOLVE)
+def test_fe_solve_known_answers(name, node, want, golden_dir):
+    d = fe_solve(name, node, golden_dir)
+    assert np.max(np.abs(d - np.array(want))) <= 1e-4, d
+
+
+def test_fe_solve_kat_fails_with_planted_shear_error(golden_dir, monkeypatch):
+    """Shear factors taken as K instead of 1/K (planted in the oracle) must
+    break the Timoshenko case."""
+    real = orc.beam_matrices
+
+    def bad(xyz, pr):
+        q = np.array(pr, float).copy()
+        q[11] = 1.0 / q[11] if q[11] > 0 else 0.0
+        q[12] = 1.0 / q[12] if q[12] > 0 else 0.0
+        return real(xyz, q)
+
+    monkeypatch.setattr(orc, "beam_matrices", bad)
+    d = fe_solve("CylinderBeam-shearY", 6, golden_dir)
+    assert abs(d[1] - 0.3450) > 1e-4

@@ -38,7 +38,10 @@ def test_sam_arrays_bit_exact(k):
 
 @pytest.mark.parametrize("k", [0, 1, 2])
 def test_fronts_reproduce_dense_schur(k):
-    part = small_parts()[k]
+    check_fronts(small_parts()[k])
+
+
+def check_fronts(part):
     R = gpu.GpuReducer()
     R.set_part(part)
     z = R.analyze()
@@ -84,3 +87,22 @@ def test_fronts_reproduce_dense_schur(k):
     i1, i2 = o["meqn1"] - 1, o["meqn2"] - 1
     Sd = Kd[np.ix_(i2, i2)] - Kd[np.ix_(i2, i1)] @ np.linalg.solve(Kd[np.ix_(i1, i1)], Kd[np.ix_(i1, i2)])
     assert np.linalg.norm(S - Sd) <= 1e-12 * np.linalg.norm(Sd)
+
+
+@pytest.mark.parametrize("name", ["CHEXA08_", "CQUAD04_", "CBAR02_"])
+def test_sam_bit_exact_with_rigid_elements(name, golden_dir):
+    """Reference parts with RBE2 elements: the library's SAM arrays (MSC -1
+    on dependent DOFs) equal the oracle's, and the fronts reproduce the dense
+    Schur complement of the constrained stiffness T^T K T."""
+    import os
+    from oracle import nastran
+    part = nastran.read(os.path.join(golden_dir, name + ".nas"))
+    R = gpu.GpuReducer()
+    R.set_part(part)
+    z = R.analyze()
+    a = R.sam()
+    o = orc.sam_numbering(part)
+    for key in ("madof", "msc", "meqn", "meqn1", "meqn2", "dofPosIn2"):
+        assert np.array_equal(np.asarray(a[key]), np.asarray(o[key])), key
+    assert z["ndof1"] == o["ndof1"] and z["ndof2"] == o["ndof2"]
+    check_fronts(part)
