@@ -8,6 +8,7 @@
 // on the diagonal, and the trailing (nrow-ncol)^2 block holds the update
 // (contribution) matrix the parent adds in.
 #include <algorithm>
+#include <climits>
 #include "kernels.hpp"
 
 namespace fgpu {
@@ -781,6 +782,39 @@ __global__ __launch_bounds__(256) void k_xty_sum(int nchunk, int nab, const doub
   if (lane == 0) C[e] = s;
 }
 
+// Sign convention of the component modes: for column i of Phi (rows in
+// permuted order, ld columns), the component of largest magnitude over the
+// internal equations taken in equation order (q -> row p1[q]; the first
+// maximum wins) is made positive.  One workgroup per column.
+__global__ __launch_bounds__(256) void k_mode_signs(int n1, const int* __restrict__ p1, const double* __restrict__ Phi,
+                                                    int ld, double* __restrict__ sign) {
+  __shared__ double sv[256], sa[256];
+  __shared__ int sq[256];
+  const int i = blockIdx.x, tid = threadIdx.x;
+  double bv = 0.0, ba = -1.0;
+  int bq = INT_MAX;
+  for (int q = tid; q < n1; q += 256) {
+    const double v = Phi[(int64_t)p1[q] * ld + i], a = fabs(v);
+    if (a > ba) { ba = a; bv = v; bq = q; }  // q ascends within a thread
+  }
+  sv[tid] = bv; sa[tid] = ba; sq[tid] = bq;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) {
+      const double a2 = sa[tid + o];
+      const int q2 = sq[tid + o];
+      if (a2 > sa[tid] || (a2 == sa[tid] && q2 < sq[tid])) { sa[tid] = a2; sv[tid] = sv[tid + o]; sq[tid] = q2; }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) sign[i] = sv[0] < 0.0 ? -1.0 : 1.0;
+}
+
+__global__ void k_scale_cols(int64_t n, int nc, const double* __restrict__ sign, double* __restrict__ X) {
+  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n * nc) X[g] *= sign[g % nc];
+}
+
 // Y[r][j] = alpha*Y[r][j] + beta * sum_a X[r][a] S[a][j]; S is (na x nb) row-major.
 __global__ __launch_bounds__(256) void k_gemm_small(int n, int na, int nb, const double* __restrict__ X,
                                                     int ldx, const double* __restrict__ Sm,
@@ -892,6 +926,11 @@ void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, i
   if (n > 0 && nb > 0)
     k_gemm_small<<<nblk((int64_t)n * nb, 256), 256, sizeof(double) * na * nb, st>>>(n, na, nb, X, ldx, Sm, Y,
                                                                                    ldy, alpha, beta);
+}
+void launch_mode_signs(hipStream_t st, int n1, const int* p1, double* Phi, int64_t nrows, int nev, double* sign) {
+  if (nev <= 0) return;
+  k_mode_signs<<<nev, 256, 0, st>>>(n1, p1, Phi, nev, sign);
+  if (nrows > 0) k_scale_cols<<<nblk(nrows * nev, 256), 256, 0, st>>>(nrows, nev, sign, Phi);
 }
 void launch_identity_rows(hipStream_t st, double* X, int64_t nrows, int nrhs, int ndof1) {
   if (nrows * nrhs > 0) k_identity_rows<<<nblk(nrows * nrhs, 256), 256, 0, st>>>(X, nrows, nrhs, ndof1);

@@ -44,6 +44,10 @@ void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx,
                 double* part, double* C);
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta);
+// sign convention of the modes Phi (nrows x nev row-major): largest-magnitude
+// internal component (rows p1[0..n1-1], equation order) made positive;
+// sign: nev doubles of scratch
+void launch_mode_signs(hipStream_t st, int n1, const int* p1, double* Phi, int64_t nrows, int nev, double* sign);
 void launch_identity_rows(hipStream_t st, double* X, int64_t nrows, int nrhs, int ndof1);
 
 }  // namespace fgpu

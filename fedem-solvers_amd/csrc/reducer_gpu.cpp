@@ -15,6 +15,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include "dense_small.hpp"
 #include "kernels.hpp"
 
@@ -73,10 +74,46 @@ struct SolvePlan {
   DBuf<double> part;
 };
 
+// A captured launch sequence (HIP graph), replayed while the buffers it
+// was captured with are unchanged (key = their addresses).
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void*> key;
+  ~GraphCache() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+// Replays gc when its key matches, else captures body() on stream st (kernel
+// launches only: no memset or host-synchronous call inside) and launches it.
+template <class F>
+static hipError_t run_graph(hipStream_t st, GraphCache& gc, const std::vector<const void*>& key, F&& body) {
+  if (gc.exec && gc.key == key) return hipGraphLaunch(gc.exec, st);
+  if (gc.exec) {
+    (void)hipGraphExecDestroy(gc.exec);
+    gc.exec = nullptr;
+  }
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  body();
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(st, &g);
+  if (e != hipSuccess) return e;
+  e = hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    gc.exec = nullptr;
+    return e;
+  }
+  gc.key = key;
+  return hipGraphLaunch(gc.exec, st);
+}
+
 struct DeviceReducer::Impl {
+  GraphCache g_factor, g_lanczos;
   // part
   DBuf<double> xyz, eprop;
-  DBuf<int> eptr, enodes, hexlist, shlist, bmlist;
+  DBuf<int> eptr, enodes, hexlist, shlist, bmlist, p1;
   DBuf<int64_t> eoff;
   // structure
   DBuf<int> col0, ncol, nrow, rptr, rows, chptr, chlist, relptr, rel, rowptr, colind, cptr;
@@ -280,6 +317,11 @@ int DeviceReducer::upload(std::string& err) {
   HIPCHK(d.eptr.upload(part_.eptr));
   HIPCHK(d.enodes.upload(part_.enodes));
   HIPCHK(d.hexlist.upload(d.hexl));
+  {
+    std::vector<int> p1(sym_.ndof1);  // permuted row of every internal equation, in MEQN1 order
+    for (int q = 0; q < sym_.ndof1; q++) p1[q] = sym_.eq_to_perm[sam_.meqn1[q] - 1];
+    HIPCHK(d.p1.upload(p1));
+  }
   HIPCHK(d.shlist.upload(d.shl));
   HIPCHK(d.bmlist.upload(d.bml));
   HIPCHK(d.eoff.upload(sym_.elem_off));
@@ -452,19 +494,28 @@ int DeviceReducer::reduce(std::string& err) {
     for (auto& e : d.upd_ev) HIPCHK(hipEventCreate(&e));
   }
   int64_t ui = 0;
-  for (const Op& op : d.fops) {
-    switch (op.kind) {
-      case EXTADD: launch_extend_add(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p); break;
-      case DIAG: launch_diag(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p, d.piv.p); break;
-      case TRSM: launch_trsm(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p); break;
-      case UPDATE:
-        if (opt_.time_updates) HIPCHK(hipEventRecord(d.upd_ev[2 * ui], st));
-        launch_update(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p);
-        if (opt_.time_updates) HIPCHK(hipEventRecord(d.upd_ev[2 * ui + 1], st));
-        ui++;
-        break;
-      case INV_OFF: launch_inv_off(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p); break;
+  auto factor_ops = [&]() {
+    for (const Op& op : d.fops) {
+      switch (op.kind) {
+        case EXTADD: launch_extend_add(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p); break;
+        case DIAG: launch_diag(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p, d.piv.p); break;
+        case TRSM: launch_trsm(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p); break;
+        case UPDATE:
+          if (opt_.time_updates) (void)hipEventRecord(d.upd_ev[2 * ui], st);
+          launch_update(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p);
+          if (opt_.time_updates) (void)hipEventRecord(d.upd_ev[2 * ui + 1], st);
+          ui++;
+          break;
+        case INV_OFF: launch_inv_off(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p); break;
+      }
     }
+  };
+  if (opt_.time_updates) {
+    factor_ops();  // per-launch events need the plain stream
+  } else {
+    // the factor schedule is fixed per symbolic structure: one graph
+    HIPCHK(run_graph(st, d.g_factor, {d.ftasks.p, d.pool.p, d.ipool.p, d.ioff.p, d.piv.p, d.S.foff, d.S.rows},
+                     factor_ops));
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[3], st));
@@ -577,14 +628,57 @@ int DeviceReducer::reduce(std::string& err) {
     int nb = 0;  // number of completed blocks in H
     std::vector<double> theta, Svec, Rlast;
     bool conv = false;
+    // symmetrised block-tridiagonal projection of the first mj Lanczos vectors
+    auto projected = [&](int mj) {
+      std::vector<double> Hs((size_t)mj * mj);
+      for (int a = 0; a < mj; a++)
+        for (int b = 0; b < mj; b++) {
+          double h1 = H[(size_t)a * mcap + b], h2 = H[(size_t)b * mcap + a];
+          Hs[(size_t)a * mj + b] = (a <= b + bs && b <= a + bs) ? 0.5 * (h1 + h2) : 0.0;
+        }
+      return Hs;
+    };
+    // Ritz pairs of the projection and their relative residuals
+    // ||R_{j+1} s_last|| / |theta| (theta = 1 / (lambda - shift))
+    struct Check {
+      int mj = 0;
+      bool ok = false;
+      double worst = 0.0;
+      std::vector<double> w, V;
+    };
+    const double tol = opt_.tol_eig;
+    auto run_check = [bs, nev, tol](int mj, std::vector<double> Hs, std::vector<double> Rn) {
+      Check c;
+      c.mj = mj;
+      sym_eig_top(mj, Hs, nev, c.w, c.V);  // V column i: i-th largest theta
+      c.ok = true;
+      for (int i = 0; i < nev; i++) {
+        double th = c.w[mj - 1 - i], r2 = 0.0;
+        for (int a = 0; a < bs; a++) {
+          double s = 0;
+          for (int b = 0; b < bs; b++) s += Rn[(size_t)a * bs + b] * c.V[(size_t)(mj - bs + b) * nev + i];
+          r2 += s * s;
+        }
+        double rel = std::sqrt(r2) / std::fabs(th);
+        c.worst = std::max(c.worst, rel);
+        if (!(rel <= tol)) c.ok = false;
+      }
+      return c;
+    };
+    std::future<Check> pending;
+    Check done;
     for (int j = 0; (j + 1) * bs <= mcap; j++) {
       const int mj = (j + 1) * bs;
       // W = K_ii^{-1} (M Q_j); P_j is stored in Yl after orth_block
       HIPCHK(hipMemcpy2DAsync(d.Tmp.p, sizeof(double) * bs, d.P.p + (size_t)j * bs, sizeof(double) * mcap,
                               sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
-      run_forward(st, d, *pl, d.Tmp.p, d.Yl.p, n1, true, false);
-      launch_fill(st, d.Wl.p, (int64_t)neq * bs, 0.0);  // boundary rows stay zero
-      run_backward(st, d, *pl, d.Yl.p, d.Wl.p);
+      HIPCHK(run_graph(st, d.g_lanczos,
+                       {pl, pl->tasks.p, pl->part.p, d.Tmp.p, d.Yl.p, d.Wl.p, d.W.p, d.V.p, d.pool.p, d.ipool.p,
+                        d.S.foff, d.S.woff}, [&]() {
+                         run_forward(st, d, *pl, d.Tmp.p, d.Yl.p, n1, true, false);
+                         launch_fill(st, d.Wl.p, (int64_t)neq * bs, 0.0);  // boundary rows stay zero
+                         run_backward(st, d, *pl, d.Yl.p, d.Wl.p);
+                       }));
       // two passes of full reorthogonalisation against Q_0..Q_j
       Cfull.assign((size_t)mj * bs, 0.0);
       for (int pass = 0; pass < 2; pass++) {
@@ -603,42 +697,34 @@ int DeviceReducer::reduce(std::string& err) {
       if ((rc = orth_block(j + 1, Rn))) return rc;
       for (int a = 0; a < bs; a++)
         for (int b = 0; b < bs; b++) H[(size_t)(mj + a) * mcap + j * bs + b] = Rn[(size_t)a * bs + b];
-      // convergence test on the projected problem once enough vectors exist
-      if (mj >= std::max(2 * nev, 3 * bs) && (j % 2 == 1 || mj + 2 * bs > mcap)) {
-        std::vector<double> Hs((size_t)mj * mj);
-        for (int a = 0; a < mj; a++)
-          for (int b = 0; b < mj; b++) {
-            double h1 = H[(size_t)a * mcap + b], h2 = H[(size_t)b * mcap + a];
-            Hs[(size_t)a * mj + b] = (a <= b + bs && b <= a + bs) ? 0.5 * (h1 + h2) : 0.0;
-          }
-        std::vector<double> w, V;
-        sym_eig_top(mj, Hs, nev, w, V);  // V column i: i-th largest theta
-        bool ok = true;
-        double worst = 0.0;
-        for (int i = 0; i < nev; i++) {
-          double th = w[mj - 1 - i], r2 = 0.0;
-          for (int a = 0; a < bs; a++) {
-            double s = 0;
-            for (int b = 0; b < bs; b++) s += Rn[(size_t)a * bs + b] * V[(size_t)(mj - bs + b) * nev + i];
-            r2 += s * s;
-          }
-          double rel = std::sqrt(r2) / std::fabs(th);
-          worst = std::max(worst, rel);
-          if (!(rel <= opt_.tol_eig)) ok = false;
+      // Convergence tests on the projected problem run on a host thread while
+      // the device computes the next blocks; the first test that passes ends
+      // the iteration and its Ritz pairs are used as they are.
+      if (pending.valid() && pending.wait_for(std::chrono::seconds(0)) == std::future_status::ready) {
+        Check c = pending.get();
+        res_.lanczos_resid = c.worst;
+        if (c.ok) {
+          done = std::move(c);
+          conv = true;
+          break;
         }
-        res_.lanczos_resid = worst;
-        if (ok) { conv = true; break; }
       }
+      if (!pending.valid() && mj >= std::max(2 * nev, 3 * bs))
+        pending = std::async(std::launch::async, run_check, mj, projected(mj), Rn);
     }
-    const int m = nb * bs;
-    std::vector<double> Hs((size_t)m * m);
-    for (int a = 0; a < m; a++)
-      for (int b = 0; b < m; b++) {
-        double h1 = H[(size_t)a * mcap + b], h2 = H[(size_t)b * mcap + a];
-        Hs[(size_t)a * m + b] = (a <= b + bs && b <= a + bs) ? 0.5 * (h1 + h2) : 0.0;
-      }
+    if (!conv && pending.valid()) {
+      Check c = pending.get();
+      res_.lanczos_resid = c.worst;
+      if (c.mj == nb * bs) done = std::move(c);
+    }
+    const int m = done.mj > 0 ? done.mj : nb * bs;
     std::vector<double> w, V;
-    sym_eig_top(m, Hs, nev, w, V);
+    if (done.mj > 0) {
+      w.swap(done.w);
+      V.swap(done.V);
+    } else {
+      sym_eig_top(m, projected(m), nev, w, V);
+    }
     (void)conv;
     res_.lanczos_vectors = m;
     // Ritz vectors for the nev largest theta (lowest lambda), lambda ascending
@@ -655,22 +741,7 @@ int DeviceReducer::reduce(std::string& err) {
     HIPCHK(d.Phi.alloc((size_t)neq * nev));
     launch_gemm_small(st, neq, m, nev, d.Q.p, mcap, d.small.p, d.Phi.p, nev, 0.0, 1.0);
     // sign convention: the largest-magnitude component of every mode is positive
-    std::vector<double> ph((size_t)neq * nev);
-    HIPCHK(hipMemcpyAsync(ph.data(), d.Phi.p, sizeof(double) * ph.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    std::vector<double> sg((size_t)nev * nev, 0.0);
-    for (int i = 0; i < nev; i++) {
-      double best = -1.0, val = 1.0;
-      for (int q = 0; q < n1; q++) {  // equation order, first maximum wins
-        double v = ph[(size_t)y.eq_to_perm[sam_.meqn1[q] - 1] * nev + i];
-        if (std::fabs(v) > best) { best = std::fabs(v); val = v; }
-      }
-      sg[(size_t)i * nev + i] = val < 0 ? -1.0 : 1.0;
-    }
-    HIPCHK(hipMemcpyAsync(d.small.p, sg.data(), sizeof(double) * sg.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(d.Tmp.alloc((size_t)neq * std::max(bs, nev)));
-    launch_gemm_small(st, neq, nev, nev, d.Phi.p, nev, d.small.p, d.Tmp.p, nev, 0.0, 1.0);
-    HIPCHK(hipMemcpyAsync(d.Phi.p, d.Tmp.p, sizeof(double) * neq * nev, hipMemcpyDeviceToDevice, st));
+    launch_mode_signs(st, n1, d.p1.p, d.Phi.p, neq, nev, d.Cc.p);
   }
   HIPCHK(hipEventRecord(d.ev[5], st));
   // ---- reduced mass: Z = M [B; I], M22 = [B; I]^T Z ; modes: K11, M11, M12
