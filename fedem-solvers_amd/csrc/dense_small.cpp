@@ -98,14 +98,14 @@ void sym_eig(int n, const std::vector<double>& Ain, std::vector<double>& w, std:
       if (m == l) break;
       if (++iter > 60) break;
       double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = std::hypot(g, 1.0);
+      double r = std::sqrt(g * g + 1.0);
       g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
       double s = 1.0, c = 1.0, pp = 0.0;
       int i = m - 1;
       bool early = false;
       for (; i >= l; i--) {
         double f = s * e[i], b = c * e[i];
-        r = std::hypot(f, g);
+        r = std::sqrt(f * f + g * g);
         e[i + 1] = r;
         if (r == 0.0) {
           d[i + 1] -= pp;
@@ -152,6 +152,9 @@ namespace {
 // stored in R row k (entries k+1..n-1, unit 2-norm), beta[k] = 0 when skipped.
 void tridiag(int n, std::vector<double>& A, std::vector<double>& R, std::vector<char>& used, std::vector<double>& d,
              std::vector<double>& e) {
+  // Householder reduction working on the lower triangle only (row-major
+  // storage, A[i*n + j] with j <= i): p = A v as a symmetric product, then
+  // the rank-2 update A -= v q^T + q v^T on the lower triangle.
   R.assign((size_t)n * n, 0.0);
   used.assign(n, 0);
   std::vector<double> p(n), q(n);
@@ -171,19 +174,42 @@ void tridiag(int n, std::vector<double>& A, std::vector<double>& R, std::vector<
     vn = std::sqrt(vn);
     for (int i = k + 1; i < n; i++) v[i] /= vn;
     used[k] = 1;
-    for (int i = k; i < n; i++) {
+    // column k itself becomes (alpha, 0, ...); the trailing block k+1.. is
+    // transformed by I - 2 v v^T
+    A[(size_t)(k + 1) * n + k] = alpha;
+    for (int i = k + 2; i < n; i++) A[(size_t)i * n + k] = 0.0;
+    for (int i = k + 1; i < n; i++) p[i] = 0.0;
+    for (int i = k + 1; i < n; i++) {
       const double* Ai = &A[(size_t)i * n];
-      double s = 0.0;
-      for (int j = k + 1; j < n; j++) s += Ai[j] * v[j];
-      p[i] = 2.0 * s;
+      const double vi = v[i];
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;  // four partial sums (vectorisable)
+      int j = k + 1;
+      for (; j + 3 < i; j += 4) {
+        s0 += Ai[j] * v[j];
+        s1 += Ai[j + 1] * v[j + 1];
+        s2 += Ai[j + 2] * v[j + 2];
+        s3 += Ai[j + 3] * v[j + 3];
+        p[j] += Ai[j] * vi;
+        p[j + 1] += Ai[j + 1] * vi;
+        p[j + 2] += Ai[j + 2] * vi;
+        p[j + 3] += Ai[j + 3] * vi;
+      }
+      for (; j < i; j++) {
+        s0 += Ai[j] * v[j];
+        p[j] += Ai[j] * vi;
+      }
+      p[i] += ((s0 + s1) + (s2 + s3)) + Ai[i] * vi;
     }
     double K = 0.0;
-    for (int i = k + 1; i < n; i++) K += v[i] * p[i];
-    for (int i = k; i < n; i++) q[i] = p[i] - K * v[i];
-    for (int i = k; i < n; i++) {
+    for (int i = k + 1; i < n; i++) {
+      p[i] *= 2.0;
+      K += v[i] * p[i];
+    }
+    for (int i = k + 1; i < n; i++) q[i] = p[i] - K * v[i];
+    for (int i = k + 1; i < n; i++) {
       double* Ai = &A[(size_t)i * n];
       const double vi = v[i], qi = q[i];
-      for (int j = k; j < n; j++) Ai[j] -= vi * q[j] + qi * v[j];
+      for (int j = k + 1; j <= i; j++) Ai[j] -= vi * q[j] + qi * v[j];
     }
   }
   d.resize(n);
@@ -204,13 +230,13 @@ void tql_values(int n, std::vector<double> d, std::vector<double> e, std::vector
       }
       if (m == l || ++iter > 60) break;
       double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = std::hypot(g, 1.0);
+      double r = std::sqrt(g * g + 1.0);
       g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
       double s = 1.0, c = 1.0, pp = 0.0;
       bool early = false;
       for (int i = m - 1; i >= l; i--) {
         double f = s * e[i], b = c * e[i];
-        r = std::hypot(f, g);
+        r = std::sqrt(f * f + g * g);
         e[i + 1] = r;
         if (r == 0.0) { d[i + 1] -= pp; e[m] = 0.0; early = true; break; }
         s = f / r;

@@ -525,7 +525,7 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
                                             double* __restrict__ V, double* __restrict__ X,
                                             double* __restrict__ part, int nrhs) {
   __shared__ double sA[64][65];
-  __shared__ double sB[64][64];
+  __shared__ double sB[64][NJ];
   const SolveTask t = tasks[blockIdx.x];
   const int sn = t.sn, ld = S.nrow[sn], nc = S.ncol[sn];
   const double* F = pool + S.foff[sn];
@@ -537,15 +537,16 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
   const int r0 = t.r0, rn = t.rn;
   const double* Bsrc = MODE == 1 ? Vf : (MODE == 2 ? Wf + (int64_t)nc * nrhs : Wf);
   constexpr bool rowmajorA = (MODE == 0 || MODE == 1);
-  constexpr int CPT = NJ / 4;  // right-hand sides per thread: wv*CPT .. wv*CPT+CPT-1
+  constexpr int CPT = NJ / 4;   // right-hand sides per thread: wv*CPT .. wv*CPT+CPT-1
+  constexpr int BPT = NJ / 4;   // B-tile values loaded per thread (64 x NJ tile)
   double acc[CPT];
 #pragma unroll
   for (int u = 0; u < CPT; u++) acc[u] = 0.0;
   const int kend = t.k0 + t.kn;
-  for (int kc = t.k0; kc < kend; kc += 64) {
+  double ra[16], rb[BPT];
+  // global loads of the 64-wide chunk starting at kc (zero outside the task)
+  auto load = [&](int kc) {
     const int kn = min(64, kend - kc);
-    double ra[16], rb[16];
-    // issue every global load of the chunk before the first LDS store
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q;
@@ -560,20 +561,33 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
         else v = kk >= r ? Xi[(int64_t)kk + (int64_t)r * nc] : 0.0;
       }
       ra[q] = v;
-      const int kb = idx >> 6, jb = idx & 63;
+    }
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+      const int idx = tid + 256 * q, kb = idx / NJ, jb = idx % NJ;
       rb[q] = (kb < kn && jb < nj) ? Bsrc[(int64_t)(kc + kb) * nrhs + t.j0 + jb] : 0.0;
     }
-    __syncthreads();
+  };
+  if (t.k0 < kend) load(t.k0);
+  for (int kc = t.k0; kc < kend; kc += 64) {
+    const int kn = min(64, kend - kc);
+    __syncthreads();  // the previous chunk's products are done with the LDS tiles
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q;
       const int i = rowmajorA ? (idx & 63) : (idx >> 6);
       const int k = rowmajorA ? (idx >> 6) : (idx & 63);
       sA[i][k] = ra[q];
-      sB[idx >> 6][idx & 63] = rb[q];
+    }
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+      const int idx = tid + 256 * q;
+      sB[idx / NJ][idx % NJ] = rb[q];
     }
     __syncthreads();
-    // sB is zero beyond nj, so every thread runs the same unguarded loop
+    // the next chunk's loads are in flight while this one is multiplied
+    if (kc + 64 < kend) load(kc + 64);
+    // sB is zero beyond nj and sA beyond kn, so every thread runs the same unguarded loop
 #pragma unroll 4
     for (int k = 0; k < kn; k++) {
       const double a = sA[lane][k];
@@ -606,11 +620,21 @@ __global__ __launch_bounds__(256) void k_mv_red(const SolveTask* __restrict__ ta
   double* Wf = W + S.woff[sn] * (int64_t)nrhs;
   double* Vf = V + S.woff[sn] * (int64_t)nrhs;
   const int nj = min(64, nrhs - t.j0);
-  for (int idx = threadIdx.x; idx < t.rn * 64; idx += 256) {
-    const int i = idx >> 6, jj = idx & 63;
-    if (jj >= nj) continue;
+  for (int idx = threadIdx.x; idx < t.rn * nj; idx += 256) {
+    const int i = idx / nj, jj = idx % nj;
+    const double* p = part + (int64_t)t.k0 * 4096 + i * 64 + jj;
+    // slots summed in order; the loads of four slots are issued together
     double v = 0.0;
-    for (int q = 0; q < t.kn; q++) v += part[(int64_t)(t.k0 + q) * 4096 + i * 64 + jj];
+    int q = 0;
+    for (; q + 3 < t.kn; q += 4) {
+      const double a0 = p[(int64_t)q * 4096], a1 = p[(int64_t)(q + 1) * 4096];
+      const double a2 = p[(int64_t)(q + 2) * 4096], a3 = p[(int64_t)(q + 3) * 4096];
+      v += a0;
+      v += a1;
+      v += a2;
+      v += a3;
+    }
+    for (; q < t.kn; q++) v += p[(int64_t)q * 4096];
     mv_apply(mode, S, sn, nc, t.r0 + i, t.j0 + jj, nrhs, v, Wf, Vf, X);
   }
 }

@@ -647,23 +647,36 @@ int DeviceReducer::reduce(std::string& err) {
       std::vector<double> w, V;
     };
     const double tol = opt_.tol_eig;
-    auto run_check = [bs, nev, tol](int mj, std::vector<double> Hs, std::vector<double> Rn) {
+    // G = the M-Gram matrix of the next (unnormalised) Lanczos block: the
+    // residual of Ritz vector s is sqrt(s_last^T G s_last)
+    auto run_check = [bs, nev, tol](int mj, std::vector<double> Hs, std::vector<double> G) {
       Check c;
       c.mj = mj;
       sym_eig_top(mj, Hs, nev, c.w, c.V);  // V column i: i-th largest theta
       c.ok = true;
       for (int i = 0; i < nev; i++) {
         double th = c.w[mj - 1 - i], r2 = 0.0;
+        const double* v = &c.V[(size_t)(mj - bs) * nev + i];
         for (int a = 0; a < bs; a++) {
           double s = 0;
-          for (int b = 0; b < bs; b++) s += Rn[(size_t)a * bs + b] * c.V[(size_t)(mj - bs + b) * nev + i];
-          r2 += s * s;
+          for (int b = 0; b < bs; b++) s += G[(size_t)a * bs + b] * v[(size_t)b * nev];
+          r2 += v[(size_t)a * nev] * s;
         }
-        double rel = std::sqrt(r2) / std::fabs(th);
+        double rel = std::sqrt(std::max(r2, 0.0)) / std::fabs(th);
         c.worst = std::max(c.worst, rel);
         if (!(rel <= tol)) c.ok = false;
       }
       return c;
+    };
+    auto gram_of = [&](const std::vector<double>& Rm) {  // R^T R
+      std::vector<double> Gm((size_t)bs * bs, 0.0);
+      for (int a = 0; a < bs; a++)
+        for (int b = 0; b < bs; b++) {
+          double s = 0.0;
+          for (int k = 0; k < bs; k++) s += Rm[(size_t)k * bs + a] * Rm[(size_t)k * bs + b];
+          Gm[(size_t)a * bs + b] = s;
+        }
+      return Gm;
     };
     std::future<Check> pending;
     Check done;
@@ -710,21 +723,24 @@ int DeviceReducer::reduce(std::string& err) {
         }
       }
       if (!pending.valid() && mj >= std::max(2 * nev, 3 * bs))
-        pending = std::async(std::launch::async, run_check, mj, projected(mj), Rn);
+        pending = std::async(std::launch::async, run_check, mj, projected(mj), gram_of(Rn));
     }
-    if (!conv && pending.valid()) {
-      Check c = pending.get();
-      res_.lanczos_resid = c.worst;
-      if (c.mj == nb * bs) done = std::move(c);
+    if (!conv && pending.valid()) (void)pending.get();
+    // without a converged test the Ritz pairs come from the full basis; the
+    // M-Gram matrix of the last (reorthogonalised) block gives their residuals
+    if (!conv) {
+      launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs);
+      launch_xty(st, neq, bs, bs, d.Wl.p, bs, d.Zl.p, bs, d.part.p, d.Cc.p);
+      std::vector<double> Gl((size_t)bs * bs);
+      HIPCHK(hipMemcpyAsync(Gl.data(), d.Cc.p, sizeof(double) * bs * bs, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      done = run_check(nb * bs, projected(nb * bs), Gl);
     }
-    const int m = done.mj > 0 ? done.mj : nb * bs;
+    res_.lanczos_resid = done.worst;
+    const int m = done.mj;
     std::vector<double> w, V;
-    if (done.mj > 0) {
-      w.swap(done.w);
-      V.swap(done.V);
-    } else {
-      sym_eig_top(m, projected(m), nev, w, V);
-    }
+    w.swap(done.w);
+    V.swap(done.V);
     (void)conv;
     res_.lanczos_vectors = m;
     // Ritz vectors for the nev largest theta (lowest lambda), lambda ascending
