@@ -253,17 +253,24 @@ __device__ __forceinline__ void zero_acc(f64x4 (&acc)[2][2]) {
 }
 
 // ---------------------------------------------------------------- panel
+// Value of v in lane src (uniform src), through the scalar unit: no LDS
+// round trip and no memory wait between the broadcasts of a sweep.
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
 // k_diag task: sn, k0 = first panel column.  One wave: lane i keeps row i
-// of the kb x kb (kb <= 64) diagonal block in registers and the block is
-// factored right-looking (LDL^T), one column exchanged through LDS per step.
-// Then lane j builds column j of the inverse of the unit lower factor.  L and
-// D go back into the front, the inverse into the diagonal tile of the
-// pivot-block inverse pool.
+// of the kb x kb (kb <= 64) diagonal block and row i of the inverse of its
+// unit lower factor in registers.  Right-looking LDL^T: at step j column j
+// of L is broadcast lane by lane to update the trailing rows, and row j of
+// X = L^{-1} (complete at step j) updates the rows below it
+// (x_i -= L_ij x_j).  L and D go back into the front, X into the diagonal
+// tile of the pivot-block inverse pool.
 __global__ __launch_bounds__(64) void k_diag(const FTask* __restrict__ tasks, DevSym S,
                                              double* __restrict__ pool, const int64_t* __restrict__ ioff,
                                              double* __restrict__ ipool, int* __restrict__ piv) {
-  __shared__ double sL[64][65];
-  __shared__ double scol[64];
   const FTask t = tasks[blockIdx.x];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
   const int kb = min(64, nc - k0);
@@ -271,23 +278,28 @@ __global__ __launch_bounds__(64) void k_diag(const FTask* __restrict__ tasks, De
   double* Xg = ipool + ioff[sn];
   const int lane = threadIdx.x;
   const bool act = lane < kb;
-  double a[64];
+  double a[64], x[64];
   double myd = 1.0;
 #pragma unroll
-  for (int m = 0; m < 64; m++) a[m] = (act && m <= lane && m < kb) ? F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] : 0.0;
+  for (int m = 0; m < 64; m++) {
+    a[m] = (act && m <= lane && m < kb) ? F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] : 0.0;
+    x[m] = (m == lane) ? 1.0 : 0.0;
+  }
 #pragma unroll
   for (int j = 0; j < 64; j++) {
     if (j < kb) {
-      const double dj = __shfl(a[j], j);
+      const double dj = lane_bcast(a[j], j);
       if (lane == j) myd = dj;
       const double l = (lane > j) ? a[j] / dj : 0.0;
       if (lane > j) a[j] = l;
-      scol[lane] = l;
-      __syncthreads();
+      const double ldj = l * dj;
 #pragma unroll
-      for (int m = j + 1; m < 64; m++)
-        if (m <= lane) a[m] -= l * dj * scol[m];
-      __syncthreads();
+      for (int m = j + 1; m < 64; m++) {
+        const double lm = lane_bcast(l, m);  // L[m][j]
+        if (m <= lane) a[m] -= ldj * lm;
+      }
+#pragma unroll
+      for (int m = 0; m <= j; m++) x[m] -= l * lane_bcast(x[m], j);  // X[j][m]
     }
   }
   if (act) {
@@ -297,27 +309,10 @@ __global__ __launch_bounds__(64) void k_diag(const FTask* __restrict__ tasks, De
     }
 #pragma unroll
     for (int m = 0; m < 64; m++)
-      if (m <= lane) F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] = a[m];
-  }
-#pragma unroll
-  for (int m = 0; m < 64; m++) sL[lane][m] = (m < lane) ? a[m] : 0.0;
-  __syncthreads();
-  double x[64];
-#pragma unroll
-  for (int i = 0; i < 64; i++) {
-    double v = (i == lane) ? 1.0 : 0.0;
-    if (i > lane) {
-      double s = 0.0;
-#pragma unroll
-      for (int k = 0; k < i; k++) s += sL[i][k] * x[k];
-      v = -s;
-    }
-    x[i] = v;
-  }
-  if (act) {
-#pragma unroll
-    for (int i = 0; i < 64; i++)
-      if (i >= lane && i < kb) Xg[(int64_t)(k0 + i) + (int64_t)(k0 + lane) * nc] = x[i];
+      if (m <= lane) {
+        F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] = a[m];
+        Xg[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * nc] = x[m];
+      }
   }
 }
 

@@ -679,6 +679,7 @@ int DeviceReducer::reduce(std::string& err) {
       return Gm;
     };
     std::future<Check> pending;
+    int pending_step = 0;
     Check done;
     for (int j = 0; (j + 1) * bs <= mcap; j++) {
       const int mj = (j + 1) * bs;
@@ -711,21 +712,29 @@ int DeviceReducer::reduce(std::string& err) {
       for (int a = 0; a < bs; a++)
         for (int b = 0; b < bs; b++) H[(size_t)(mj + a) * mcap + j * bs + b] = Rn[(size_t)a * bs + b];
       // Convergence tests on the projected problem run on a host thread while
-      // the device computes the next blocks; the first test that passes ends
-      // the iteration and its Ritz pairs are used as they are.
-      if (pending.valid() && pending.wait_for(std::chrono::seconds(0)) == std::future_status::ready) {
+      // the device computes the next blocks.  A test started after block j is
+      // collected after block j + 2 whatever the timing, so where the
+      // iteration stops (and so every result bit) depends on the data only.
+      if (pending.valid() && j >= pending_step + 2) {
         Check c = pending.get();
-        res_.lanczos_resid = c.worst;
         if (c.ok) {
           done = std::move(c);
           conv = true;
           break;
         }
       }
-      if (!pending.valid() && mj >= std::max(2 * nev, 3 * bs))
+      if (!pending.valid() && mj >= std::max(2 * nev, 3 * bs)) {
         pending = std::async(std::launch::async, run_check, mj, projected(mj), gram_of(Rn));
+        pending_step = j;
+      }
     }
-    if (!conv && pending.valid()) (void)pending.get();
+    if (!conv && pending.valid()) {
+      Check c = pending.get();
+      if (c.ok) {
+        done = std::move(c);
+        conv = true;
+      }
+    }
     // without a converged test the Ritz pairs come from the full basis; the
     // M-Gram matrix of the last (reorthogonalised) block gives their residuals
     if (!conv) {
