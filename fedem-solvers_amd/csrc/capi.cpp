@@ -234,6 +234,13 @@ int fgpu_reduce_loads(fgpu_reducer* h, int nrhs, const double* F, double* Fred) 
   END_GUARD(h)
 }
 
+int fgpu_recover(fgpu_reducer* h, int nrhs, const double* Ub, const double* Fi, double* Ui) {
+  GUARD(h)
+  if (!h->reduced || nrhs <= 0 || !Ui) return -1;
+  return h->red.recover(nrhs, Ub, Fi, Ui, h->err);
+  END_GUARD(h)
+}
+
 int fgpu_get_times(const fgpu_reducer* h, double* t) {
   if (!h || !h->reduced || !t) return -1;
   const fgpu::PhaseTimes& p = h->red.times();

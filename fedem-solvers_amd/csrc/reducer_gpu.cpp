@@ -79,9 +79,12 @@ struct SolvePlan {
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> key;
-  ~GraphCache() {
+  void reset() {
     if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
   }
+  ~GraphCache() { reset(); }
 };
 
 // Replays gc when its key matches, else captures body() on stream st (kernel
@@ -360,6 +363,9 @@ int DeviceReducer::upload(std::string& err) {
   for (auto& op : d.fops)
     if (op.kind == UPDATE) upd_launches_++;
   d.plans.clear();
+  // a new structure: every captured launch sequence is stale
+  d.g_factor.reset();
+  d.g_lanczos.reset();
   // value buffers
   const int64_t nnz = y.rowptr[y.neq];
   HIPCHK(d.Kp.alloc(y.elem_pool));
@@ -510,7 +516,7 @@ int DeviceReducer::reduce(std::string& err) {
       }
     }
   };
-  if (opt_.time_updates) {
+  if (opt_.time_updates || !opt_.use_graphs) {
     factor_ops();  // per-launch events need the plain stream
   } else {
     // the factor schedule is fixed per symbolic structure: one graph
@@ -686,13 +692,15 @@ int DeviceReducer::reduce(std::string& err) {
       // W = K_ii^{-1} (M Q_j); P_j is stored in Yl after orth_block
       HIPCHK(hipMemcpy2DAsync(d.Tmp.p, sizeof(double) * bs, d.P.p + (size_t)j * bs, sizeof(double) * mcap,
                               sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
-      HIPCHK(run_graph(st, d.g_lanczos,
+      auto solve_step = [&]() {
+        run_forward(st, d, *pl, d.Tmp.p, d.Yl.p, n1, true, false);
+        launch_fill(st, d.Wl.p, (int64_t)neq * bs, 0.0);  // boundary rows stay zero
+        run_backward(st, d, *pl, d.Yl.p, d.Wl.p);
+      };
+      if (!opt_.use_graphs) solve_step();
+      else HIPCHK(run_graph(st, d.g_lanczos,
                        {pl, pl->tasks.p, pl->part.p, d.Tmp.p, d.Yl.p, d.Wl.p, d.W.p, d.V.p, d.pool.p, d.ipool.p,
-                        d.S.foff, d.S.woff}, [&]() {
-                         run_forward(st, d, *pl, d.Tmp.p, d.Yl.p, n1, true, false);
-                         launch_fill(st, d.Wl.p, (int64_t)neq * bs, 0.0);  // boundary rows stay zero
-                         run_backward(st, d, *pl, d.Yl.p, d.Wl.p);
-                       }));
+                        d.S.foff, d.S.woff}, solve_step));
       // two passes of full reorthogonalisation against Q_0..Q_j
       Cfull.assign((size_t)mj * bs, 0.0);
       for (int pass = 0; pass < 2; pass++) {
@@ -1021,6 +1029,63 @@ int DeviceReducer::reduce_loads(int nrhs, const double* F_eq, std::vector<double
   }
   (void)err;
   return 0;
+}
+
+int DeviceReducer::recover(int nrhs, const double* Ub, const double* Fi, double* Ui, std::string& err) {
+  Impl& d = *d_;
+  const Symbolic& y = sym_;
+  const int neq = y.neq, n1 = y.ndof1, n2 = y.ndof2;
+  if (nrhs <= 0) return 0;
+  std::vector<double> acc((size_t)n1 * nrhs, 0.0);
+  if (n2 > 0 && Ub) {
+    // Y = [B; I] U_b in permuted rows, the boundary block of U_b in LDS-sized
+    // column chunks
+    const int cj = std::max(1, std::min(nrhs, 8192 / std::max(n2, 1)));
+    DBuf<double> S, Yd;
+    HIPCHK(Yd.alloc((size_t)neq * cj));
+    std::vector<double> hs((size_t)n2 * cj), hy((size_t)neq * cj);
+    for (int j0 = 0; j0 < nrhs; j0 += cj) {
+      const int nj = std::min(cj, nrhs - j0);
+      for (int a = 0; a < n2; a++)
+        for (int j = 0; j < nj; j++) hs[(size_t)a * nj + j] = Ub[a + (size_t)(j0 + j) * n2];
+      std::vector<double> hsn(hs.begin(), hs.begin() + (size_t)n2 * nj);
+      HIPCHK(S.upload(hsn));
+      launch_gemm_small(st_, neq, n2, nj, d.X.p, n2, S.p, Yd.p, nj, 0.0, 1.0);
+      HIPCHK(hipMemcpyAsync(hy.data(), Yd.p, sizeof(double) * (size_t)neq * nj, hipMemcpyDeviceToHost, st_));
+      HIPCHK(hipStreamSynchronize(st_));
+      for (int q = 0; q < n1; q++) {
+        const int r = y.eq_to_perm[sam_.meqn1[q] - 1];
+        for (int j = 0; j < nj; j++) acc[q + (size_t)(j0 + j) * n1] = hy[(size_t)r * nj + j];
+      }
+    }
+  }
+  if (Fi) {
+    std::vector<double> x((size_t)n1 * nrhs);
+    int rc = solve_kii(nrhs, Fi, x.data(), err);
+    if (rc) return rc;
+    for (size_t k = 0; k < x.size(); k++) acc[k] += x[k];
+  }
+  std::copy(acc.begin(), acc.end(), Ui);
+  return 0;
+}
+
+size_t DeviceReducer::footprint_bytes() const {
+  const Symbolic& y = sym_;
+  const size_t nnz = y.rowptr.empty() ? 0 : (size_t)y.rowptr.back();
+  size_t ipool = 0, wrows = 0;
+  for (int s = 0; s < y.nsup; s++) {
+    ipool += (size_t)y.ncol[s] * y.ncol[s];
+    wrows += y.nrow[s];
+  }
+  const int nev = opt_.ngen > 0 ? std::max(opt_.neval, opt_.ngen) : 0;
+  const int bs = opt_.lanczos_block;
+  const size_t mcap = nev > 0 ? (size_t)std::max(10 * nev + 40, 16 * bs) : 0;
+  const size_t rhs = (size_t)std::max(std::max(y.ndof2, bs), 3);
+  size_t b = 8 * ((size_t)y.fpool + ipool + 2 * nnz + 2 * (size_t)y.elem_pool);  // fronts, inverses, K, M, elements
+  b += 12 * y.cidx.size() + 4 * y.cptr.size() + 8 * nnz + 4 * nnz;                // contributions, fmap, colind
+  b += 8 * (size_t)y.neq * (3 * (size_t)y.ndof2 + 2 * mcap + 8 * (size_t)bs);     // B solves and Lanczos basis
+  b += 8 * 2 * wrows * rhs;                                                        // solve work rows
+  return b;
 }
 
 }  // namespace fgpu

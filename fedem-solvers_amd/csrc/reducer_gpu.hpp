@@ -18,6 +18,7 @@ struct ReduceOptions {
   bool want_vgi = false;   // K_ii^{-1} g_i (dispfile)
   bool no_mass = false;
   bool time_updates = false;  // bracket every trailing-update launch with HIP events
+  bool use_graphs = true;     // replay the factorisation and Lanczos solve steps as HIP graphs
   uint64_t seed = 0x5eed;  // Lanczos start block
   SymbolicOptions sym;
 };
@@ -64,6 +65,12 @@ class DeviceReducer {
   // reduced load vectors [f_e + B^T f_i (dofPosIn2 order); Phi^T f_i], ndim x nrhs,
   // from full load vectors F_eq (neq x nrhs, equation order)
   int reduce_loads(int nrhs, const double* F_eq, std::vector<double>& out, std::string& err);
+  // internal displacements from boundary displacements (and internal loads):
+  // U_i = B U_b (+ K_ii^{-1} F_i); U_b ndof2 x nrhs (MEQN2 rows), F_i and U_i
+  // ndof1 x nrhs (MEQN1 rows), F_i may be null
+  int recover(int nrhs, const double* Ub, const double* Fi, double* Ui, std::string& err);
+  // device memory one reduction of the analysed part needs (bytes, estimate)
+  size_t footprint_bytes() const;
 
   const Sam& sam() const { return sam_; }
   const Symbolic& sym() const { return sym_; }

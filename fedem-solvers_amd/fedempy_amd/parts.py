@@ -237,3 +237,49 @@ def write_nastran(part, path, loads=None):
     out.append("ENDDATA")
     with open(path, "w") as fh:
         fh.write("\n".join(out) + "\n")
+
+
+def random_part(rng, target_dof=None, kinds=("plate", "hex", "beam")):
+    """One synthetic part of the batched / served mixes: type uniform over
+    shell plate (nx, ny in [10, 80]), HEX8 block (n in [6, 24] per axis) and
+    beam frame (50 to 500 elements, tube section), 2 to 16 external nodes,
+    0 to 10 component modes.  With target_dof the size is chosen to come
+    close to that DOF count (beam frames stay within 500 elements).
+    Returns (part, ngen)."""
+    kind = kinds[int(rng.integers(len(kinds)))]
+    n_ext = int(rng.integers(2, 17))
+    ngen = int(rng.integers(0, 11))
+    seed = int(rng.integers(1 << 30))
+    if kind == "plate":
+        if target_dof:
+            nx = ny = max(4, int(round(np.sqrt(target_dof / 6.0))) - 1)
+        else:
+            nx, ny = int(rng.integers(10, 81)), int(rng.integers(10, 81))
+        i, j = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+        i, j = i.ravel(), j.ravel()
+        edge = np.nonzero((i == 0) | (i == nx) | (j == 0) | (j == ny))[0]
+        ext = np.random.default_rng(seed).choice(edge, size=min(n_ext, len(edge)), replace=False)
+        th = np.exp(np.random.default_rng(seed + 1).normal(np.log(0.005), 0.3, nx * ny))
+        part = plate(nx, ny, 0.02 * nx, 0.02 * ny, thickness=th, ext_nodes=ext)
+    elif kind == "hex":
+        if target_dof:
+            n = max(3, int(round((target_dof / 3.0) ** (1.0 / 3.0))) - 1)
+            nx = ny = nz = n
+        else:
+            nx, ny, nz = (int(rng.integers(6, 25)) for _ in range(3))
+        part = hex_block(nx, ny, nz, 0.01 * nx, 0.01 * ny, 0.01 * nz, jitter=0.05, seed=seed, n_ext=n_ext)
+    else:
+        n_el = int(min(500, max(50, target_dof / 6.0))) if target_dof else int(rng.integers(50, 501))
+        part = beam_frame(n_el=n_el, seed=seed, n_ext=n_ext)
+    return part, ngen
+
+
+def random_loads(rng, part, nlc, n_nodes=4, scale=1000.0):
+    """nlc load cases of nodal forces N(0, scale) on n_nodes random nodes
+    each (three force components per node): [(node1, dof1, value), ...]."""
+    nnod = len(part["xyz"])
+    cases = []
+    for _ in range(nlc):
+        nodes = rng.choice(nnod, size=min(n_nodes, nnod), replace=False)
+        cases.append([(int(n) + 1, d + 1, float(rng.normal(0.0, scale))) for n in nodes for d in range(3)])
+    return cases

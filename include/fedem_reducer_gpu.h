@@ -132,6 +132,44 @@ int fgpu_get_gravity_internal(fgpu_reducer* h, double* gi);
  * content. */
 int fgpu_reduce_loads(fgpu_reducer* h, int nrhs, const double* F, double* Fred);
 
+/* Internal displacements on the resident factor and B (displacement
+ * recovery, displacementModule.f90:555-578): Ui = B Ub + K_ii^-1 Fi, Ub
+ * ndof2 x nrhs (MEQN2 rows, may be NULL), Fi and Ui ndof1 x nrhs (MEQN1
+ * rows, Fi may be NULL). */
+int fgpu_recover(fgpu_reducer* h, int nrhs, const double* Ub, const double* Fi, double* Ui);
+
+/* ---- continuous-batching server ----------------------------------------
+ * No counterpart in the reference (its reducer handles one part per call):
+ * the batched/served form of the same reduction.  `slots` worker threads,
+ * each with its own reducer and HIP stream, take requests in arrival order;
+ * a request starts when its estimated device footprint fits the budget
+ * next to the running ones.  After the reduction each load case gives the
+ * reduced load vector (as fgpu_reduce_loads) and the recovered internal
+ * displacements (as fgpu_recover with Fi = the case's internal loads).
+ * Results of a request are bitwise the same alone or among others. */
+typedef struct fgpu_server fgpu_server;
+int fgpu_server_create(fgpu_server** s, int slots, double budget_gb);
+void fgpu_server_destroy(fgpu_server* s);
+/* Part as fgpu_set_part, ngen component modes, nlc load cases: case c has
+ * nodal loads k in [lptr[c], lptr[c+1]) (0-based) on node lnode[k] (1-based),
+ * DOF ldof[k] (1..6), value lval[k]; ub (nub = ndof2 values per case, reduced
+ * dofPosIn2 order) the boundary displacements or NULL.  Returns the request
+ * id (>= 0) or a negative error. */
+int fgpu_server_submit(fgpu_server* s, int nnod, const double* xyz, const int* status, int nel, const int* melcon,
+                       const int* mpmnpc, const int* mmnpc, const double* eprop, int ngen, int nlc, const int* lptr,
+                       const int* lnode, const int* ldof, const double* lval, int nub, const double* ub);
+/* Blocks until request id is done; its status (0 or negative). */
+int fgpu_server_wait(fgpu_server* s, int id);
+/* sizes[0..4] = ndim, ndof1, ndof2, nlc, nev of a finished request. */
+int fgpu_server_sizes(fgpu_server* s, int id, int* sizes);
+/* SK, SM (ndim^2), G (ndim x 3), Fred (ndim x nlc), Ui (ndof1 x nlc);
+ * times[0..3] = queueing ms, service ms, latency ms, footprint bytes. */
+int fgpu_server_result(fgpu_server* s, int id, double* SK, double* SM, double* G, double* Fred, double* Ui,
+                       double* times);
+int fgpu_server_error(fgpu_server* s, int id, char* buf, int len);
+/* Frees a finished request's results. */
+int fgpu_server_release(fgpu_server* s, int id);
+
 /* Diagnostics: times[0..9] (ms) = elements, assembly, factor, B, modes,
  * mass, gravity, total, summed duration of the trailing-update (MFMA)
  * launches and their count (option "timeUpdates"); info[0..9] = nsup, nlevels, front pool doubles,
