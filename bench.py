@@ -20,6 +20,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fedem-solvers_amd"))
 
@@ -62,6 +64,81 @@ def measured_traffic():
     return None
 
 
+def batched(n_parts, slots, seed):
+    """Config-3 mix (plates, HEX8 blocks, beam frames; uneven sizes, 0-10
+    modes) through the continuous-batching server with `slots` concurrent
+    streams, and through one slot (the plain sequential GPU path)."""
+    from fedempy_amd import parts
+    from fedempy_amd.server import Server
+    rng = np.random.default_rng(seed)
+    reqs = [parts.random_part(rng) for _ in range(n_parts)]
+    out = {}
+    for k in (1, slots):
+        srv = Server(slots=k)
+        for p, g in reqs[:4]:  # warm the slots (first-touch allocations)
+            srv.result(srv.submit(p, g))
+        t0 = time.perf_counter()
+        ids = [srv.submit(p, g) for p, g in reqs]
+        for i in ids:
+            srv.result(i)
+            srv.release(i)
+        out[k] = n_parts / (time.perf_counter() - t0)
+        srv.close()
+    return {"workload": "%d parts: shell plates 10-80 squared, HEX8 blocks 6-24 cubed, beam frames 50-500 "
+                        "elements, 2-16 external nodes, 0-10 modes, FP64" % n_parts,
+            "slots": slots, "parts_per_s": out[slots], "plain_one_stream_parts_per_s": out[1],
+            "speedup": out[slots] / out[1]}
+
+
+def _drain(srv, reqs, gaps=None):
+    """Submit reqs (back to back, or at the given inter-arrival gaps) and
+    wait for all; returns (wall seconds, latencies in ms)."""
+    t0 = time.perf_counter()
+    ids, due = [], t0
+    for k, r in enumerate(reqs):
+        if gaps is not None:
+            due += gaps[k]
+            while time.perf_counter() < due:
+                time.sleep(min(0.001, max(0.0, due - time.perf_counter())))
+        ids.append(srv.submit(*r))
+    lat = []
+    for i in ids:
+        lat.append(srv.result(i)["latency_ms"])
+        srv.release(i)
+    return time.perf_counter() - t0, np.array(lat)
+
+
+def served(n_req, slots, seed):
+    """Config-4 serving mix: DOF count lognormal (median 20k, cap 300k),
+    every request a reduction plus 1-8 load cases (reduced loads and
+    recovered internal displacements).  Capacity = requests/s with the queue
+    kept full, on `slots` concurrent streams and on one (the plain sequential
+    FP64 path); latency p50/p99 under Poisson arrivals at 70 % of capacity."""
+    from fedempy_amd import parts
+    from fedempy_amd.server import Server
+    rng = np.random.default_rng(seed)
+    reqs = [parts.served_request(rng) for _ in range(n_req)]
+    cap = {}
+    for k in (1, slots):
+        srv = Server(slots=k)
+        for r in reqs[:4]:  # warm the slots
+            srv.result(srv.submit(*r))
+        el, _ = _drain(srv, reqs)
+        cap[k] = n_req / el
+        srv.close()
+    rate = 0.7 * cap[slots]
+    srv = Server(slots=slots)
+    for r in reqs[:4]:
+        srv.result(srv.submit(*r))
+    el, lat = _drain(srv, reqs, rng.exponential(1.0 / rate, n_req))
+    srv.close()
+    return {"workload": "%d requests, DOF lognormal median 20k cap 300k (mean %.0f), 1-8 load cases + "
+                        "recovery each, FP64" % (n_req, np.mean([6 * len(r[0]["xyz"]) for r in reqs])),
+            "slots": slots, "requests_per_s": cap[slots], "plain_one_stream_requests_per_s": cap[1],
+            "speedup": cap[slots] / cap[1], "poisson_rate_per_s": rate, "poisson_requests_per_s": n_req / el,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +146,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain device copy (torch.neg over N MiB of FP64) whose HBM "
                          "counters calibrate FETCH_SIZE/WRITE_SIZE of the profiled kernels")
@@ -137,6 +215,9 @@ def main():
                      "nnzL": info["nnzL"], "fronts": info["nsup"], "lanczos_vectors": nlz,
                      "lanczos_resid": resid},
     }
+    if not a.no_serving:
+        line["batched"] = batched(128, 16, a.seed)
+        line["served"] = served(160, 16, a.seed)
     if not a.no_cpu_baseline:
         rate_sample = cpu_baseline(a.seed)
         Rs = gpu.GpuReducer()

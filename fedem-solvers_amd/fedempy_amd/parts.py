@@ -242,9 +242,10 @@ def write_nastran(part, path, loads=None):
 def random_part(rng, target_dof=None, kinds=("plate", "hex", "beam")):
     """One synthetic part of the batched / served mixes: type uniform over
     shell plate (nx, ny in [10, 80]), HEX8 block (n in [6, 24] per axis) and
-    beam frame (50 to 500 elements, tube section), 2 to 16 external nodes,
+    beam frame (50 to 500 elements, tube section), 2 to 16 external nodes
+    (at least 4 for solid blocks, whose nodes have no rotations),
     0 to 10 component modes.  With target_dof the size is chosen to come
-    close to that DOF count (beam frames stay within 500 elements).
+    close to that DOF count, for every type.
     Returns (part, ngen)."""
     kind = kinds[int(rng.integers(len(kinds)))]
     n_ext = int(rng.integers(2, 17))
@@ -267,9 +268,11 @@ def random_part(rng, target_dof=None, kinds=("plate", "hex", "beam")):
             nx = ny = nz = n
         else:
             nx, ny, nz = (int(rng.integers(6, 25)) for _ in range(3))
-        part = hex_block(nx, ny, nz, 0.01 * nx, 0.01 * ny, 0.01 * nz, jitter=0.05, seed=seed, n_ext=n_ext)
+        # solid nodes carry translations only: fewer than three (non-collinear)
+        # external nodes leave a rigid rotation of the block free
+        part = hex_block(nx, ny, nz, 0.01 * nx, 0.01 * ny, 0.01 * nz, jitter=0.05, seed=seed, n_ext=max(n_ext, 4))
     else:
-        n_el = int(min(500, max(50, target_dof / 6.0))) if target_dof else int(rng.integers(50, 501))
+        n_el = int(max(50, target_dof / 6.0)) if target_dof else int(rng.integers(50, 501))
         part = beam_frame(n_el=n_el, seed=seed, n_ext=n_ext)
     return part, ngen
 
@@ -283,3 +286,17 @@ def random_loads(rng, part, nlc, n_nodes=4, scale=1000.0):
         nodes = rng.choice(nnod, size=min(n_nodes, nnod), replace=False)
         cases.append([(int(n) + 1, d + 1, float(rng.normal(0.0, scale))) for n in nodes for d in range(3)])
     return cases
+
+
+def served_request(rng, median_dof=20000, cap_dof=300000):
+    """One request of the served mix: part type as random_part, DOF count
+    lognormal (median 20k, sigma 1, capped at 300k), 1 to 8 load cases of
+    N(0, 1 kN) nodal forces and N(0, 1 mm) boundary displacements for the
+    recovery.  Returns (part, ngen, loads, ub)."""
+    dof = min(cap_dof, float(np.exp(rng.normal(np.log(median_dof), 1.0))))
+    part, ngen = random_part(rng, target_dof=max(dof, 600.0))
+    nlc = int(rng.integers(1, 9))
+    loads = random_loads(rng, part, nlc)
+    n2 = sum(6 for s in part["status"] if s == 1)
+    ub = rng.normal(0.0, 1e-3, (n2, nlc))
+    return part, ngen, loads, ub
