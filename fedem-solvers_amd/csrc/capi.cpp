@@ -207,6 +207,29 @@ int fgpu_get_vgi(const fgpu_reducer* h, double* vgi) {
   return 0;
 }
 
+int fgpu_get_fe_mass(const fgpu_reducer* h, double* smass) {
+  if (!h || !h->reduced || !smass) return -1;
+  *smass = h->red.results().mass_fe;
+  return 0;
+}
+
+int fgpu_get_ext_nodes(const fgpu_reducer* h, int* nenod, double* xyz, int* medof) {
+  if (!h || !h->analyzed || !nenod) return -1;
+  const fgpu::Sam& s = h->red.sam();
+  const fgpu::Part& p = h->red.part();
+  *nenod = s.nenod;
+  int k = 0;
+  if (medof) medof[0] = 1;
+  for (int n = 0; n < s.nnod; n++) {
+    if (p.status[n] != 1) continue;
+    if (xyz)
+      for (int c = 0; c < 3; c++) xyz[3 * k + c] = p.xyz[3 * (size_t)n + c];
+    if (medof) medof[k + 1] = medof[k] + s.nndof[n];
+    k++;
+  }
+  return 0;
+}
+
 int fgpu_solve_internal(fgpu_reducer* h, int nrhs, const double* R, double* X) {
   GUARD(h)
   if (!h->reduced || nrhs <= 0 || !R || !X) return -1;

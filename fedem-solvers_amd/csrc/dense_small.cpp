@@ -345,4 +345,44 @@ void sym_eig_top(int n, const std::vector<double>& Ain, int k, std::vector<doubl
   }
 }
 
+int sym_gen_eig_lowest(int n, const double* A, const double* B, int nval, double* lambda) {
+  // R upper with B = R^T R, from the upper triangle (column-major B(i,j), i <= j)
+  auto up = [n](const double* M, int i, int j) { return i <= j ? M[i + (size_t)j * n] : M[j + (size_t)i * n]; };
+  std::vector<double> R((size_t)n * n, 0.0);  // row-major R[i*n+j]
+  for (int j = 0; j < n; j++) {
+    double s = up(B, j, j);
+    for (int k = 0; k < j; k++) s -= R[(size_t)k * n + j] * R[(size_t)k * n + j];
+    if (!(s > 0.0)) return n + j + 1;
+    const double d = std::sqrt(s);
+    R[(size_t)j * n + j] = d;
+    for (int i = j + 1; i < n; i++) {
+      double t = up(B, j, i);
+      for (int k = 0; k < j; k++) t -= R[(size_t)k * n + j] * R[(size_t)k * n + i];
+      R[(size_t)j * n + i] = t / d;
+    }
+  }
+  std::vector<double> Ri;
+  inv_upper(n, R, Ri);
+  // C = Ri^T A Ri (row-major), T = A Ri first
+  std::vector<double> T((size_t)n * n, 0.0), C((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < n; k++) {
+      const double a = up(A, i, k);
+      if (a == 0.0) continue;
+      for (int j = k; j < n; j++) T[(size_t)i * n + j] += a * Ri[(size_t)k * n + j];
+    }
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i <= k; i++) {
+      const double r = Ri[(size_t)i * n + k];
+      if (r == 0.0) continue;
+      for (int j = 0; j < n; j++) C[(size_t)k * n + j] += r * T[(size_t)i * n + j];
+    }
+  for (int i = 0; i < n; i++)  // symmetrise the rounding
+    for (int j = 0; j < i; j++) C[(size_t)i * n + j] = C[(size_t)j * n + i] = 0.5 * (C[(size_t)i * n + j] + C[(size_t)j * n + i]);
+  std::vector<double> w, V;
+  sym_eig(n, C, w, V);
+  for (int i = 0; i < nval && i < n; i++) lambda[i] = w[i];
+  return 0;
+}
+
 }  // namespace fgpu

@@ -21,4 +21,12 @@ void sym_eig(int n, const std::vector<double>& A, std::vector<double>& w, std::v
 // orthogonal), back-transformation by the reflectors.
 void sym_eig_top(int n, const std::vector<double>& A, int k, std::vector<double>& w, std::vector<double>& V);
 
+// Lowest nval eigenvalues (ascending) of the generalized problem A z = l B z
+// with A symmetric and B symmetric positive definite (the task of LAPACK
+// DSYGVX with ITYPE 1, RANGE 'I', 1..nval): B = R^T R, C = R^-T A R^-1,
+// eigenvalues of C by sym_eig.  Both matrices n x n, only their upper
+// triangles are read.  Returns 0, or n + j when the leading minor of order j
+// of B is not positive definite (DSYGVX's INFO convention).
+int sym_gen_eig_lowest(int n, const double* A, const double* B, int nval, double* lambda);
+
 }  // namespace fgpu

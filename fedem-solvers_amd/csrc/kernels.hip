@@ -860,6 +860,37 @@ __global__ void k_identity_rows(double* __restrict__ X, int64_t nrows, int nrhs,
   X[g] = (r >= ndof1 && r - ndof1 == j) ? 1.0 : 0.0;
 }
 
+// Element mass EMASS of lhsINADD (inadd.f90:258-265): the x-translation mass
+// u_x^T M_e u_x of each element's packed node-pair blocks (diagonal blocks
+// once, off-diagonal blocks twice; entry (0,0) of a block couples the two
+// nodes' x translations).  One lane per element of one type list.
+__global__ void k_emass(int n, const int* __restrict__ elist, int nen, int d, const int64_t* __restrict__ eoff,
+                        const double* __restrict__ Mp, double* __restrict__ emass) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int e = elist[t];
+  const double* m = Mp + eoff[e];
+  double s = 0.0;
+  for (int a = 0; a < nen; a++)
+    for (int b = 0; b <= a; b++) s += (a == b ? 1.0 : 2.0) * m[(int64_t)(a * (a + 1) / 2 + b) * d * d];
+  emass[e] = s;
+}
+
+// out[0] = sum of x[0..n-1] by one workgroup in a fixed order (strided lane
+// sums, then a tree), so the total is the same bits on every run
+__global__ __launch_bounds__(256) void k_sum_fixed(int n, const double* __restrict__ x, double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -950,6 +981,13 @@ void launch_mode_signs(hipStream_t st, int n1, const int* p1, double* Phi, int64
   if (nev <= 0) return;
   k_mode_signs<<<nev, 256, 0, st>>>(n1, p1, Phi, nev, sign);
   if (nrows > 0) k_scale_cols<<<nblk(nrows * nev, 256), 256, 0, st>>>(nrows, nev, sign, Phi);
+}
+void launch_emass(hipStream_t st, int n, const int* elist, int nen, int d, const int64_t* eoff, const double* Mp,
+                  double* emass) {
+  if (n > 0) k_emass<<<nblk(n, 256), 256, 0, st>>>(n, elist, nen, d, eoff, Mp, emass);
+}
+void launch_sum_fixed(hipStream_t st, int n, const double* x, double* out) {
+  k_sum_fixed<<<1, 256, 0, st>>>(n, x, out);
 }
 void launch_identity_rows(hipStream_t st, double* X, int64_t nrows, int nrhs, int ndof1) {
   if (nrows * nrhs > 0) k_identity_rows<<<nblk(nrows * nrhs, 256), 256, 0, st>>>(X, nrows, nrhs, ndof1);

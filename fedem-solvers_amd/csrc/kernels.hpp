@@ -49,5 +49,10 @@ void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, i
 // sign: nev doubles of scratch
 void launch_mode_signs(hipStream_t st, int n1, const int* p1, double* Phi, int64_t nrows, int nev, double* sign);
 void launch_identity_rows(hipStream_t st, double* X, int64_t nrows, int nrhs, int ndof1);
+// element masses (x-translation mass of each element's mass blocks; nen
+// nodes of d DOFs) of one element type list, and their fixed-order total
+void launch_emass(hipStream_t st, int n, const int* elist, int nen, int d, const int64_t* eoff, const double* Mp,
+                  double* emass);
+void launch_sum_fixed(hipStream_t st, int n, const double* x, double* out);
 
 }  // namespace fgpu

@@ -41,6 +41,19 @@ struct Card {
   std::vector<std::string> f;
 };
 
+// free-field fields of a line, empty ones included (also after a trailing comma)
+std::vector<std::string> split_free(const std::string& ln) {
+  std::vector<std::string> r;
+  size_t a = 0;
+  while (true) {
+    size_t b = ln.find(',', a);
+    r.push_back(trim(ln.substr(a, b == std::string::npos ? std::string::npos : b - a)));
+    if (b == std::string::npos) break;
+    a = b + 1;
+  }
+  return r;
+}
+
 std::vector<Card> read_cards(std::istream& in) {
   std::vector<Card> cards;
   std::string ln;
@@ -58,10 +71,8 @@ std::vector<Card> read_cards(std::istream& in) {
       if (cards.empty()) continue;
       std::vector<std::string> add;
       if (ln.find(',') != std::string::npos) {
-        std::stringstream ss(ln);
-        std::string t;
-        std::getline(ss, t, ',');
-        while (std::getline(ss, t, ',')) add.push_back(trim(t));
+        std::vector<std::string> all = split_free(ln);
+        add.assign(all.begin() + 1, all.end());
       } else {
         add = split_fixed(ln, large ? 16 : 8);
       }
@@ -70,11 +81,9 @@ std::vector<Card> read_cards(std::istream& in) {
     }
     Card c;
     if (ln.find(',') != std::string::npos) {
-      std::stringstream ss(ln);
-      std::string t;
-      std::getline(ss, t, ',');
-      c.name = trim(t);
-      while (std::getline(ss, t, ',')) c.f.push_back(trim(t));
+      std::vector<std::string> all = split_free(ln);
+      c.name = all[0];
+      c.f.assign(all.begin() + 1, all.end());
       large = !c.name.empty() && c.name.back() == '*';
     } else {
       c.name = trim(ln.substr(0, std::min<size_t>(8, ln.size())));

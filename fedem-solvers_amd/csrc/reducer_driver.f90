@@ -182,6 +182,25 @@ module fgpu_bind
        type(c_ptr), value :: h, vgi
      end function fgpu_get_vgi
 
+     integer(c_int) function fgpu_get_fe_mass (h,smass) bind(C,name='fgpu_get_fe_mass')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: smass
+     end function fgpu_get_fe_mass
+
+     integer(c_int) function fgpu_get_ext_nodes (h,nenod,xyz,medof) bind(C,name='fgpu_get_ext_nodes')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, xyz, medof
+       integer(c_int), intent(out) :: nenod
+     end function fgpu_get_ext_nodes
+
+     integer(c_int) function fgpu_reduced_eigen (n,sk,sm,nval,lambda) bind(C,name='fgpu_reduced_eigen')
+       import :: c_int, c_double
+       integer(c_int), value :: n, nval
+       real(c_double), intent(in)  :: sk(*), sm(*)
+       real(c_double), intent(out) :: lambda(*)
+     end function fgpu_reduced_eigen
+
      integer(c_int) function fgpu_get_gravity_internal (h,gi) bind(C,name='fgpu_get_gravity_internal')
        import :: c_int, c_ptr
        type(c_ptr), value :: h, gi
@@ -345,6 +364,99 @@ contains
     if (ierr /= 0) write(lpu,"(' *** Error: Can not write ',A,' to ',A)") dtype, trim(fname)
   end subroutine write_db
 
+  !> @brief Mass, centre of gravity and inertia of the reduced part
+  !> (JCMS, cmstrs.f90:1083-1184): with R the rigid-body motions of the
+  !> external nodes (unit translations, and unit rotations about a point c),
+  !> I = R^T SM R.  The mass is the mean of the three translational masses;
+  !> the centre of gravity follows from the translation-rotation couplings
+  !> about the origin, and the inertia is reported about both points.
+  subroutine mass_props (lpu,sm,nenod,tenc,medof,rmass)
+    integer , intent(in)  :: lpu, nenod, medof(:)
+    real(dp), intent(in)  :: sm(:,:), tenc(:,:)
+    real(dp), intent(out) :: rmass
+    real(dp), allocatable :: R(:,:)
+    real(dp) :: Io(6,6), Ic(6,6), cg(3)
+    allocate(R(size(sm,1),6))
+    call rigid_modes ([0.0_dp,0.0_dp,0.0_dp])
+    Io = matmul(transpose(R),matmul(sm,R))
+    rmass = (Io(1,1)+Io(2,2)+Io(3,3)) / 3.0_dp
+    if (rmass > epsilon(1.0_dp)) then
+       cg = [Io(2,6), Io(3,4), Io(1,5)] / rmass
+    else
+       cg = 0.0_dp
+    end if
+    call rigid_modes (cg)
+    Ic = matmul(transpose(R),matmul(sm,R))
+    write(lpu,600) rmass, cg, Io(4,4), Ic(4,4), Io(5,5), Ic(5,5), Io(6,6), Ic(6,6), &
+         &         Io(4,5), Ic(4,5), Io(5,6), Ic(5,6), Io(6,4), Ic(6,4)
+600 format(//4X,'MASS AND INERTIA PROPERTIES OF THE REDUCED SYSTEM' &
+         &  /4X,49('-') &
+         & //4X,'MASS      = ',1PE14.5 &
+         &  /4X,'CG X_COOR = ',1PE14.5 /4X,'CG Y_COOR = ',1PE14.5 /4X,'CG Z_COOR = ',1PE14.5 &
+         & //4X,'               About origin      About CG' &
+         &  /4X,'IXX       = ',1PE14.5,E18.5 /4X,'IYY       = ',1PE14.5,E18.5 &
+         &  /4X,'IZZ       = ',1PE14.5,E18.5 /4X,'IXY       = ',1PE14.5,E18.5 &
+         &  /4X,'IYZ       = ',1PE14.5,E18.5 /4X,'IZX       = ',1PE14.5,E18.5)
+  contains
+    subroutine rigid_modes (c)
+      real(dp), intent(in) :: c(3)
+      real(dp) :: x(3)
+      integer  :: i, j, n
+      R = 0.0_dp
+      do i = 1, nenod
+         j = medof(i) - 1
+         n = medof(i+1) - medof(i)
+         x = tenc(:,i) - c
+         R(j+1,1) = 1.0_dp
+         R(j+2,2) = 1.0_dp
+         R(j+3,3) = 1.0_dp
+         R(j+2,4) = -x(3) ! e_x cross x
+         R(j+3,4) =  x(2)
+         R(j+1,5) =  x(3) ! e_y cross x
+         R(j+3,5) = -x(1)
+         R(j+1,6) = -x(2) ! e_z cross x
+         R(j+2,6) =  x(1)
+         if (n >= 4) R(j+4,4) = 1.0_dp
+         if (n >= 5) R(j+5,5) = 1.0_dp
+         if (n >= 6) R(j+6,6) = 1.0_dp
+      end do
+    end subroutine rigid_modes
+  end subroutine mass_props
+
+  !> @brief Free-free eigenfrequencies of the reduced part (EIGCMS,
+  !> cmstrs.f90:1215-1310): the lowest nevred eigenvalues of
+  !> SK z = lambda SM z, printed as f = sqrt(lambda)/(2 pi) (negative
+  !> eigenvalues as imaginary frequencies).  A mass matrix that is not
+  !> positive definite gives the reference's warning and no frequencies.
+  subroutine reduced_frequencies (lpu,sk,sm,nevred)
+    integer , intent(in) :: lpu, nevred
+    real(dp), intent(in) :: sk(:,:), sm(:,:)
+    real(dp), allocatable :: lambda(:)
+    integer :: i, n, rc
+    n = size(sk,1)
+    allocate(lambda(nevred))
+    rc = fgpu_reduced_eigen(int(n,c_int),sk,sm,int(nevred,c_int),lambda)
+    if (rc > n) then
+       write(lpu,"(/' *** Error: No eigenvalues can be computed.'/5X,'The leading minor of order',I6, &
+            &      ' of the mass matrix is not positive definite.'/5X, &
+            &      'Check that you have a proper mass distribution.')") rc - n
+    else if (rc /= 0) then
+       write(lpu,"(/' *** Error: Eigenvalues of the reduced system failed, rc =',I6)") rc
+    end if
+    if (rc /= 0) then
+       write(lpu,"('  ** Warning: A dynamics simulation using this superelement may fail.')")
+       return
+    end if
+    write(lpu,"(//4X,'EIGEN FREQUENCIES OF THE REDUCED SYSTEM'/4X,39('-')//4X,'MODE  FREQUENCY [Hz]')")
+    do i = 1, nevred
+       if (lambda(i) < 0.0_dp) then
+          write(lpu,"(I7,1PE17.8,'*i')") i, -sqrt(-lambda(i))*0.5_dp/pi_p
+       else
+          write(lpu,"(I7,1PE17.8)") i, sqrt(lambda(i))*0.5_dp/pi_p
+       end if
+    end do
+  end subroutine reduced_frequencies
+
 end module fgpu_bind
 
 
@@ -362,7 +474,10 @@ subroutine fgpu_reducer_run (ierr) bind(C,name='fgpu_reducer_run')
   character(len=4096) :: fname
   real(dp), allocatable, target :: sk(:,:), sm(:,:), g(:,:), B(:,:), phi(:,:), vgi(:,:)
   real(dp), allocatable, target :: F(:,:), Fred(:,:)
-  real(dp), allocatable, target :: lambda(:), evl(:)
+  real(dp), allocatable, target :: lambda(:), evl(:), tenc(:,:)
+  integer , allocatable, target :: medof(:)
+  real(dp) :: sMass, rMass
+  integer  :: nenod, nevred
 
   h = c_null_ptr
   call file_name ('resfile','.res',fname)
@@ -480,6 +595,26 @@ subroutine fgpu_reducer_run (ierr) bind(C,name='fgpu_reducer_run')
      call report (lpu,h,'Result transfer',ierr)
      goto 900
   end if
+  ! --- Mass properties of the reduced part vs the FE mass (reducer.f90:535-567)
+  sMass = 0.0_dp
+  rMass = 0.0_dp
+  if (.not. noMass) then
+     rc = fgpu_get_fe_mass(h,sMass)
+     rc = fgpu_get_ext_nodes(h,nenod,c_null_ptr,c_null_ptr)
+     allocate(tenc(3,max(nenod,1)),medof(nenod+1))
+     rc = fgpu_get_ext_nodes(h,nenod,c_loc(tenc),c_loc(medof))
+     write(lpu,"(/4X,'TOTAL MASS OF SUBSTRUCTURE :',1PD14.5)") sMass
+     call mass_props (lpu,sm,nenod,tenc,medof,rMass)
+     deallocate(tenc,medof)
+     if (abs(sMass) <= epsilon(1.0_dp)) then
+        write(lpu,"(/'  ** Note: This FE part has no mass')")
+     else if (abs(rMass-sMass) > 1.0e-6_dp*sMass) then
+        write(lpu,"(/'  ** Warning: The total mass computed from the reduced substructure matrices' &
+             &      /14X,'deviates from the mass computed during finite element assembly.' &
+             &      /14X,'Mass deviation = ',1PE12.5,'  (',2PF6.2,'%)')") rMass-sMass, abs(rMass-sMass)/sMass
+     end if
+  end if
+
   call file_name ('stiffile','_S.fmx',fname)
   call write_db (lpu,fname,'stiffness matrix',chk,sk,int(size(sk),c_int64_t),ierr)
   if (ierr /= 0) goto 900
@@ -510,6 +645,11 @@ subroutine fgpu_reducer_run (ierr) bind(C,name='fgpu_reducer_run')
      call file_name ('loadfile','_L.fmx',fname)
      call write_db (lpu,fname,'load vectors',chk,Fred,int(size(Fred),c_int64_t),ierr)
      if (ierr /= 0) goto 900
+  end if
+  ! --- Free-free frequencies of the reduced part (reducer.f90:609-650)
+  nevred = fgpu_opt_int(cs('nevred'))
+  if (nevred > 0 .and. sMass > epsilon(1.0_dp) .and. rMass > epsilon(1.0_dp)) then
+     call reduced_frequencies (lpu,sk,sm,min(nevred,ndim))
   end if
   if (.not. noGrav) then
      call file_name ('gravfile','_G.fmx',fname)

@@ -129,6 +129,7 @@ struct DeviceReducer::Impl {
   DBuf<double> Kp, Mp, Kv, Mv, pool, W, V, ipool;
   DBuf<int> ebad, piv;
   DBuf<double> X, Z, Y, part, small, acc, grav, ccoef;
+  DBuf<double> emass;  // nel element masses, then their total
   // component modes
   DBuf<double> Q, P, Wl, Zl, Yl, Cc, Phi, Tmp;
   std::vector<int> hexl, shl, bml;  // elements by type: HEX8, ANDES4, BEAM2
@@ -374,6 +375,7 @@ int DeviceReducer::upload(std::string& err) {
   HIPCHK(d.Mv.alloc(nnz));
   HIPCHK(d.pool.alloc(y.fpool));
   HIPCHK(d.ebad.alloc(part_.nel));
+  HIPCHK(d.emass.alloc(part_.nel + 1));
   HIPCHK(d.piv.alloc(2));
   // gravity directions: unit accelerations on translational equation DOFs
   std::vector<double> a((size_t)y.neq * 3, 0.0);
@@ -482,6 +484,12 @@ int DeviceReducer::reduce(std::string& err) {
                 d.Mp.p, d.ebad.p);
   launch_beam(st, (int)d.bml.size(), d.bmlist.p, d.eptr.p, d.enodes.p, d.xyz.p, d.eprop.p, d.eoff.p, d.Kp.p,
               d.Mp.p, d.ebad.p);
+  if (!opt_.no_mass) {  // total FE mass (SMASS of lhsINADD) for the reduced-mass check
+    launch_emass(st, (int)d.hexl.size(), d.hexlist.p, 8, 3, d.eoff.p, d.Mp.p, d.emass.p);
+    launch_emass(st, (int)d.shl.size(), d.shlist.p, 4, 6, d.eoff.p, d.Mp.p, d.emass.p);
+    launch_emass(st, (int)d.bml.size(), d.bmlist.p, 2, 6, d.eoff.p, d.Mp.p, d.emass.p);
+    launch_sum_fixed(st, part_.nel, d.emass.p, d.emass.p + part_.nel);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[1], st));
   // ---- assembly (CSR K, M and the K entries of the fronts)
@@ -836,6 +844,8 @@ int DeviceReducer::reduce(std::string& err) {
   if (n2 > 0) {
     HIPCHK(hipMemcpyAsync(S22.data(), d.pool.p + y.foff[schur], sizeof(double) * n2 * n2, hipMemcpyDeviceToHost, st));
   }
+  if (!opt_.no_mass)
+    HIPCHK(hipMemcpyAsync(&res_.mass_fe, d.emass.p + part_.nel, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   times_.elements = ev_ms(d.ev[0], d.ev[1]);
   times_.assembly = ev_ms(d.ev[1], d.ev[2]);

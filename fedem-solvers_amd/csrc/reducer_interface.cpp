@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 #include "../../include/fedem_reducer_gpu.h"
+#include "dense_small.hpp"
 #include "nastran.hpp"
 #include "options.hpp"
 
@@ -390,5 +391,14 @@ int fgpu_write_sam(fgpu_reducer* h, const char* file, int checksum, int link_id,
 }
 
 int fgpu_copy_error(const fgpu_reducer* h, char* buf, int len) { return to_fortran(fgpu_last_error(h), buf, len); }
+
+int fgpu_reduced_eigen(int n, const double* SK, const double* SM, int nval, double* lambda) {
+  if (n <= 0 || nval <= 0 || nval > n || !SK || !SM || !lambda) return -1;
+  try {
+    return fgpu::sym_gen_eig_lowest(n, SK, SM, nval, lambda);
+  } catch (...) {
+    return -1;
+  }
+}
 
 }  // extern "C"

@@ -224,7 +224,10 @@ def write_nastran(part, path, loads=None):
         elif t == 24:
             out.append("PSHELL,%d,%d,%s" % (e + 1, e + 1, r(pr[e, 4])))
         else:
-            out.append("PBAR,%d,%d,%s" % (e + 1, e + 1, ",".join(r(v) for v in (*pr[e, 4:8], pr[e, 13]))))
+            # PID MID A I1 I2 J NSM, blank fields up to C1..F2, then the shear
+            # factors K1 K2 (fields 17 and 18, one free-field line)
+            out.append("PBAR,%d,%d,%s%s%s,%s" % (e + 1, e + 1, ",".join(r(v) for v in (*pr[e, 4:8], pr[e, 13])),
+                                                 "," * 10, r(pr[e, 11]), r(pr[e, 12])))
     ext = [i + 1 for i in range(len(status)) if status[i] == 1]
     for k in range(0, len(ext), 6):
         out.append("ASET1,123456," + ",".join(str(v) for v in ext[k:k + 6]))

@@ -120,6 +120,15 @@ int fgpu_get_B(fgpu_reducer* h, double* B);
 int fgpu_get_phi(fgpu_reducer* h, double* phi);
 /* vgi = K_ii^-1 g_i (ndof1 x 3), the _V.fmx content (flags bit0). */
 int fgpu_get_vgi(const fgpu_reducer* h, double* vgi);
+/* Total FE mass: the sum over elements of the element mass EMASS (SMASS of
+ * lhsINADD, inadd.f90:258-265 and 319), computed on the device from the
+ * element mass matrices as the x-translation mass of each element. */
+int fgpu_get_fe_mass(const fgpu_reducer* h, double* smass);
+/* External nodes in node order, as JCMS needs them (tenc/medof of
+ * readReducerData, inputReducerModule.f90:216-233): *nenod, then (when not
+ * NULL) xyz(3, nenod) and medof(nenod+1), 1-based DOF pointers into the
+ * reduced matrices.  Call with NULL arrays first to get nenod. */
+int fgpu_get_ext_nodes(const fgpu_reducer* h, int* nenod, double* xyz, int* medof);
 /* X = K_ii^-1 R on the resident factor, R and X ndof1 x nrhs (MEQN1 rows). */
 int fgpu_solve_internal(fgpu_reducer* h, int nrhs, const double* R, double* X);
 
@@ -235,6 +244,13 @@ int fgpu_write_fmx(const char* file, const char* dtype, int checksum, const doub
 int fgpu_write_sam(fgpu_reducer* h, const char* file, int checksum, int link_id, int neval, int ngen);
 /* Last error text of a handle into a blank-padded Fortran buffer. */
 int fgpu_copy_error(const fgpu_reducer* h, char* buf, int len);
+/* Lowest nval eigenvalues (ascending) of SK z = lambda SM z for the n x n
+ * reduced matrices (column-major, upper triangles read): the
+ * solveEigenvalues / DSYGVX call of EIGCMS (cmstrs.f90:1215-1310;
+ * denseMatrixModule.f90:314-340).  Returns 0, n + j when the leading minor
+ * of order j of SM is not positive definite (DSYGVX's INFO), -1 on bad
+ * arguments.  Host arithmetic on matrices of at most a few hundred rows. */
+int fgpu_reduced_eigen(int n, const double* SK, const double* SM, int nval, double* lambda);
 /* The Fortran phase drivers themselves (ierr-style status in *ierr). */
 int fgpu_reducer_run(int* ierr);
 int fgpu_solver_run(int n, const int* nodes, double* displ, int* ierr);

@@ -224,3 +224,44 @@ def test_oracle_load_reduction_reproduces_full_solve():
     o2 = orc.reduce(p, loads=planted)
     ub2 = np.linalg.solve(SK[np.ix_(b, b)], o2["L"][:n2][b])
     assert not np.allclose(ub2, ub, rtol=1e-9, atol=0.0)
+
+
+def test_reduced_eigen_matches_generalized_eigh(lib):
+    """fgpu_reduced_eigen (the EIGCMS eigenvalue call) against scipy's
+    generalized symmetric solver on seeded SPD pairs, and DSYGVX's INFO
+    convention (n + order of the failing minor) for a singular mass."""
+    import scipy.linalg as sla
+    f = lib.fgpu_reduced_eigen
+    f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    rng = np.random.default_rng(11)
+    for n, nval in ((1, 1), (12, 12), (92, 12), (200, 30)):
+        X = rng.normal(size=(n, n))
+        A = X @ X.T + 1e-3 * np.eye(n)
+        A[0, 0] -= 2.0 * A[0, 0]  # one negative direction: indefinite stiffness is allowed
+        Y = rng.normal(size=(n, n))
+        B = Y @ Y.T + n * np.eye(n)
+        A, B = np.asfortranarray(A), np.asfortranarray(B)
+        lam = np.zeros(nval)
+        assert f(n, A.ctypes.data, B.ctypes.data, nval, lam.ctypes.data) == 0
+        ref = sla.eigh(A, B, eigvals_only=True, subset_by_index=[0, nval - 1])
+        assert np.allclose(lam, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max()), (n, lam - ref)
+    # a zero-padded DOF (fixed DOF of an external node) makes the mass singular
+    n = 6
+    B = np.eye(n)
+    B[3, 3] = 0.0
+    lam = np.zeros(2)
+    A = np.asfortranarray(np.eye(n))
+    assert f(n, A.ctypes.data, np.asfortranarray(B).ctypes.data, 2, lam.ctypes.data) == n + 4
+    assert f(n, A.ctypes.data, A.ctypes.data, n + 1, lam.ctypes.data) == -1
+
+
+def test_written_beam_and_shell_parts_roundtrip(tmp_path):
+    """The bulk-data writer keeps every property the element kernels read:
+    PBAR with its continuation (shear factors K1, K2), PSHELL thickness,
+    CBAR orientation vectors."""
+    for p in (parts.beam_frame(n_el=60, seed=3, n_ext=3), parts.config0()):
+        f = tmp_path / "part.nas"
+        parts.write_nastran(p, str(f))
+        back = nastran.read(str(f))
+        assert np.array_equal(back["xyz"], p["xyz"])
+        assert np.array_equal(back["eprop"], p["eprop"])

@@ -5,6 +5,7 @@ FedemReducer(...).run() reduces a part written as Nastran bulk data and the
 Tolerances: FP64 path, 1e-10 relative Frobenius error (matrices, B, modes,
 load vectors), displacements 1e-10 relative."""
 import os
+import re
 
 import numpy as np
 import pytest
@@ -174,3 +175,108 @@ def test_reference_fe_solve_kats_through_solvepartdis(name, node, want, tmp_path
     from oracle import nastran
     ref = orc.direct_solve_displacements(nastran.read(str(nas)), [node])[0]
     assert np.linalg.norm(d[0] - ref) <= 1e-8 * np.linalg.norm(ref)
+
+
+def _res_value(res, label):
+    for ln in res.splitlines():
+        if ln.strip().startswith(label):
+            return [float(v.replace("D", "E")) for v in re.split("[=:]", ln, 1)[1].split()]
+    raise AssertionError("%s not in .res" % label)
+
+
+def _res_frequencies(res):
+    lines = res.split("EIGEN FREQUENCIES OF THE REDUCED SYSTEM")[1].splitlines()[4:]
+    out = []
+    for ln in lines:
+        f = ln.split()
+        if len(f) != 2 or not f[0].isdigit():
+            break
+        out.append(-float(f[1][:-2]) if f[1].endswith("*i") else float(f[1]))
+    return np.array(out)
+
+
+def test_mass_properties_of_free_solid_block(tmp_path):
+    """JCMS of the drop-in: a free, jittered 8x4x4 HEX8 block (faces planar)
+    has the rigid-body mass, centre and inertia of the 0.4 x 0.2 x 0.2 m box
+    (trilinear elements reproduce rigid motions, the 27-point rule
+    integrates their kinetic energy exactly, and the static condensation
+    keeps rigid motions exact), so the FE mass, the reduced mass and the
+    inertia about the CG match the closed forms.  The external nodes of a
+    solid carry 6 DOFs of which the rotations have no mass, so the reduced
+    mass is singular and EIGCMS reports it as DSYGVX does."""
+    a, b, c, rho = 0.4, 0.2, 0.2, 7850.0
+    p = parts.hex_block(8, 4, 4, a, b, c, jitter=0.05, seed=4, n_ext=6)
+    nas = tmp_path / "free.nas"
+    parts.write_nastran(p, str(nas))
+    print("drop-in reduce: free 8x4x4 block, JCMS", flush=True)
+    red = FedemReducer(solver_options=["-linkfile=%s" % nas, "-ngen=3"])
+    assert red.run() == 0, (tmp_path / "free.res").read_text()
+    red.release()
+    res = (tmp_path / "free.res").read_text()
+    m = rho * a * b * c
+    assert abs(_res_value(res, "TOTAL MASS OF SUBSTRUCTURE")[0] - m) <= 1e-5 * m
+    assert abs(_res_value(res, "MASS      =")[0] - m) <= 1e-5 * m
+    assert "Mass deviation" not in res
+    for lab, x in (("CG X_COOR", a / 2), ("CG Y_COOR", b / 2), ("CG Z_COOR", c / 2)):
+        assert abs(_res_value(res, lab)[0] - x) <= 1e-5 * a, lab
+    for lab, I in (("IXX", m * (b * b + c * c) / 12), ("IYY", m * (a * a + c * c) / 12),
+                   ("IZZ", m * (a * a + b * b) / 12)):
+        assert abs(_res_value(res, lab)[1] - I) <= 1e-5 * I, lab
+    for lab in ("IXY", "IYZ", "IZX"):
+        assert abs(_res_value(res, lab)[1]) <= 1e-8 * m * a * a, lab
+    assert "leading minor of order     4 of the mass matrix is not positive definite" in res
+    assert "A dynamics simulation using this superelement may fail" in res
+
+
+def test_reduced_frequencies_of_free_beam_frame(tmp_path):
+    """EIGCMS of the drop-in on a free tube-beam frame (6-DOF nodes with
+    translational and rotational mass): the FE mass is rho A times the total
+    member length, the reduced mass equals it (no deviation warning), and
+    the 12 free-free frequencies match the generalized eigenvalues of the
+    oracle's reduced matrices (printed to 9 digits, compared to 1e-6): six
+    rigid-body modes, then the elastic ones."""
+    p = parts.beam_frame(n_el=120, seed=2, n_ext=5)
+    nas = tmp_path / "frame.nas"
+    parts.write_nastran(p, str(nas))
+    print("drop-in reduce: free beam frame, JCMS / EIGCMS", flush=True)
+    red = FedemReducer(solver_options=["-linkfile=%s" % nas, "-ngen=4"])
+    assert red.run() == 0, (tmp_path / "frame.res").read_text()
+    red.release()
+    res = (tmp_path / "frame.res").read_text()
+    x = p["xyz"]
+    con = p["mmnpc"].reshape(-1, 2) - 1
+    L = np.linalg.norm(x[con[:, 1]] - x[con[:, 0]], axis=1).sum()
+    A = np.pi * (0.020 ** 2 - 0.018 ** 2)
+    m = 7850.0 * A * L
+    assert abs(_res_value(res, "TOTAL MASS OF SUBSTRUCTURE")[0] - m) <= 1e-5 * m
+    assert abs(_res_value(res, "MASS      =")[0] - m) <= 1e-5 * m
+    assert "Mass deviation" not in res
+    f = _res_frequencies(res)
+    assert len(f) == 12
+    o = orc.reduce(p, ngen=4)
+    import scipy.linalg as sla
+    lam = sla.eigh(o["SK"], o["SM"], eigvals_only=True, subset_by_index=[0, 11])
+    fo = np.sign(lam) * np.sqrt(np.abs(lam)) / (2 * np.pi)
+    assert np.all(np.abs(f[:6]) <= 1e-3 * f[6])  # six rigid-body modes
+    assert np.allclose(f[6:], fo[6:], rtol=1e-6, atol=0)
+
+
+def test_clamped_plate_reports_mass_deviation(tmp_path):
+    """The cantilever plate (x = 0 edge clamped): the FE mass is rho t A and
+    the reduced mass of a clamped part deviates from it, which the reducer
+    reports as the reference does; the 12 x 12 reduced mass is positive
+    definite, so all 12 free-free frequencies are printed."""
+    p = parts.config0()
+    nas = tmp_path / "plate.nas"
+    parts.write_nastran(p, str(nas))
+    red = FedemReducer(solver_options=["-linkfile=%s" % nas])
+    assert red.run() == 0, (tmp_path / "plate.res").read_text()
+    red.release()
+    res = (tmp_path / "plate.res").read_text()
+    m = 7850.0 * 0.005 * 1.0 * 0.5
+    assert abs(_res_value(res, "TOTAL MASS OF SUBSTRUCTURE")[0] - m) <= 1e-5 * m
+    assert "Mass deviation" in res
+    o = orc.reduce(p, ngen=0)
+    import scipy.linalg as sla
+    lam = sla.eigh(o["SK"], o["SM"], eigvals_only=True)
+    assert np.allclose(_res_frequencies(res), np.sqrt(lam) / (2 * np.pi), rtol=1e-6, atol=0)
