@@ -131,6 +131,9 @@ int fgpu_set_option(fgpu_reducer* h, const char* name, double value) {
   else if (n == "seed") o.seed = (uint64_t)value;
   else if (n == "leafNodes") o.sym.leaf_nodes = (int)value;
   else if (n == "timeUpdates") o.time_updates = value != 0.0;
+  else if (n == "mixedPrecision") o.mixed = value != 0.0;
+  else if (n == "refineTol") o.ir_tol = value;
+  else if (n == "refineMaxSteps") o.ir_maxit = (int)value;
   else { h->err = "fgpu_set_option: unknown option " + n; return -1; }
   return 0;
 }
@@ -267,8 +270,8 @@ int fgpu_recover(fgpu_reducer* h, int nrhs, const double* Ub, const double* Fi, 
 int fgpu_get_times(const fgpu_reducer* h, double* t) {
   if (!h || !h->reduced || !t) return -1;
   const fgpu::PhaseTimes& p = h->red.times();
-  double v[10] = {p.elements, p.assembly, p.factor, p.bsolve, p.modes, p.mass, p.gravity, p.total,
-                  p.update_kernel_ms, (double)p.update_launches};
+  double v[11] = {p.elements, p.assembly, p.factor, p.bsolve, p.modes, p.mass, p.gravity, p.total,
+                  p.update_kernel_ms, (double)p.update_launches, (double)p.refine_steps};
   std::memcpy(t, v, sizeof v);
   return 0;
 }

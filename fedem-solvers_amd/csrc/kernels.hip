@@ -177,8 +177,10 @@ __global__ __launch_bounds__(256) void k_assemble(int64_t nnz, const int* __rest
   }
   Kv[k] = kv;
   Mv[k] = mv;
-  int64_t f = fmap[k];
-  if (f >= 0) pool[f] = kv;
+  if (pool) {  // FP64 factor: original entries straight into the fronts
+    int64_t f = fmap[k];
+    if (f >= 0) pool[f] = kv;
+  }
 }
 
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
@@ -192,12 +194,13 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
 // workgroup adds, child after child, every child update-matrix column whose
 // parent column falls in its block.  Blocks are disjoint, so no two
 // workgroups touch the same parent entry; children are added in list order.
+template <class T>
 __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ tasks, DevSym S,
-                                                    double* __restrict__ pool) {
+                                                    T* __restrict__ pool) {
   const FTask t = tasks[blockIdx.x];
   const int sn = t.sn, j0 = t.a, j1 = t.a + 16;
   const int ldp = S.nrow[sn];
-  double* P = pool + S.foff[sn];
+  T* P = pool + S.foff[sn];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int q = S.chptr[sn]; q < S.chptr[sn + 1]; q++) {
     const int c = S.chlist[q];
@@ -209,11 +212,11 @@ __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ ta
     hi = nu;
     while (lo < hi) { int m = (lo + hi) >> 1; if (rel[m] < j1) lo = m + 1; else hi = m; }
     const int b = lo;
-    const double* C = pool + S.foff[c] + (int64_t)nc * ldc + nc;
+    const T* C = pool + S.foff[c] + (int64_t)nc * ldc + nc;
     for (int jj = a + w; jj < b; jj += 4) {
       const int pc = rel[jj];
-      double* dst = P + (int64_t)pc * ldp;
-      const double* src = C + (int64_t)jj * ldc;
+      T* dst = P + (int64_t)pc * ldp;
+      const T* src = C + (int64_t)jj * ldc;
       for (int ii = jj + lane; ii < nu; ii += 64) dst[rel[ii]] += src[ii];
     }
     __syncthreads();
@@ -221,18 +224,40 @@ __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ ta
 }
 
 // ---------------------------------------------------------------- MFMA tile
-// acc[x][y] += sum_m sA[wr+16x+(lane&15)][m] * sB[wc+16y+...][m] on
-// v_mfma_f64_16x16x4f64 for a 64x64 tile split in four 32x32 wave quarters.
-// The A operand of the instruction takes the sB rows and its B operand the
-// sA rows; the 16x16 result D[p][q] sits in lane q + 16*(p % 4), register
-// p / 4, so for fixed register consecutive lanes hold consecutive rows of sA
-// (coalesced stores into column-major fronts).  Result element of
-// acc[x][y][r]: row  wr + 16x + (lane & 15),  column  wc + 16y + (lane >> 4) + 4r.
-__device__ __forceinline__ void mfma_tile(const double (*sA)[65], const double (*sB)[65], int kbr,
-                                          f64x4 (&acc)[2][2], int lane, int wr, int wc) {
+// acc[x][y] += sum_m sA[wr+16x+(lane&15)][m] * sB[wc+16y+...][m] on the
+// 16x16x4 MFMA of the storage type (v_mfma_f64_16x16x4f64 for the FP64
+// factor, v_mfma_f32_16x16x4f32 for the mixed-precision one) for a 64x64
+// tile split in four 32x32 wave quarters.  The A operand of the instruction
+// takes the sB rows and its B operand the sA rows, so for fixed register
+// consecutive lanes hold consecutive rows of sA (coalesced stores into
+// column-major fronts).  The two instructions place the 16x16 result
+// differently: D[p][q] sits in lane q + 16*(p % 4), register p / 4 for FP64
+// and in lane q + 16*(p / 4), register p % 4 for FP32.  Result element of
+// acc[x][y][r]: row wr + 16x + (lane & 15), column wc + 16y + MF<T>::col(lane >> 4, r).
+using f32x4 = __attribute__((__vector_size__(4 * sizeof(float)))) float;
+
+template <class T> struct MF;
+template <> struct MF<double> {
+  using acc = f64x4;
+  static __device__ __forceinline__ acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int col(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct MF<float> {
+  using acc = f32x4;
+  static __device__ __forceinline__ acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int col(int lk, int r) { return 4 * lk + r; }
+};
+
+template <class T>
+__device__ __forceinline__ void mfma_tile(const T (*sA)[65], const T (*sB)[65], int kbr,
+                                          typename MF<T>::acc (&acc)[2][2], int lane, int wr, int wc) {
   const int lr = lane & 15, lk = lane >> 4;
   for (int m0 = 0; m0 < kbr; m0 += 4) {
-    double ab[2], aa[2];
+    T ab[2], aa[2];
 #pragma unroll
     for (int x = 0; x < 2; x++) {
       ab[x] = sB[wc + 16 * x + lr][m0 + lk];
@@ -241,15 +266,16 @@ __device__ __forceinline__ void mfma_tile(const double (*sA)[65], const double (
 #pragma unroll
     for (int x = 0; x < 2; x++)
 #pragma unroll
-      for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[y], aa[x], acc[x][y], 0, 0, 0);
+      for (int y = 0; y < 2; y++) acc[x][y] = MF<T>::op(ab[y], aa[x], acc[x][y]);
   }
 }
 
-__device__ __forceinline__ void zero_acc(f64x4 (&acc)[2][2]) {
+template <class A>
+__device__ __forceinline__ void zero_acc(A (&acc)[2][2]) {
 #pragma unroll
   for (int x = 0; x < 2; x++)
 #pragma unroll
-    for (int y = 0; y < 2; y++) acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int y = 0; y < 2; y++) acc[x][y] = A{};
 }
 
 // ---------------------------------------------------------------- panel
@@ -267,22 +293,25 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 // of L is broadcast lane by lane to update the trailing rows, and row j of
 // X = L^{-1} (complete at step j) updates the rows below it
 // (x_i -= L_ij x_j).  L and D go back into the front, X into the diagonal
-// tile of the pivot-block inverse pool.
+// tile of the pivot-block inverse pool.  The arithmetic is FP64 for both
+// storage types (the FP32 factor of the mixed-precision path keeps its
+// diagonal blocks' factorisation in FP64 and rounds only the stored result).
+template <class T>
 __global__ __launch_bounds__(64) void k_diag(const FTask* __restrict__ tasks, DevSym S,
-                                             double* __restrict__ pool, const int64_t* __restrict__ ioff,
-                                             double* __restrict__ ipool, int* __restrict__ piv) {
+                                             T* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                             T* __restrict__ ipool, int* __restrict__ piv) {
   const FTask t = tasks[blockIdx.x];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
   const int kb = min(64, nc - k0);
-  double* F = pool + S.foff[sn];
-  double* Xg = ipool + ioff[sn];
+  T* F = pool + S.foff[sn];
+  T* Xg = ipool + ioff[sn];
   const int lane = threadIdx.x;
   const bool act = lane < kb;
   double a[64], x[64];
   double myd = 1.0;
 #pragma unroll
   for (int m = 0; m < 64; m++) {
-    a[m] = (act && m <= lane && m < kb) ? F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] : 0.0;
+    a[m] = (act && m <= lane && m < kb) ? (double)F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] : 0.0;
     x[m] = (m == lane) ? 1.0 : 0.0;
   }
 #pragma unroll
@@ -310,8 +339,8 @@ __global__ __launch_bounds__(64) void k_diag(const FTask* __restrict__ tasks, De
 #pragma unroll
     for (int m = 0; m < 64; m++)
       if (m <= lane) {
-        F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] = a[m];
-        Xg[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * nc] = x[m];
+        F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] = (T)a[m];
+        Xg[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * nc] = (T)x[m];
       }
   }
 }
@@ -319,27 +348,28 @@ __global__ __launch_bounds__(64) void k_diag(const FTask* __restrict__ tasks, De
 // k_trsm task: sn, a = 64-row tile below the diagonal block, k0 = panel.
 // L_r = F_r L_kk^{-T} D^{-1}, as an FP64 MFMA product with the inverse of the
 // diagonal block built by k_diag.
+template <class T>
 __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, DevSym S,
-                                              double* __restrict__ pool, const int64_t* __restrict__ ioff,
-                                              const double* __restrict__ ipool) {
-  __shared__ double sA[64][65];
-  __shared__ double sB[64][65];
+                                              T* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                              const T* __restrict__ ipool) {
+  __shared__ T sA[64][65];
+  __shared__ T sB[64][65];
   const FTask t = tasks[blockIdx.x];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
   const int kb = min(64, nc - k0);
   const int R0 = k0 + kb + 64 * t.a;
-  double* F = pool + S.foff[sn];
-  const double* Xg = ipool + ioff[sn];
+  T* F = pool + S.foff[sn];
+  const T* Xg = ipool + ioff[sn];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int kbr = (kb + 3) & ~3;
   {
-    double ra[16], rb[16];
+    T ra[16], rb[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
-      ra[q] = (m < kb && R0 + i < ld) ? F[(int64_t)(R0 + i) + (int64_t)(k0 + m) * ld] : 0.0;
+      ra[q] = (m < kb && R0 + i < ld) ? F[(int64_t)(R0 + i) + (int64_t)(k0 + m) * ld] : (T)0;
       // sB[j][m] = Linv_kk[j][m] (lower)
-      rb[q] = (m < kb && i < kb && i >= m) ? Xg[(int64_t)(k0 + i) + (int64_t)(k0 + m) * nc] : 0.0;
+      rb[q] = (m < kb && i < kb && i >= m) ? Xg[(int64_t)(k0 + i) + (int64_t)(k0 + m) * nc] : (T)0;
     }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -350,9 +380,9 @@ __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, D
   }
   __syncthreads();
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
-  f64x4 acc[2][2];
+  typename MF<T>::acc acc[2][2];
   zero_acc(acc);
-  mfma_tile(sA, sB, kbr, acc, lane, wr, wc);
+  mfma_tile<T>(sA, sB, kbr, acc, lane, wr, wc);
 #pragma unroll
   for (int x = 0; x < 2; x++)
 #pragma unroll
@@ -360,7 +390,7 @@ __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, D
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = R0 + wr + 16 * x + lr;
-        const int j = wc + 16 * y + lk + 4 * r;
+        const int j = wc + 16 * y + MF<T>::col(lk, r);
         if (i < ld && j < kb) F[(int64_t)i + (int64_t)(k0 + j) * ld] = acc[x][y][r] / F[(int64_t)(k0 + j) * (ld + 1)];
       }
 }
@@ -368,27 +398,28 @@ __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, D
 // ---------------------------------------------------------------- update
 // Task: sn, a = ti, b = tj (64x64 tiles of the trailing lower triangle),
 // k0 = panel.  F[I,J] -= (L_I D) L_J^T with the panel of width kb <= 64.
+template <class T>
 __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks, DevSym S,
-                                                double* __restrict__ pool) {
-  __shared__ double sI[64][65];
-  __shared__ double sJ[64][65];
+                                                T* __restrict__ pool) {
+  __shared__ T sI[64][65];
+  __shared__ T sJ[64][65];
   const FTask t = tasks[blockIdx.x];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn];
   const int kb = min(64, S.ncol[sn] - k0);
   const int t0 = k0 + kb;
   const int I0 = t0 + 64 * t.a, J0 = t0 + 64 * t.b;
-  double* F = pool + S.foff[sn];
+  T* F = pool + S.foff[sn];
   const int tid = threadIdx.x;
   const int kbr = (kb + 3) & ~3;
   {
-    double ri[16], rj[16], rd[16];
+    T ri[16], rj[16], rd[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
       const bool okm = m < kb;
-      rd[q] = okm ? F[(int64_t)(k0 + m) * (ld + 1)] : 0.0;
-      ri[q] = (okm && I0 + i < ld) ? F[(int64_t)(I0 + i) + (int64_t)(k0 + m) * ld] : 0.0;
-      rj[q] = (okm && J0 + i < ld) ? F[(int64_t)(J0 + i) + (int64_t)(k0 + m) * ld] : 0.0;
+      rd[q] = okm ? F[(int64_t)(k0 + m) * (ld + 1)] : (T)0;
+      ri[q] = (okm && I0 + i < ld) ? F[(int64_t)(I0 + i) + (int64_t)(k0 + m) * ld] : (T)0;
+      rj[q] = (okm && J0 + i < ld) ? F[(int64_t)(J0 + i) + (int64_t)(k0 + m) * ld] : (T)0;
     }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -400,9 +431,9 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
   __syncthreads();
   const int lane = tid & 63, wv = tid >> 6;
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
-  f64x4 acc[2][2];
+  typename MF<T>::acc acc[2][2];
   zero_acc(acc);
-  mfma_tile(sI, sJ, kbr, acc, lane, wr, wc);
+  mfma_tile<T>(sI, sJ, kbr, acc, lane, wr, wc);
   const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int x = 0; x < 2; x++)
@@ -411,7 +442,7 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int i = I0 + wr + 16 * x + lr;
-        const int j = J0 + wc + 16 * y + lk + 4 * r;
+        const int j = J0 + wc + 16 * y + MF<T>::col(lk, r);
         if (i < ld && j < ld && i >= j) F[(int64_t)i + (int64_t)j * ld] -= acc[x][y][r];
       }
 }
@@ -427,33 +458,34 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
 // k_inv_off task: sn, a = tile column j, b = tile row i > j:
 //   X_ij = - X_ii * sum_{k=j}^{i-1} L_ik X_kj
 // (all tiles at smaller distance i-j are complete).
+template <class T>
 __global__ __launch_bounds__(256) void k_inv_off(const FTask* __restrict__ tasks, DevSym S,
-                                                 const double* __restrict__ pool, const int64_t* __restrict__ ioff,
-                                                 double* __restrict__ ipool) {
-  __shared__ double sA[64][65];
-  __shared__ double sB[64][65];
+                                                 const T* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                                 T* __restrict__ ipool) {
+  __shared__ T sA[64][65];
+  __shared__ T sB[64][65];
   const FTask t = tasks[blockIdx.x];
   const int sn = t.sn, ld = S.nrow[sn], nc = S.ncol[sn];
   const int rj = 64 * t.a, nj = min(64, nc - rj);
   const int ri = 64 * t.b, ni = min(64, nc - ri);
-  const double* F = pool + S.foff[sn];
-  double* X = ipool + ioff[sn];
+  const T* F = pool + S.foff[sn];
+  T* X = ipool + ioff[sn];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
-  f64x4 acc[2][2];
+  typename MF<T>::acc acc[2][2];
   zero_acc(acc);
   for (int kt = t.a; kt < t.b; kt++) {
     const int rk = 64 * kt, nk = min(64, nc - rk);
-    double ra[16], rb[16];
+    T ra[16], rb[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q, r = idx & 63, m = idx >> 6;
       // sA[r][m] = L[ri+r][rk+m]  (rows of tile i, columns of tile k)
-      ra[q] = (r < ni && m < nk) ? F[(int64_t)(ri + r) + (int64_t)(rk + m) * ld] : 0.0;
+      ra[q] = (r < ni && m < nk) ? F[(int64_t)(ri + r) + (int64_t)(rk + m) * ld] : (T)0;
       // sB[c][m] = X[rk+m][rj+c]  (transposed, lower part only on the diagonal tile)
       const int c = idx >> 6, mm = idx & 63;
       const bool ok = c < nj && mm < nk && (kt > t.a || mm >= c);
-      rb[q] = ok ? X[(int64_t)(rk + mm) + (int64_t)(rj + c) * nc] : 0.0;
+      rb[q] = ok ? X[(int64_t)(rk + mm) + (int64_t)(rj + c) * nc] : (T)0;
     }
     __syncthreads();
 #pragma unroll
@@ -463,7 +495,7 @@ __global__ __launch_bounds__(256) void k_inv_off(const FTask* __restrict__ tasks
       sB[idx >> 6][idx & 63] = rb[q];
     }
     __syncthreads();
-    mfma_tile(sA, sB, 64, acc, lane, wr, wc);
+    mfma_tile<T>(sA, sB, 64, acc, lane, wr, wc);
   }
   // T = acc (rows i, cols j) -> LDS as sB[c][m] = T[m][c] for the second product
   __syncthreads();
@@ -472,21 +504,21 @@ __global__ __launch_bounds__(256) void k_inv_off(const FTask* __restrict__ tasks
 #pragma unroll
     for (int y = 0; y < 2; y++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) sB[wc + 16 * y + lk + 4 * r][wr + 16 * x + lr] = acc[x][y][r];
+      for (int r = 0; r < 4; r++) sB[wc + 16 * y + MF<T>::col(lk, r)][wr + 16 * x + lr] = acc[x][y][r];
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     int r = idx & 63, m = idx >> 6;
-    sA[r][m] = (r < ni && m < ni && r >= m) ? X[(int64_t)(ri + r) + (int64_t)(ri + m) * nc] : 0.0;
+    sA[r][m] = (r < ni && m < ni && r >= m) ? X[(int64_t)(ri + r) + (int64_t)(ri + m) * nc] : (T)0;
   }
   __syncthreads();
   zero_acc(acc);
-  mfma_tile(sA, sB, 64, acc, lane, wr, wc);
+  mfma_tile<T>(sA, sB, 64, acc, lane, wr, wc);
 #pragma unroll
   for (int x = 0; x < 2; x++)
 #pragma unroll
     for (int y = 0; y < 2; y++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int i = wr + 16 * x + lr, c = wc + 16 * y + lk + 4 * r;
+        const int i = wr + 16 * x + lr, c = wc + 16 * y + MF<T>::col(lk, r);
         if (i < ni && c < nj) X[(int64_t)(ri + i) + (int64_t)(rj + c) * nc] = -acc[x][y][r];
       }
 }
@@ -513,18 +545,18 @@ __device__ __forceinline__ void mv_apply(int mode, const DevSym& S, int sn, int 
   }
 }
 
-template <int MODE, int NJ>
+template <int MODE, int NJ, class T>
 __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks, DevSym S,
-                                            const double* __restrict__ pool, const int64_t* __restrict__ ioff,
-                                            const double* __restrict__ ipool, double* __restrict__ W,
+                                            const T* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                            const T* __restrict__ ipool, double* __restrict__ W,
                                             double* __restrict__ V, double* __restrict__ X,
                                             double* __restrict__ part, int nrhs) {
   __shared__ double sA[64][65];
   __shared__ double sB[64][NJ];
   const SolveTask t = tasks[blockIdx.x];
   const int sn = t.sn, ld = S.nrow[sn], nc = S.ncol[sn];
-  const double* F = pool + S.foff[sn];
-  const double* Xi = ipool + ioff[sn];
+  const T* F = pool + S.foff[sn];
+  const T* Xi = ipool + ioff[sn];
   double* Wf = W + S.woff[sn] * (int64_t)nrhs;
   double* Vf = V + S.woff[sn] * (int64_t)nrhs;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -550,10 +582,10 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
       const int r = r0 + i, kk = kc + k;
       double v = 0.0;
       if (i < rn && k < kn) {
-        if (MODE == 0) v = kk <= r ? Xi[(int64_t)r + (int64_t)kk * nc] : 0.0;
+        if (MODE == 0) v = kk <= r ? (double)Xi[(int64_t)r + (int64_t)kk * nc] : 0.0;
         else if (MODE == 1) v = F[(int64_t)(nc + r) + (int64_t)kk * ld];
         else if (MODE == 2) v = F[(int64_t)(nc + kk) + (int64_t)r * ld];
-        else v = kk >= r ? Xi[(int64_t)kk + (int64_t)r * nc] : 0.0;
+        else v = kk >= r ? (double)Xi[(int64_t)kk + (int64_t)r * nc] : 0.0;
       }
       ra[q] = v;
     }
@@ -675,19 +707,20 @@ __global__ __launch_bounds__(256) void k_fw_init(const SolveTask* __restrict__ t
 // Copy solved rows out to Y, task = (front, 64-row chunk): internal fronts
 // give their pivot rows (V, optionally scaled by 1/D), the Schur front its
 // rows (W) to the boundary rows of Y.
+template <class T>
 __global__ __launch_bounds__(256) void k_fw_out(const SolveTask* __restrict__ tasks, DevSym S,
-                                                const double* __restrict__ pool,
+                                                const T* __restrict__ pool,
                                                 const double* __restrict__ W, const double* __restrict__ V,
                                                 double* __restrict__ Y, int nrhs, int scale, int ndof1) {
   const SolveTask t = tasks[blockIdx.x];
   const int sn = t.sn, nc = S.ncol[sn], nr = S.nrow[sn], c0 = S.col0[sn];
-  const double* F = pool + S.foff[sn];
+  const T* F = pool + S.foff[sn];
   const double* Wf = (nc > 0 ? V : W) + S.woff[sn] * (int64_t)nrhs;
   const int base = nc > 0 ? c0 : ndof1;
   for (int idx = threadIdx.x; idx < t.rn * nrhs; idx += 256) {
     const int i = t.r0 + idx / nrhs, j = idx % nrhs;
     double v = Wf[(int64_t)i * nrhs + j];
-    if (scale && nc > 0) v /= F[(int64_t)i * (nr + 1)];
+    if (scale && nc > 0) v /= (double)F[(int64_t)i * (nr + 1)];
     Y[(int64_t)(base + i) * nrhs + j] = v;
   }
 }
@@ -891,6 +924,85 @@ __global__ __launch_bounds__(256) void k_sum_fixed(int n, const double* __restri
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// ---------------------------------------------------------------- mixed precision
+// Symmetric diagonal equilibration of the stiffness matrix for the FP32
+// factor: s[r] = 1/sqrt(K[r][r]) for every perm row r of the CSR matrix
+// (1 for a non-positive diagonal, which the factorisation then reports).
+__global__ void k_diag_scale(int n, const int* __restrict__ rowptr, const int* __restrict__ colind,
+                             const double* __restrict__ val, double* __restrict__ s) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double d = 0.0;
+  for (int k = rowptr[r]; k < rowptr[r + 1]; k++)
+    if (colind[k] == r) d = val[k];
+  s[r] = d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+}
+
+// The equilibrated stiffness s_r K_rc s_c of every lower CSR entry into its
+// FP32 front position (the FP64 factor takes the entries from k_assemble).
+__global__ void k_front_scatter32(int n, const int* __restrict__ rowptr, const int* __restrict__ colind,
+                                  const double* __restrict__ val, const double* __restrict__ s,
+                                  const int64_t* __restrict__ fmap, float* __restrict__ pool) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int k = rowptr[r]; k < rowptr[r + 1]; k++) {
+    const int64_t f = fmap[k];
+    if (f >= 0) pool[f] = (float)(s[r] * val[k] * s[colind[k]]);
+  }
+}
+
+// X[r][j] *= rs[r] (rows 0..nrows-1) and, with cs, *= cs[j] (row-major, nrhs columns)
+__global__ void k_scale_rc(int64_t nrows, int nrhs, const double* __restrict__ rs, const double* __restrict__ cs,
+                           double* __restrict__ X) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows * nrhs) return;
+  const int64_t r = g / nrhs;
+  const int j = (int)(g % nrhs);
+  double v = X[g] * rs[r];
+  if (cs) v *= cs[j];
+  X[g] = v;
+}
+
+// y[i] = 1 / x[i]
+__global__ void k_recip(int n, const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = 1.0 / x[i];
+}
+
+// Y[g] = a * X[g] + b * Y[g] over the first n entries
+__global__ void k_axpby(int64_t n, double a, const double* __restrict__ X, double b, double* __restrict__ Y) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) Y[g] = a * X[g] + b * Y[g];
+}
+
+// part[blk] = (sum x^2, sum y^2) over the block's slice of the first n
+// entries; k_sumsq_fin adds the blocks in order (deterministic norms)
+__global__ __launch_bounds__(256) void k_sumsq2(int64_t n, const double* __restrict__ x,
+                                                const double* __restrict__ y, double* __restrict__ part) {
+  __shared__ double r0[256], r1[256];
+  double a = 0.0, b = 0.0;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk, hi = min(n, lo + chunk);
+  for (int64_t i = lo + threadIdx.x; i < hi; i += 256) {
+    a += x[i] * x[i];
+    b += y[i] * y[i];
+  }
+  r0[threadIdx.x] = a;
+  r1[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r0[threadIdx.x] += r0[threadIdx.x + w];
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = r0[0];
+    part[2 * blockIdx.x + 1] = r1[0];
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -908,38 +1020,47 @@ void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t
 void launch_fill(hipStream_t st, double* p, int64_t n, double v) {
   if (n > 0) k_fill<<<(unsigned)std::min<int64_t>(nblk(n, 256), 16384), 256, 0, st>>>(p, n, v);
 }
-void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool) {
-  if (n > 0) k_extend_add<<<n, 256, 0, st>>>(t, S, pool);
-}
-void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, const int64_t* ioff,
-                 double* ipool, int* piv) {
-  if (n > 0) k_diag<<<n, 64, 0, st>>>(t, S, pool, ioff, ipool, piv);
-}
-void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, const int64_t* ioff,
-                 const double* ipool) {
-  if (n > 0) k_trsm<<<n, 256, 0, st>>>(t, S, pool, ioff, ipool);
-}
-void launch_update(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool) {
-  if (n > 0) k_update<<<n, 256, 0, st>>>(t, S, pool);
-}
-void launch_inv_off(hipStream_t st, const FTask* t, int n, const DevSym& S, const double* pool,
-                    const int64_t* ioff, double* ipool) {
-  if (n > 0) k_inv_off<<<n, 256, 0, st>>>(t, S, pool, ioff, ipool);
-}
-void launch_mv(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool, const int64_t* ioff,
-               const double* ipool, double* W, double* V, double* X, double* part, int nrhs, int mode) {
-  if (n <= 0) return;
-#define FGPU_MV(M)                                                                   \
-  if (nrhs <= 16) k_mv<M, 16><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); \
-  else k_mv<M, 64><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs);
-  switch (mode) {
-    case 0: FGPU_MV(0) break;
-    case 1: FGPU_MV(1) break;
-    case 2: FGPU_MV(2) break;
-    default: FGPU_MV(3) break;
+#define FGPU_FACTOR_LAUNCHERS(T)                                                                           \
+  void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool) {                \
+    if (n > 0) k_extend_add<T><<<n, 256, 0, st>>>(t, S, pool);                                               \
+  }                                                                                                        \
+  void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,   \
+                   T* ipool, int* piv) {                                                                   \
+    if (n > 0) k_diag<T><<<n, 64, 0, st>>>(t, S, pool, ioff, ipool, piv);                                    \
+  }                                                                                                        \
+  void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,   \
+                   const T* ipool) {                                                                       \
+    if (n > 0) k_trsm<T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool);                                        \
+  }                                                                                                        \
+  void launch_update(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool) {                    \
+    if (n > 0) k_update<T><<<n, 256, 0, st>>>(t, S, pool);                                                   \
+  }                                                                                                        \
+  void launch_inv_off(hipStream_t st, const FTask* t, int n, const DevSym& S, const T* pool,               \
+                      const int64_t* ioff, T* ipool) {                                                     \
+    if (n > 0) k_inv_off<T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool);                                     \
+  }                                                                                                        \
+  void launch_mv(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const T* pool,                \
+                 const int64_t* ioff, const T* ipool, double* W, double* V, double* X, double* part,       \
+                 int nrhs, int mode) {                                                                     \
+    if (n <= 0) return;                                                                                    \
+    switch (mode * 2 + (nrhs <= 16 ? 0 : 1)) {                                                             \
+      case 0: k_mv<0, 16, T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); break;      \
+      case 1: k_mv<0, 64, T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); break;      \
+      case 2: k_mv<1, 16, T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); break;      \
+      case 3: k_mv<1, 64, T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); break;      \
+      case 4: k_mv<2, 16, T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); break;      \
+      case 5: k_mv<2, 64, T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); break;      \
+      case 6: k_mv<3, 16, T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); break;      \
+      default: k_mv<3, 64, T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, W, V, X, part, nrhs); break;     \
+    }                                                                                                      \
+  }                                                                                                        \
+  void launch_fw_out(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const T* pool,            \
+                     const double* W, const double* V, double* Y, int nrhs, int scale, int ndof1) {        \
+    if (n > 0) k_fw_out<T><<<n, 256, 0, st>>>(t, S, pool, W, V, Y, nrhs, scale, ndof1);                      \
   }
-#undef FGPU_MV
-}
+FGPU_FACTOR_LAUNCHERS(double)
+FGPU_FACTOR_LAUNCHERS(float)
+#undef FGPU_FACTOR_LAUNCHERS
 void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, double* W, double* V, double* X,
                    const double* part, int nrhs, int mode) {
   if (n > 0) k_mv_red<<<n, 256, 0, st>>>(t, S, W, V, X, part, nrhs, mode);
@@ -947,10 +1068,6 @@ void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, d
 void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* R, double* W,
                     int nrhs, int ndof1) {
   if (n > 0) k_fw_init<<<n, 256, 0, st>>>(t, S, R, W, nrhs, ndof1);
-}
-void launch_fw_out(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
-                   const double* W, const double* V, double* Y, int nrhs, int scale, int ndof1) {
-  if (n > 0) k_fw_out<<<n, 256, 0, st>>>(t, S, pool, W, V, Y, nrhs, scale, ndof1);
 }
 void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y,
                     const double* X, double* W, int nrhs) {
@@ -988,6 +1105,25 @@ void launch_emass(hipStream_t st, int n, const int* elist, int nen, int d, const
 }
 void launch_sum_fixed(hipStream_t st, int n, const double* x, double* out) {
   k_sum_fixed<<<1, 256, 0, st>>>(n, x, out);
+}
+void launch_diag_scale(hipStream_t st, int n, const int* rowptr, const int* colind, const double* val, double* s) {
+  if (n > 0) k_diag_scale<<<nblk(n, 256), 256, 0, st>>>(n, rowptr, colind, val, s);
+}
+void launch_front_scatter32(hipStream_t st, int n, const int* rowptr, const int* colind, const double* val,
+                            const double* s, const int64_t* fmap, float* pool) {
+  if (n > 0) k_front_scatter32<<<nblk(n, 256), 256, 0, st>>>(n, rowptr, colind, val, s, fmap, pool);
+}
+void launch_scale_rc(hipStream_t st, int64_t nrows, int nrhs, const double* rs, const double* cs, double* X) {
+  if (nrows * nrhs > 0) k_scale_rc<<<nblk(nrows * nrhs, 256), 256, 0, st>>>(nrows, nrhs, rs, cs, X);
+}
+void launch_recip(hipStream_t st, int n, const double* x, double* y) {
+  if (n > 0) k_recip<<<nblk(n, 256), 256, 0, st>>>(n, x, y);
+}
+void launch_axpby(hipStream_t st, int64_t n, double a, const double* X, double b, double* Y) {
+  if (n > 0) k_axpby<<<nblk(n, 256), 256, 0, st>>>(n, a, X, b, Y);
+}
+void launch_sumsq2(hipStream_t st, int64_t n, const double* x, const double* y, double* part) {
+  k_sumsq2<<<SUMSQ_BLOCKS, 256, 0, st>>>(n, x, y, part);
 }
 void launch_identity_rows(hipStream_t st, double* X, int64_t nrows, int nrhs, int ndof1) {
   if (nrows * nrhs > 0) k_identity_rows<<<nblk(nrows * nrhs, 256), 256, 0, st>>>(X, nrows, nrhs, ndof1);

@@ -18,23 +18,31 @@ void launch_beam(hipStream_t st, int nbeam, const int* elist, const int* eptr, c
 void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* ccoef,
                      const double* Kp, const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool);
 void launch_fill(hipStream_t st, double* p, int64_t n, double v);
-void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool);
-void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, const int64_t* ioff,
-                 double* ipool, int* piv);
-void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool, const int64_t* ioff,
-                 const double* ipool);
-void launch_update(hipStream_t st, const FTask* t, int n, const DevSym& S, double* pool);
-void launch_inv_off(hipStream_t st, const FTask* t, int n, const DevSym& S, const double* pool,
-                    const int64_t* ioff, double* ipool);
-// solve GEMM tasks, mode 0 FW_INV, 1 FW_UPD, 2 BW_UPD, 3 BW_INV
-void launch_mv(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool, const int64_t* ioff,
-               const double* ipool, double* W, double* V, double* X, double* part, int nrhs, int mode);
+// Factorisation and solve launchers for both factor storage types: double
+// (FP64 path) and float (mixed precision: FP32 fronts and FP32 MFMA updates,
+// diagonal blocks factored in FP64; solve vectors stay FP64).
+#define FGPU_FACTOR_DECLS(T)                                                                              \
+  void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool);                \
+  void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,  \
+                   T* ipool, int* piv);                                                                   \
+  void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,  \
+                   const T* ipool);                                                                       \
+  void launch_update(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool);                    \
+  void launch_inv_off(hipStream_t st, const FTask* t, int n, const DevSym& S, const T* pool,              \
+                      const int64_t* ioff, T* ipool);                                                     \
+  /* solve GEMM tasks, mode 0 FW_INV, 1 FW_UPD, 2 BW_UPD, 3 BW_INV */                                     \
+  void launch_mv(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const T* pool,               \
+                 const int64_t* ioff, const T* ipool, double* W, double* V, double* X, double* part,      \
+                 int nrhs, int mode);                                                                     \
+  void launch_fw_out(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const T* pool,           \
+                     const double* W, const double* V, double* Y, int nrhs, int scale, int ndof1);
+FGPU_FACTOR_DECLS(double)
+FGPU_FACTOR_DECLS(float)
+#undef FGPU_FACTOR_DECLS
 void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, double* W, double* V, double* X,
                    const double* part, int nrhs, int mode);
 void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* R, double* W,
                     int nrhs, int ndof1);
-void launch_fw_out(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* pool,
-                   const double* W, const double* V, double* Y, int nrhs, int scale, int ndof1);
 void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y,
                     const double* X, double* W, int nrhs);
 void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* colind, const double* val,
@@ -49,6 +57,18 @@ void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, i
 // sign: nev doubles of scratch
 void launch_mode_signs(hipStream_t st, int n1, const int* p1, double* Phi, int64_t nrows, int nev, double* sign);
 void launch_identity_rows(hipStream_t st, double* X, int64_t nrows, int nrhs, int ndof1);
+// mixed precision: s = 1/sqrt(diag K) per perm row; equilibrated lower
+// entries of K into the FP32 fronts; X = diag(rs) X diag(cs) (cs may be
+// null); Y = a X + b Y; partial sums of squares of x and y over
+// SUMSQ_BLOCKS blocks (2 doubles each, summed on the host in block order)
+constexpr int SUMSQ_BLOCKS = 512;
+void launch_diag_scale(hipStream_t st, int n, const int* rowptr, const int* colind, const double* val, double* s);
+void launch_front_scatter32(hipStream_t st, int n, const int* rowptr, const int* colind, const double* val,
+                            const double* s, const int64_t* fmap, float* pool);
+void launch_scale_rc(hipStream_t st, int64_t nrows, int nrhs, const double* rs, const double* cs, double* X);
+void launch_recip(hipStream_t st, int n, const double* x, double* y);
+void launch_axpby(hipStream_t st, int64_t n, double a, const double* X, double b, double* Y);
+void launch_sumsq2(hipStream_t st, int64_t n, const double* x, const double* y, double* part);
 // element masses (x-translation mass of each element's mass blocks; nen
 // nodes of d DOFs) of one element type list, and their fixed-order total
 void launch_emass(hipStream_t st, int n, const int* elist, int nen, int d, const int64_t* eoff, const double* Mp,

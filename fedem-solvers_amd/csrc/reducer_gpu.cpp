@@ -14,6 +14,8 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <future>
 #include "dense_small.hpp"
@@ -130,6 +132,12 @@ struct DeviceReducer::Impl {
   DBuf<int> ebad, piv;
   DBuf<double> X, Z, Y, part, small, acc, grav, ccoef;
   DBuf<double> emass;  // nel element masses, then their total
+  // mixed precision: FP32 fronts and pivot-block inverses, equilibration
+  // scale, refinement work (correction, residual, K X) and norm partials
+  bool mixed = false;
+  DBuf<float> pool32, ipool32;
+  DBuf<double> dscale, dcs, Dl, Rr, Zr, irpart;
+  std::vector<double> s22;  // mixed: Schur complement from the refined B (n2 x n2, column-major)
   // component modes
   DBuf<double> Q, P, Wl, Zl, Yl, Cc, Phi, Tmp;
   std::vector<int> hexl, shl, bml;  // elements by type: HEX8, ANDES4, BEAM2
@@ -347,7 +355,14 @@ int DeviceReducer::upload(std::string& err) {
   std::vector<int64_t> ioff(y.nsup + 1, 0);
   for (int s = 0; s < y.nsup; s++) ioff[s + 1] = ioff[s] + (int64_t)y.ncol[s] * y.ncol[s];
   HIPCHK(d.ioff.upload(ioff));
-  HIPCHK(d.ipool.alloc(ioff[y.nsup]));
+  d.mixed = opt_.mixed;
+  if (d.mixed) {
+    d.ipool.release();
+    HIPCHK(d.ipool32.alloc(ioff[y.nsup]));
+  } else {
+    d.ipool32.release();
+    HIPCHK(d.ipool.alloc(ioff[y.nsup]));
+  }
   HIPCHK(d.rowptr.upload(y.rowptr));
   HIPCHK(d.colind.upload(y.colind));
   HIPCHK(d.cptr.upload(y.cptr));
@@ -373,7 +388,16 @@ int DeviceReducer::upload(std::string& err) {
   HIPCHK(d.Mp.alloc(y.elem_pool));
   HIPCHK(d.Kv.alloc(nnz));
   HIPCHK(d.Mv.alloc(nnz));
-  HIPCHK(d.pool.alloc(y.fpool));
+  if (d.mixed) {
+    d.pool.release();
+    HIPCHK(d.pool32.alloc(y.fpool));
+    HIPCHK(d.dscale.alloc(y.neq));
+    HIPCHK(d.dcs.alloc(std::max(1, y.ndof2)));
+    HIPCHK(d.irpart.alloc(2 * SUMSQ_BLOCKS));
+  } else {
+    d.pool32.release();
+    HIPCHK(d.pool.alloc(y.fpool));
+  }
   HIPCHK(d.ebad.alloc(part_.nel));
   HIPCHK(d.emass.alloc(part_.nel + 1));
   HIPCHK(d.piv.alloc(2));
@@ -417,7 +441,12 @@ static void run_mv_op(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan& s
   switch (op.kind) {
     case FW_INV: case FW_UPD: case BW_UPD: case BW_INV: {
       int mode = op.kind == FW_INV ? 0 : op.kind == FW_UPD ? 1 : op.kind == BW_UPD ? 2 : 3;
-      launch_mv(st, t, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p, d.W.p, d.V.p, X, sp.part.p, sp.nrhs, mode_of[mode]);
+      if (d.mixed)
+        launch_mv(st, t, op.n, d.S, d.pool32.p, d.ioff.p, d.ipool32.p, d.W.p, d.V.p, X, sp.part.p, sp.nrhs,
+                  mode_of[mode]);
+      else
+        launch_mv(st, t, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p, d.W.p, d.V.p, X, sp.part.p, sp.nrhs,
+                  mode_of[mode]);
       break;
     }
     case FW_INV_R: case FW_UPD_R: case BW_UPD_R: case BW_INV_R: {
@@ -436,7 +465,10 @@ static void run_forward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan&
     switch (op.kind) {
       case FW_INIT: launch_fw_init(st, sp.tasks.p + op.off, op.n, d.S, R, d.W.p, sp.nrhs, ndof1); break;
       case FW_OUT:
-        launch_fw_out(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, d.V.p, Yout, sp.nrhs, scale, ndof1);
+        if (d.mixed)
+          launch_fw_out(st, sp.tasks.p + op.off, op.n, d.S, d.pool32.p, d.W.p, d.V.p, Yout, sp.nrhs, scale, ndof1);
+        else
+          launch_fw_out(st, sp.tasks.p + op.off, op.n, d.S, d.pool.p, d.W.p, d.V.p, Yout, sp.nrhs, scale, ndof1);
         break;
       default: run_mv_op(st, d, sp, op, nullptr); break;
     }
@@ -448,6 +480,94 @@ static void run_backward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan
   for (const Op& op : sp.bw) {
     if (op.kind == BW_INIT) launch_bw_init(st, sp.tasks.p + op.off, op.n, d.S, Yin, X, d.W.p, sp.nrhs);
     else run_mv_op(st, d, sp, op, X);
+  }
+}
+
+// Relative size ||D|| / ||X|| of a correction D against the solution X
+// over their first n entries (device partial sums, added in block order).
+static int rel_correction(hipStream_t st, DeviceReducer::Impl& d, int64_t n, const double* D, const double* X,
+                          double& rel, std::string& err) {
+  launch_sumsq2(st, n, D, X, d.irpart.p);
+  double h[2 * SUMSQ_BLOCKS];
+  HIPCHK(hipMemcpyAsync(h, d.irpart.p, sizeof h, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double a = 0.0, b = 0.0;
+  for (int q = 0; q < SUMSQ_BLOCKS; q++) {
+    a += h[2 * q];
+    b += h[2 * q + 1];
+  }
+  rel = b > 0.0 ? std::sqrt(a / b) : (a > 0.0 ? INFINITY : 0.0);
+  return 0;
+}
+
+// Stopping rule of the mixed-precision refinement, from the relative size of
+// the latest correction: done when it is below tol, or when it no longer
+// halves (the residual has reached FP64 rounding level, where the
+// correction is noise of the size cond(K) * eps64); failure when it stays
+// large or grows past the step limit.
+static int refine_state(double rel, double prev, int steps, const ReduceOptions& opt, std::string& err,
+                        const char* what) {
+  if (rel <= opt.ir_tol) return 1;
+  if (steps >= 2 && rel > 0.5 * prev && rel < 1e-6) return 1;
+  if (steps >= opt.ir_maxit || !std::isfinite(rel)) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "mixed-precision iterative refinement of %s did not converge "
+                  "(correction %.3e after %d steps)", what, rel, steps);
+    err = buf;
+    return -1;
+  }
+  return 0;
+}
+
+// One approximate solve D = K_ii^{-1} Rin on the mixed-precision factor:
+// D = S (K~_ii)^{-1} S Rin with K~ = S K S the equilibrated matrix whose FP32
+// LDL^T the fronts hold.  Rin (neq x nrhs, internal rows read) is scaled in
+// place; D's boundary rows are zero.
+static void approx_solve(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan& sp, int neq, int ndof1,
+                         double* Rin, double* Yw, double* D) {
+  launch_scale_rc(st, ndof1, sp.nrhs, d.dscale.p, nullptr, Rin);
+  run_forward(st, d, sp, Rin, Yw, ndof1, true, false);
+  launch_fill(st, D, (int64_t)neq * sp.nrhs, 0.0);
+  run_backward(st, d, sp, Yw, D);
+  launch_scale_rc(st, ndof1, sp.nrhs, d.dscale.p, nullptr, D);
+}
+
+// X = K_ii^{-1} R on the internal rows (X's boundary rows zero), R and X
+// neq x nrhs row-major in perm order; Yw is neq x nrhs work.  FP64 factor:
+// one forward and one backward pass.  Mixed precision: iterative refinement,
+// X += K_ii^{-1}~ (R - K_ii X) with the residual from the FP64 CSR matrix,
+// until the correction is below ir_tol relative to X.
+static int solve_int(hipStream_t st, DeviceReducer::Impl& d, const Symbolic& y, const ReduceOptions& opt,
+                     const SolvePlan& sp, const double* R, double* Yw, double* X, int* steps, std::string& err) {
+  const int neq = y.neq, n1 = y.ndof1, nr = sp.nrhs;
+  const int64_t nall = (int64_t)neq * nr, nint = (int64_t)n1 * nr;
+  if (!d.mixed) {
+    run_forward(st, d, sp, R, Yw, n1, true, false);
+    launch_fill(st, X, nall, 0.0);
+    run_backward(st, d, sp, Yw, X);
+    if (steps) *steps = 0;
+    return 0;
+  }
+  HIPCHK(d.Dl.alloc(nall));
+  HIPCHK(d.Rr.alloc(nall));
+  HIPCHK(d.Zr.alloc(nall));
+  HIPCHK(hipMemcpyAsync(d.Rr.p, R, sizeof(double) * nall, hipMemcpyDeviceToDevice, st));
+  launch_fill(st, X, nall, 0.0);
+  double prev = INFINITY;
+  for (int it = 0;; it++) {
+    approx_solve(st, d, sp, neq, n1, d.Rr.p, Yw, d.Dl.p);
+    launch_axpby(st, nint, 1.0, d.Dl.p, 1.0, X);
+    double rel = 0.0;
+    if (rel_correction(st, d, nint, d.Dl.p, X, rel, err)) return -3;
+    if (steps) *steps = it + 1;
+    const int state = refine_state(rel, prev, it + 1, opt, err, "a K_ii solve");
+    if (state > 0) return 0;
+    if (state < 0) return -6;
+    prev = rel;
+    // residual R - K_ii X (X's boundary rows are zero, so K X has the K_ii X rows)
+    launch_spmm(st, neq, nr, d.rowptr.p, d.colind.p, d.Kv.p, X, nr, d.Zr.p, nr);
+    HIPCHK(hipMemcpyAsync(d.Rr.p, R, sizeof(double) * nint, hipMemcpyDeviceToDevice, st));
+    launch_axpby(st, nint, -1.0, d.Zr.p, 1.0, d.Rr.p);
   }
 }
 
@@ -470,6 +590,7 @@ int DeviceReducer::reduce(std::string& err) {
   const int ngen = opt_.ngen;
   hipStream_t st = st_;
   res_ = Results();
+  times_.refine_steps = 0;
   res_.ndof1 = n1;
   res_.ndof2 = n2;
   res_.ngen = ngen;
@@ -493,9 +614,14 @@ int DeviceReducer::reduce(std::string& err) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[1], st));
   // ---- assembly (CSR K, M and the K entries of the fronts)
-  launch_fill(st, d.pool.p, y.fpool, 0.0);
+  if (d.mixed) HIPCHK(hipMemsetAsync(d.pool32.p, 0, sizeof(float) * y.fpool, st));
+  else launch_fill(st, d.pool.p, y.fpool, 0.0);
   launch_assemble(st, nnz, d.cptr.p, d.cidx.p, y.ccoef.empty() ? nullptr : d.ccoef.p, d.Kp.p, d.Mp.p, d.Kv.p, d.Mv.p,
-                  d.fmap.p, d.pool.p);
+                  d.fmap.p, d.mixed ? nullptr : d.pool.p);
+  if (d.mixed) {  // equilibrated stiffness into the FP32 fronts
+    launch_diag_scale(st, neq, d.rowptr.p, d.colind.p, d.Kv.p, d.dscale.p);
+    launch_front_scatter32(st, neq, d.rowptr.p, d.colind.p, d.Kv.p, d.dscale.p, d.fmap.p, d.pool32.p);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[2], st));
   // ---- multifrontal LDL^T; the Schur front ends up holding K_bb - K_bi K_ii^-1 K_ib
@@ -508,27 +634,32 @@ int DeviceReducer::reduce(std::string& err) {
     for (auto& e : d.upd_ev) HIPCHK(hipEventCreate(&e));
   }
   int64_t ui = 0;
-  auto factor_ops = [&]() {
+  auto factor_with = [&](auto* pool, auto* ipool) {
     for (const Op& op : d.fops) {
       switch (op.kind) {
-        case EXTADD: launch_extend_add(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p); break;
-        case DIAG: launch_diag(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p, d.piv.p); break;
-        case TRSM: launch_trsm(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p); break;
+        case EXTADD: launch_extend_add(st, d.ftasks.p + op.off, op.n, d.S, pool); break;
+        case DIAG: launch_diag(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool, d.piv.p); break;
+        case TRSM: launch_trsm(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool); break;
         case UPDATE:
           if (opt_.time_updates) (void)hipEventRecord(d.upd_ev[2 * ui], st);
-          launch_update(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p);
+          launch_update(st, d.ftasks.p + op.off, op.n, d.S, pool);
           if (opt_.time_updates) (void)hipEventRecord(d.upd_ev[2 * ui + 1], st);
           ui++;
           break;
-        case INV_OFF: launch_inv_off(st, d.ftasks.p + op.off, op.n, d.S, d.pool.p, d.ioff.p, d.ipool.p); break;
+        case INV_OFF: launch_inv_off(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool); break;
       }
     }
+  };
+  auto factor_ops = [&]() {
+    if (d.mixed) factor_with(d.pool32.p, d.ipool32.p);
+    else factor_with(d.pool.p, d.ipool.p);
   };
   if (opt_.time_updates || !opt_.use_graphs) {
     factor_ops();  // per-launch events need the plain stream
   } else {
     // the factor schedule is fixed per symbolic structure: one graph
-    HIPCHK(run_graph(st, d.g_factor, {d.ftasks.p, d.pool.p, d.ipool.p, d.ioff.p, d.piv.p, d.S.foff, d.S.rows},
+    HIPCHK(run_graph(st, d.g_factor,
+                     {d.ftasks.p, d.pool.p, d.ipool.p, d.pool32.p, d.ipool32.p, d.ioff.p, d.piv.p, d.S.foff, d.S.rows},
                      factor_ops));
   }
   HIPCHK(hipGetLastError());
@@ -544,6 +675,43 @@ int DeviceReducer::reduce(std::string& err) {
     launch_identity_rows(st, d.X.p, neq, n2, n1);
     launch_fill(st, d.Y.p, (int64_t)neq * n2, 0.0);
     run_backward(st, d, *pb, d.Y.p, d.X.p);
+    if (d.mixed) {
+      // the backward pass of the equilibrated factor gives S_i^-1 B S_b
+      launch_recip(st, n2, d.dscale.p + n1, d.dcs.p);
+      launch_scale_rc(st, n1, n2, d.dscale.p, d.dcs.p, d.X.p);
+      // refinement against the FP64 matrix: Z = K [B; I] has the residual
+      // -(K_ii B + K_ib) in its internal rows and K_bb + K_bi B in its
+      // boundary rows (the Schur complement once B has converged)
+      HIPCHK(d.Dl.alloc((size_t)neq * n2));
+      bool conv = false;
+      int steps = 0;
+      double prev = INFINITY;
+      for (;;) {
+        launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Kv.p, d.X.p, n2, d.Z.p, n2);
+        if (conv) break;
+        double rres = 0.0;
+        if (std::getenv("FGPU_TRACE_REFINE") && rel_correction(st, d, (int64_t)n1 * n2, d.Z.p, d.X.p, rres, err))
+          return -3;
+        approx_solve(st, d, *pb, neq, n1, d.Z.p, d.Y.p, d.Dl.p);
+        launch_axpby(st, (int64_t)n1 * n2, -1.0, d.Dl.p, 1.0, d.X.p);
+        double rel = 0.0;
+        if (rel_correction(st, d, (int64_t)n1 * n2, d.Dl.p, d.X.p, rel, err)) return -3;
+        steps++;
+        if (std::getenv("FGPU_TRACE_REFINE")) std::fprintf(stderr, "refine B step %d: |R|/|X| %.3e  |D|/|X| %.3e\n", steps, rres, rel);
+        const int state = refine_state(rel, prev, steps, opt_, err, "B");
+        if (state < 0) return -6;
+        conv = state > 0;
+        prev = rel;
+      }
+      times_.refine_steps = steps;
+      std::vector<double> zb((size_t)n2 * n2);
+      HIPCHK(hipMemcpyAsync(zb.data(), d.Z.p + (size_t)n1 * n2, sizeof(double) * zb.size(), hipMemcpyDeviceToHost,
+                            st));
+      HIPCHK(hipStreamSynchronize(st));
+      d.s22.assign((size_t)n2 * n2, 0.0);
+      for (int b = 0; b < n2; b++)
+        for (int a = 0; a < n2; a++) d.s22[a + (size_t)b * n2] = 0.5 * (zb[(size_t)a * n2 + b] + zb[(size_t)b * n2 + a]);
+    }
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[4], st));
@@ -705,7 +873,9 @@ int DeviceReducer::reduce(std::string& err) {
         launch_fill(st, d.Wl.p, (int64_t)neq * bs, 0.0);  // boundary rows stay zero
         run_backward(st, d, *pl, d.Yl.p, d.Wl.p);
       };
-      if (!opt_.use_graphs) solve_step();
+      if (d.mixed) {
+        if (int rc2 = solve_int(st, d, y, opt_, *pl, d.Tmp.p, d.Yl.p, d.Wl.p, nullptr, err)) return rc2;
+      } else if (!opt_.use_graphs) solve_step();
       else HIPCHK(run_graph(st, d.g_lanczos,
                        {pl, pl->tasks.p, pl->part.p, d.Tmp.p, d.Yl.p, d.Wl.p, d.W.p, d.V.p, d.pool.p, d.ipool.p,
                         d.S.foff, d.S.woff}, solve_step));
@@ -826,9 +996,7 @@ int DeviceReducer::reduce(std::string& err) {
       if (ensure_work(d, std::max(std::max(n2, 3), opt_.lanczos_block), err)) return -3;
       HIPCHK(d.Zl.alloc((size_t)neq * std::max(3, opt_.lanczos_block)));
       HIPCHK(d.Wl.alloc((size_t)neq * std::max(3, opt_.lanczos_block)));
-      run_forward(st, d, *p3, d.Tmp.p, d.Zl.p, n1, true, false);
-      launch_fill(st, d.Wl.p, (int64_t)neq * 3, 0.0);
-      run_backward(st, d, *p3, d.Zl.p, d.Wl.p);
+      if (int rc2 = solve_int(st, d, y, opt_, *p3, d.Tmp.p, d.Zl.p, d.Wl.p, nullptr, err)) return rc2;
       std::vector<double> v((size_t)neq * 3);
       HIPCHK(hipMemcpyAsync(v.data(), d.Wl.p, sizeof(double) * v.size(), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
@@ -841,7 +1009,8 @@ int DeviceReducer::reduce(std::string& err) {
   }
   HIPCHK(hipEventRecord(d.ev[7], st));
   // ---- Schur complement out of the root front
-  if (n2 > 0) {
+  if (n2 > 0 && d.mixed) S22 = d.s22;
+  else if (n2 > 0) {
     HIPCHK(hipMemcpyAsync(S22.data(), d.pool.p + y.foff[schur], sizeof(double) * n2 * n2, hipMemcpyDeviceToHost, st));
   }
   if (!opt_.no_mass)
@@ -930,6 +1099,10 @@ int DeviceReducer::get_schur(std::vector<double>& S) {
   std::vector<double> f((size_t)n2 * n2);
   S.assign((size_t)n2 * n2, 0.0);
   if (n2 == 0) return 0;
+  if (d_->mixed) {  // from the refined B (symmetric)
+    S = d_->s22;
+    return 0;
+  }
   if (hipMemcpy(f.data(), d_->pool.p + y.foff[y.nsup - 1], sizeof(double) * f.size(), hipMemcpyDeviceToHost) != hipSuccess)
     return -3;
   for (int j = 0; j < n2; j++)
@@ -974,9 +1147,7 @@ int DeviceReducer::solve_kii(int nrhs, const double* R_eq, double* X_eq, std::st
   HIPCHK(r.upload(h));
   HIPCHK(t.alloc(h.size()));
   HIPCHK(x.alloc(h.size()));
-  run_forward(st_, d, *p, r.p, t.p, n1, true, false);
-  launch_fill(st_, x.p, (int64_t)h.size(), 0.0);
-  run_backward(st_, d, *p, t.p, x.p);
+  if (int rc = solve_int(st_, d, y, opt_, *p, r.p, t.p, x.p, nullptr, err)) return rc;
   HIPCHK(hipMemcpyAsync(h.data(), x.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
   for (int q = 0; q < n1; q++) {
@@ -1091,10 +1262,12 @@ size_t DeviceReducer::footprint_bytes() const {
   const int bs = opt_.lanczos_block;
   const size_t mcap = nev > 0 ? (size_t)std::max(10 * nev + 40, 16 * bs) : 0;
   const size_t rhs = (size_t)std::max(std::max(y.ndof2, bs), 3);
-  size_t b = 8 * ((size_t)y.fpool + ipool + 2 * nnz + 2 * (size_t)y.elem_pool);  // fronts, inverses, K, M, elements
+  const size_t fb = opt_.mixed ? 4 : 8;  // bytes per factor entry
+  size_t b = fb * ((size_t)y.fpool + ipool) + 8 * (2 * nnz + 2 * (size_t)y.elem_pool);  // fronts, inverses, K, M, elements
   b += 12 * y.cidx.size() + 4 * y.cptr.size() + 8 * nnz + 4 * nnz;                // contributions, fmap, colind
   b += 8 * (size_t)y.neq * (3 * (size_t)y.ndof2 + 2 * mcap + 8 * (size_t)bs);     // B solves and Lanczos basis
   b += 8 * 2 * wrows * rhs;                                                        // solve work rows
+  if (opt_.mixed) b += 8 * 3 * (size_t)y.neq * rhs;                                // refinement work
   return b;
 }
 

@@ -19,6 +19,13 @@ struct ReduceOptions {
   bool no_mass = false;
   bool time_updates = false;  // bracket every trailing-update launch with HIP events
   bool use_graphs = true;     // replay the factorisation and Lanczos solve steps as HIP graphs
+  // Mixed precision: FP32 fronts of the equilibrated K (FP32 MFMA trailing
+  // and Schur updates, FP64 diagonal-block factorisation), every solve with
+  // K_ii refined against the FP64 matrix until the correction falls below
+  // ir_tol relative (at most ir_maxit steps; no convergence is an error).
+  bool mixed = false;
+  double ir_tol = 1e-14;
+  int ir_maxit = 40;
   uint64_t seed = 0x5eed;  // Lanczos start block
   SymbolicOptions sym;
 };
@@ -26,6 +33,7 @@ struct ReduceOptions {
 struct PhaseTimes {  // milliseconds, measured with HIP events on the work stream
   double elements = 0, assembly = 0, factor = 0, bsolve = 0, modes = 0, mass = 0, gravity = 0, total = 0;
   double update_kernel_ms = 0;   // sum of trailing-update (MFMA) launches, when timed
+  int refine_steps = 0;          // mixed precision: refinement steps of the B solve
   int64_t update_launches = 0;
 };
 

@@ -200,10 +200,10 @@ class GpuReducer:
         return X
 
     def times(self):
-        t = np.zeros(10)
+        t = np.zeros(11)
         self._check(self.lib.fgpu_get_times(self.h, _ptr(t)), "fgpu_get_times")
         keys = ("elements", "assembly", "factor", "bsolve", "modes", "mass", "gravity", "total",
-                "update_kernel_ms", "update_launches")
+                "update_kernel_ms", "update_launches", "refine_steps")
         return dict(zip(keys, t.tolist()))
 
     def info(self):

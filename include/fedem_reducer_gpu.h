@@ -93,7 +93,12 @@ int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags);
 /* Numeric options applied at the next fgpu_analyze: "lanczosBlock",
  * "lanczosMaxVectors", "lanczosTol" (relative Ritz residual), "eigenshift",
  * "seed" (Lanczos start block), "leafNodes" (nested-dissection leaf size),
- * "timeUpdates" (1: time every trailing-update launch with HIP events). */
+ * "timeUpdates" (1: time every trailing-update launch with HIP events),
+ * "mixedPrecision" (1: FP32 factor of the equilibrated K with FP32 MFMA
+ * trailing and Schur updates and FP64 diagonal blocks; B and every K_ii
+ * solve refined against the FP64 matrix), "refineTol" (relative correction
+ * that ends the refinement, default 1e-14), "refineMaxSteps" (default 40;
+ * reduce returns -6 when the refinement does not converge). */
 int fgpu_set_option(fgpu_reducer* h, const char* name, double value);
 
 /* sizes[0..7] = nnod, ndof, neq, ndof1, ndof2, ndim, nenod, nel */
@@ -179,9 +184,10 @@ int fgpu_server_error(fgpu_server* s, int id, char* buf, int len);
 /* Frees a finished request's results. */
 int fgpu_server_release(fgpu_server* s, int id);
 
-/* Diagnostics: times[0..9] (ms) = elements, assembly, factor, B, modes,
+/* Diagnostics: times[0..10] (ms) = elements, assembly, factor, B, modes,
  * mass, gravity, total, summed duration of the trailing-update (MFMA)
- * launches and their count (option "timeUpdates"); info[0..9] = nsup, nlevels, front pool doubles,
+ * launches and their count (option "timeUpdates"), refinement steps of the
+ * mixed-precision B solve; info[0..9] = nsup, nlevels, front pool doubles,
  * nnz(L), factor flops, max front, nnz(K), Lanczos vectors, packed element
  * pool doubles, trailing-update (MFMA) flops. */
 int fgpu_get_times(const fgpu_reducer* h, double* times);
