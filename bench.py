@@ -90,6 +90,41 @@ def batched(n_parts, slots, seed):
             "speedup": out[slots] / out[1]}
 
 
+def mixed_plate(seed, reps=3):
+    """The 200 x 200 ANDES4 plate with lognormal thickness and 64 edge nodes
+    (242,406 DOF, 384 boundary DOF): the FP64 path and the mixed-precision
+    path (FP32 fronts, FP32 MFMA updates, FP64 diagonal blocks, refined B),
+    each reduction timed end to end; the Schur/factor rate is the symbolic
+    factor flop count over the measured factor phase; error of every mixed
+    output against the FP64 path."""
+    from fedempy_amd import gpu, parts
+    part = parts.config2(seed=seed)
+    out, res = {}, {}
+    for mode in ("fp64", "mixed"):
+        R = gpu.GpuReducer()
+        R.set_part(part)
+        if mode == "mixed":
+            R.set_option("mixedPrecision", 1)
+        R.analyze(ngen=0)
+        R.reduce()  # warm-up (allocations, graph capture)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            R.reduce()
+            SK, SM, G = R.reduced()
+        el = (time.perf_counter() - t0) / reps
+        tt, info = R.times(), R.info()
+        res[mode] = (SK, SM, G, R.B())
+        out[mode] = {"ms_per_reduction": 1e3 * el, "reductions_per_s": 1.0 / el,
+                     "factor_ms": tt["factor"], "bsolve_ms": tt["bsolve"], "mass_ms": tt["mass"],
+                     "schur_tflops": info["factor_flops"] / (tt["factor"] * 1e-3) / 1e12,
+                     "refine_steps": int(tt["refine_steps"])}
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["mixed"]["rel_error_vs_fp64"] = {k: rel(a, b) for k, a, b in zip(("SK", "SM", "G", "B"), res["mixed"],
+                                                                           res["fp64"])}
+    out["workload"] = "200x200 ANDES4 plate, lognormal thickness, 64 edge nodes (242,406 DOF, 384 boundary DOF)"
+    return out
+
+
 def _drain(srv, reqs, gaps=None):
     """Submit reqs (back to back, or at the given inter-arrival gaps) and
     wait for all; returns (wall seconds, latencies in ms)."""
@@ -147,6 +182,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision plate measurement")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain device copy (torch.neg over N MiB of FP64) whose HBM "
                          "counters calibrate FETCH_SIZE/WRITE_SIZE of the profiled kernels")
@@ -215,6 +251,8 @@ def main():
                      "nnzL": info["nnzL"], "fronts": info["nsup"], "lanczos_vectors": nlz,
                      "lanczos_resid": resid},
     }
+    if not a.no_mixed:
+        line["mixed_precision"] = mixed_plate(a.seed)
     if not a.no_serving:
         line["batched"] = batched(128, 16, a.seed)
         line["served"] = served(160, 16, a.seed)

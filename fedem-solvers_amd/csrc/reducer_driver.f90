@@ -521,6 +521,7 @@ subroutine fgpu_reducer_run (ierr) bind(C,name='fgpu_reducer_run')
        rc = fgpu_set_option(h,cs('lanczosBlock'),real(fgpu_opt_int(cs('lanczosBlock')),dp))
   if (fgpu_opt_int(cs('lanczosMaxVectors')) > 0) &
        rc = fgpu_set_option(h,cs('lanczosMaxVectors'),real(fgpu_opt_int(cs('lanczosMaxVectors')),dp))
+  if (fgpu_opt_bool(cs('mixedPrecision')) /= 0) rc = fgpu_set_option(h,cs('mixedPrecision'),1.0_dp)
   ierr = fgpu_link_attach(h)
   if (ierr /= 0) then
      call link_failure (lpu,ierr)
@@ -706,6 +707,7 @@ subroutine fgpu_solver_run (n,nodes,displ,ierr) bind(C,name='fgpu_solver_run')
 
   ierr = fgpu_create(h)
   if (ierr /= 0) goto 900
+  if (fgpu_opt_bool(cs('mixedPrecision')) /= 0) rc = fgpu_set_option(h,cs('mixedPrecision'),1.0_dp)
   ierr = fgpu_link_attach(h)
   if (ierr /= 0) then
      call link_failure (lpu,ierr)

@@ -55,6 +55,8 @@ void define_solver_options() {
   o.define("cwd", "", "Working directory of the run");
   // device-solver controls (no counterpart in the reference)
   o.define("lanczosBlock", "16", "Block size of the device Lanczos eigensolver");
+  o.define("mixedPrecision", "false",
+           "FP32 factorisation (FP32 MFMA updates, FP64 diagonal blocks) with FP64 iterative refinement");
   o.define("lanczosMaxVectors", "0", "Basis size cap of the device Lanczos (0: automatic)");
 }
 

@@ -280,3 +280,20 @@ def test_clamped_plate_reports_mass_deviation(tmp_path):
     import scipy.linalg as sla
     lam = sla.eigh(o["SK"], o["SM"], eigvals_only=True)
     assert np.allclose(_res_frequencies(res), np.sqrt(lam) / (2 * np.pi), rtol=1e-6, atol=0)
+
+
+def test_reducepart_mixed_precision_option(tmp_path):
+    """-mixedPrecision through the drop-in: the written reduced matrices stay
+    within the mixed-path tolerance (1e-8) of the FP64 oracle."""
+    p = clamped_block()
+    nas = tmp_path / "mx.nas"
+    parts.write_nastran(p, str(nas))
+    red = FedemReducer(solver_options=["-linkfile=%s" % nas, "-ngen=3", "-mixedPrecision"])
+    assert red.run() == 0, (tmp_path / "mx.res").read_text()
+    red.release()
+    o = orc.reduce(p, ngen=3)
+    nd = o["SK"].shape[0]
+    SK, _ = read_fmx(tmp_path / "mx_S.fmx", (nd, nd))
+    SM, _ = read_fmx(tmp_path / "mx_M.fmx", (nd, nd))
+    G, _ = read_fmx(tmp_path / "mx_G.fmx", (nd, 3))
+    assert rel(SK, o["SK"]) <= 1e-8 and rel(SM, o["SM"]) <= 1e-8 and rel(G, o["G"]) <= 1e-8
