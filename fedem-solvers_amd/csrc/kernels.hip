@@ -1090,9 +1090,16 @@ void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx,
 }
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta) {
-  if (n > 0 && nb > 0)
-    k_gemm_small<<<nblk((int64_t)n * nb, 256), 256, sizeof(double) * na * nb, st>>>(n, na, nb, X, ldx, Sm, Y,
-                                                                                   ldy, alpha, beta);
+  if (n <= 0 || nb <= 0) return;
+  // S is staged in LDS in row chunks of at most 64 KiB; chunk k adds
+  // beta X_k S_k to the result of the chunks before it (fixed order)
+  const int rows = std::max(1, (int)(65536 / (sizeof(double) * nb)));
+  for (int a0 = 0; a0 < na || a0 == 0; a0 += rows) {
+    const int ka = std::min(rows, na - a0);
+    k_gemm_small<<<nblk((int64_t)n * nb, 256), 256, sizeof(double) * std::max(ka, 1) * nb, st>>>(
+        n, ka, nb, X + a0, ldx, Sm + (size_t)a0 * nb, Y, ldy, a0 == 0 ? alpha : 1.0, beta);
+    if (na <= 0) break;
+  }
 }
 void launch_mode_signs(hipStream_t st, int n1, const int* p1, double* Phi, int64_t nrows, int nev, double* sign) {
   if (nev <= 0) return;

@@ -135,6 +135,7 @@ struct DeviceReducer::Impl {
   // mixed precision: FP32 fronts and pivot-block inverses, equilibration
   // scale, refinement work (correction, residual, K X) and norm partials
   bool mixed = false;
+  int solve_split = 128;
   DBuf<float> pool32, ipool32;
   DBuf<double> dscale, dcs, Dl, Rr, Zr, irpart;
   std::vector<double> s22;  // mixed: Schur complement from the refined B (n2 x n2, column-major)
@@ -216,7 +217,7 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
 }
 
 static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<SolveTask>& tasks,
-                             std::vector<int>& sns) {
+                             std::vector<int>& sns, int KS) {
   const int nrhs = sp.nrhs, schur = y.nsup - 1;
   // (front, 64-row chunk) tasks over all rows (init) or the solved rows (out)
   auto add_sns = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list) {
@@ -230,7 +231,6 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
   (void)sns;
   // GEMM tasks over the row tiles of a front and all right-hand-side tiles;
   // K ranges longer than KS are split and reduced in slot order
-  const int KS = 512;
   auto add_tiles = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list, bool upd_rows) {
     const int mode = kind == FW_INV ? 0 : kind == FW_UPD ? 1 : kind == BW_UPD ? 2 : 3;
     const int rkind = kind == FW_INV ? FW_INV_R : kind == FW_UPD ? FW_UPD_R : kind == BW_UPD ? BW_UPD_R : BW_INV_R;
@@ -356,6 +356,7 @@ int DeviceReducer::upload(std::string& err) {
   for (int s = 0; s < y.nsup; s++) ioff[s + 1] = ioff[s] + (int64_t)y.ncol[s] * y.ncol[s];
   HIPCHK(d.ioff.upload(ioff));
   d.mixed = opt_.mixed;
+  d.solve_split = std::max(64, (opt_.solve_split / 64) * 64);
   if (d.mixed) {
     d.ipool.release();
     HIPCHK(d.ipool32.alloc(ioff[y.nsup]));
@@ -420,7 +421,7 @@ static int get_plan(const Symbolic& y, DeviceReducer::Impl& d, int nrhs, SolvePl
   sp->nrhs = nrhs;
   std::vector<SolveTask> tasks;
   std::vector<int> sns;
-  build_solve_plan(y, *sp, tasks, sns);
+  build_solve_plan(y, *sp, tasks, sns, d.solve_split);
   HIPCHK(sp->tasks.upload(tasks));
   HIPCHK(sp->sns.upload(sns));
   HIPCHK(sp->part.alloc((size_t)std::max(1, sp->nslots) * 4096));
@@ -932,6 +933,13 @@ int DeviceReducer::reduce(std::string& err) {
       done = run_check(nb * bs, projected(nb * bs), Gl);
     }
     res_.lanczos_resid = done.worst;
+    if (!done.ok) {
+      char buf[200];
+      std::snprintf(buf, sizeof buf, "component modes did not converge: worst relative Ritz residual %.3e after %d "
+                    "Lanczos vectors (tolerance %.1e; raise lanczosMaxVectors)", done.worst, done.mj, tol);
+      err = buf;
+      return -7;
+    }
     const int m = done.mj;
     std::vector<double> w, V;
     w.swap(done.w);

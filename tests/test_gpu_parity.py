@@ -270,3 +270,25 @@ def test_repeated_reductions_are_bit_identical():
     for o in outs[1:]:
         for a, b in zip(o, outs[0]):
             assert np.array_equal(a, b)
+
+
+def test_lanczos_block_sizes_and_unconverged_basis():
+    """A wider Lanczos block (48: the reorthogonalisation coefficients no
+    longer fit one LDS stage) gives the same modes; a basis capped below
+    what the modes need is an error, not silently inaccurate modes."""
+    part = parts.hex_block(10, 7, 6, 0.10, 0.07, 0.06, seed=2, n_ext=6)
+    lams = []
+    for bs in (16, 48):
+        R = gpu.GpuReducer()
+        R.set_part(part)
+        R.set_option("lanczosBlock", bs)
+        R.analyze(ngen=6)
+        R.reduce()
+        lams.append(R.eigen()[0])
+    assert np.max(np.abs(lams[1] - lams[0]) / lams[0]) <= 1e-9
+    R = gpu.GpuReducer()
+    R.set_part(part)
+    R.set_option("lanczosMaxVectors", 32)
+    R.analyze(ngen=20)
+    with pytest.raises(gpu.ReducerError, match="did not converge"):
+        R.reduce()
