@@ -14,6 +14,7 @@ struct FTask {
   int sn;      // supernode
   int a, b;    // meaning depends on the kernel (row chunk, tile indices)
   int k0;      // first pivot column of the panel
+  int kb;      // trailing update: panel width (<= 128); the trailing block starts at k0 + kb
 };
 
 struct SolveTask {

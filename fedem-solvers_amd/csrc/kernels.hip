@@ -397,7 +397,10 @@ __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, D
 
 // ---------------------------------------------------------------- update
 // Task: sn, a = ti, b = tj (64x64 tiles of the trailing lower triangle),
-// k0 = panel.  F[I,J] -= (L_I D) L_J^T with the panel of width kb <= 64.
+// k0 = panel, kb = panel width (<= 128).  F[I,J] -= (L_I D) L_J^T over the
+// panel, staged through LDS in 64-column chunks whose products accumulate
+// in the MFMA registers, so a 128-wide panel reads and writes the trailing
+// tile once for both halves.
 template <class T>
 __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks, DevSym S,
                                                 T* __restrict__ pool) {
@@ -405,35 +408,36 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
   __shared__ T sJ[64][65];
   const FTask t = tasks[blockIdx.x];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn];
-  const int kb = min(64, S.ncol[sn] - k0);
+  const int kb = t.kb;
   const int t0 = k0 + kb;
   const int I0 = t0 + 64 * t.a, J0 = t0 + 64 * t.b;
   T* F = pool + S.foff[sn];
   const int tid = threadIdx.x;
-  const int kbr = (kb + 3) & ~3;
-  {
+  const int lane = tid & 63, wv = tid >> 6;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  typename MF<T>::acc acc[2][2];
+  zero_acc(acc);
+  for (int c0 = 0; c0 < kb; c0 += 64) {
+    const int kc = min(64, kb - c0), p0 = k0 + c0;
     T ri[16], rj[16], rd[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
-      const bool okm = m < kb;
-      rd[q] = okm ? F[(int64_t)(k0 + m) * (ld + 1)] : (T)0;
-      ri[q] = (okm && I0 + i < ld) ? F[(int64_t)(I0 + i) + (int64_t)(k0 + m) * ld] : (T)0;
-      rj[q] = (okm && J0 + i < ld) ? F[(int64_t)(J0 + i) + (int64_t)(k0 + m) * ld] : (T)0;
+      const bool okm = m < kc;
+      rd[q] = okm ? F[(int64_t)(p0 + m) * (ld + 1)] : (T)0;
+      ri[q] = (okm && I0 + i < ld) ? F[(int64_t)(I0 + i) + (int64_t)(p0 + m) * ld] : (T)0;
+      rj[q] = (okm && J0 + i < ld) ? F[(int64_t)(J0 + i) + (int64_t)(p0 + m) * ld] : (T)0;
     }
+    if (c0 > 0) __syncthreads();  // the previous chunk's products are done with the tiles
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
       sI[i][m] = ri[q] * rd[q];
       sJ[i][m] = rj[q];
     }
+    __syncthreads();
+    mfma_tile<T>(sI, sJ, (kc + 3) & ~3, acc, lane, wr, wc);
   }
-  __syncthreads();
-  const int lane = tid & 63, wv = tid >> 6;
-  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
-  typename MF<T>::acc acc[2][2];
-  zero_acc(acc);
-  mfma_tile<T>(sI, sJ, kbr, acc, lane, wr, wc);
   const int lr = lane & 15, lk = lane >> 4;
 #pragma unroll
   for (int x = 0; x < 2; x++)

@@ -168,16 +168,21 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
     for (int q = 0; q < ns; q++) {
       int s = sups[q];
       if (y.chptr[s + 1] == y.chptr[s]) continue;
-      for (int j0 = 0; j0 < y.nrow[s]; j0 += 16) tasks.push_back({s, j0, 0, 0});
+      for (int j0 = 0; j0 < y.nrow[s]; j0 += 16) tasks.push_back({s, j0, 0, 0, 0});
     }
     if ((int)tasks.size() > off) ops.push_back({EXTADD, off, (int)tasks.size() - off});
     int maxnc = 0;
     for (int q = 0; q < ns; q++) maxnc = std::max(maxnc, y.ncol[sups[q]]);
-    for (int k0 = 0; k0 < maxnc; k0 += 64) {
-      off = (int)tasks.size();
+    // Pivot columns go in 128-wide panels of two 64-column halves: diagonal
+    // block and triangular solve of the first half, its update of the second
+    // half's columns only, diagonal block and solve of the second half, then
+    // one trailing update over the whole 128-column panel (a front with fewer
+    // than 128 pivot columns left takes plain 64-column panels).
+    auto add_diag_trsm = [&](int k0) {
+      int off = (int)tasks.size();
       for (int q = 0; q < ns; q++) {
         int s = sups[q];
-        if (y.ncol[s] > k0) tasks.push_back({s, 0, 0, k0});
+        if (y.ncol[s] > k0) tasks.push_back({s, 0, 0, k0, 0});
       }
       if ((int)tasks.size() > off) ops.push_back({DIAG, off, (int)tasks.size() - off});
       off = (int)tasks.size();
@@ -185,19 +190,44 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
         int s = sups[q];
         if (y.ncol[s] <= k0) continue;
         int kb = std::min(64, y.ncol[s] - k0), rem = y.nrow[s] - k0 - kb;
-        for (int a = 0; 64 * a < rem; a++) tasks.push_back({s, a, 0, k0});
+        for (int a = 0; 64 * a < rem; a++) tasks.push_back({s, a, 0, k0, 0});
       }
       if ((int)tasks.size() > off) ops.push_back({TRSM, off, (int)tasks.size() - off});
+    };
+    // all 64x64 lower tiles of the trailing block from k0 + kb
+    auto add_full_update = [&](int s, int k0, int kb) {
+      int rem = y.nrow[s] - k0 - kb;
+      if (rem <= 0) return;
+      int T = (rem + 63) / 64;
+      for (int ti = 0; ti < T; ti++)
+        for (int tj = 0; tj <= ti; tj++) tasks.push_back({s, ti, tj, k0, kb});
+      flops += (double)rem * (rem + 1) * kb;  // lower triangle incl. diagonal, 2 flops per FMA
+    };
+    for (int k0 = 0; k0 < maxnc; k0 += 128) {
+      add_diag_trsm(k0);
+      int off = (int)tasks.size();
+      for (int q = 0; q < ns; q++) {
+        int s = sups[q];
+        const int rc = y.ncol[s] - k0;
+        if (rc <= 0) continue;
+        if (rc >= 128) {  // first half onto the second half's 64 columns (tile column 0)
+          const int rem = y.nrow[s] - k0 - 64;
+          for (int ti = 0; 64 * ti < rem; ti++) tasks.push_back({s, ti, 0, k0, 64});
+          flops += 2.0 * 64 * (64.0 * 65.0 / 2.0 + (double)(rem - 64) * 64.0);
+        } else {
+          add_full_update(s, k0, std::min(64, rc));
+        }
+      }
+      if ((int)tasks.size() > off) ops.push_back({UPDATE, off, (int)tasks.size() - off});
+      if (k0 + 64 >= maxnc) continue;
+      add_diag_trsm(k0 + 64);
       off = (int)tasks.size();
       for (int q = 0; q < ns; q++) {
         int s = sups[q];
-        if (y.ncol[s] <= k0) continue;
-        int kb = std::min(64, y.ncol[s] - k0), rem = y.nrow[s] - k0 - kb;
-        if (rem <= 0) continue;
-        int T = (rem + 63) / 64;
-        for (int ti = 0; ti < T; ti++)
-          for (int tj = 0; tj <= ti; tj++) tasks.push_back({s, ti, tj, k0});
-        flops += (double)rem * (rem + 1) * kb;  // lower triangle incl. diagonal, 2 flops per FMA
+        const int rc = y.ncol[s] - k0;
+        if (rc <= 64) continue;
+        if (rc >= 128) add_full_update(s, k0, 128);
+        else add_full_update(s, k0 + 64, rc - 64);
       }
       if ((int)tasks.size() > off) ops.push_back({UPDATE, off, (int)tasks.size() - off});
     }
@@ -209,7 +239,7 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
       off = (int)tasks.size();
       for (int q = 0; q < ns; q++) {
         int s = sups[q], T = (y.ncol[s] + 63) / 64;
-        for (int j = 0; j + dist < T; j++) tasks.push_back({s, j, j + dist, 0});
+        for (int j = 0; j + dist < T; j++) tasks.push_back({s, j, j + dist, 0, 0});
       }
       if ((int)tasks.size() > off) ops.push_back({INV_OFF, off, (int)tasks.size() - off});
     }
