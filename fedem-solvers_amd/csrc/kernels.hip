@@ -417,9 +417,10 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
   typename MF<T>::acc acc[2][2];
   zero_acc(acc);
-  for (int c0 = 0; c0 < kb; c0 += 64) {
+  T ri[16], rj[16], rd[16];
+  // global loads of the 64-column chunk at c0 (zero beyond the panel)
+  auto load = [&](int c0) {
     const int kc = min(64, kb - c0), p0 = k0 + c0;
-    T ri[16], rj[16], rd[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
@@ -428,6 +429,10 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
       ri[q] = (okm && I0 + i < ld) ? F[(int64_t)(I0 + i) + (int64_t)(p0 + m) * ld] : (T)0;
       rj[q] = (okm && J0 + i < ld) ? F[(int64_t)(J0 + i) + (int64_t)(p0 + m) * ld] : (T)0;
     }
+  };
+  load(0);
+  for (int c0 = 0; c0 < kb; c0 += 64) {
+    const int kc = min(64, kb - c0);
     if (c0 > 0) __syncthreads();  // the previous chunk's products are done with the tiles
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -436,6 +441,7 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
       sJ[i][m] = rj[q];
     }
     __syncthreads();
+    if (c0 + 64 < kb) load(c0 + 64);  // the next chunk's loads are in flight during the products
     mfma_tile<T>(sI, sJ, (kc + 3) & ~3, acc, lane, wr, wc);
   }
   const int lr = lane & 15, lk = lane >> 4;

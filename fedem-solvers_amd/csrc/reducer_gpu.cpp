@@ -159,7 +159,7 @@ DeviceReducer::~DeviceReducer() {
 }
 
 static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, std::vector<Op>& ops,
-                                  double& flops) {
+                                  double& flops, int panel) {
   flops = 0;
   for (int l = 0; l < y.nlevels; l++) {
     const int* sups = &y.lvsup[y.lvptr[l]];
@@ -173,11 +173,12 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
     if ((int)tasks.size() > off) ops.push_back({EXTADD, off, (int)tasks.size() - off});
     int maxnc = 0;
     for (int q = 0; q < ns; q++) maxnc = std::max(maxnc, y.ncol[sups[q]]);
-    // Pivot columns go in 128-wide panels of two 64-column halves: diagonal
-    // block and triangular solve of the first half, its update of the second
-    // half's columns only, diagonal block and solve of the second half, then
-    // one trailing update over the whole 128-column panel (a front with fewer
-    // than 128 pivot columns left takes plain 64-column panels).
+    // Pivot columns go in panels of P = 64 h columns (h sub-panels of 64):
+    // for each sub-panel the diagonal block and the triangular solve, then
+    // its update of the panel's remaining columns only; after the last
+    // sub-panel one trailing update over the whole panel, so the trailing
+    // block is read and written once per panel.  A front with fewer than P
+    // pivot columns left takes plain 64-column panels.
     auto add_diag_trsm = [&](int k0) {
       int off = (int)tasks.size();
       for (int q = 0; q < ns; q++) {
@@ -194,42 +195,36 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
       }
       if ((int)tasks.size() > off) ops.push_back({TRSM, off, (int)tasks.size() - off});
     };
-    // all 64x64 lower tiles of the trailing block from k0 + kb
-    auto add_full_update = [&](int s, int k0, int kb) {
-      int rem = y.nrow[s] - k0 - kb;
+    // 64x64 lower tiles of the trailing block from k0 + kb, tile columns < ntc (all when ntc < 0)
+    auto add_update = [&](int s, int k0, int kb, int ntc) {
+      const int rem = y.nrow[s] - k0 - kb;
       if (rem <= 0) return;
-      int T = (rem + 63) / 64;
+      const int T = (rem + 63) / 64;
       for (int ti = 0; ti < T; ti++)
-        for (int tj = 0; tj <= ti; tj++) tasks.push_back({s, ti, tj, k0, kb});
-      flops += (double)rem * (rem + 1) * kb;  // lower triangle incl. diagonal, 2 flops per FMA
+        for (int tj = 0; tj <= ti && (ntc < 0 || tj < ntc); tj++) tasks.push_back({s, ti, tj, k0, kb});
+      // lower-triangle entries (incl. diagonal) of the updated columns, 2 flops per FMA
+      const double w = ntc < 0 ? rem : std::min(rem, 64 * ntc);
+      flops += 2.0 * kb * (w * (w + 1) / 2 + (rem - w) * w);
     };
-    for (int k0 = 0; k0 < maxnc; k0 += 128) {
-      add_diag_trsm(k0);
-      int off = (int)tasks.size();
-      for (int q = 0; q < ns; q++) {
-        int s = sups[q];
-        const int rc = y.ncol[s] - k0;
-        if (rc <= 0) continue;
-        if (rc >= 128) {  // first half onto the second half's 64 columns (tile column 0)
-          const int rem = y.nrow[s] - k0 - 64;
-          for (int ti = 0; 64 * ti < rem; ti++) tasks.push_back({s, ti, 0, k0, 64});
-          flops += 2.0 * 64 * (64.0 * 65.0 / 2.0 + (double)(rem - 64) * 64.0);
-        } else {
-          add_full_update(s, k0, std::min(64, rc));
+    const int P = panel;
+    for (int k0 = 0; k0 < maxnc; k0 += P) {
+      for (int h = 0; h < P / 64 && k0 + 64 * h < maxnc; h++) {
+        const int kh = k0 + 64 * h;
+        add_diag_trsm(kh);
+        const int off = (int)tasks.size();
+        for (int q = 0; q < ns; q++) {
+          int s = sups[q];
+          const int rc = y.ncol[s] - k0;
+          if (rc <= 64 * h) continue;
+          if (rc >= P) {
+            if (h + 1 < P / 64) add_update(s, kh, 64, P / 64 - h - 1);  // the panel's remaining columns
+            else add_update(s, k0, P, -1);                              // the trailing block, whole panel
+          } else {
+            add_update(s, kh, std::min(64, rc - 64 * h), -1);
+          }
         }
+        if ((int)tasks.size() > off) ops.push_back({UPDATE, off, (int)tasks.size() - off});
       }
-      if ((int)tasks.size() > off) ops.push_back({UPDATE, off, (int)tasks.size() - off});
-      if (k0 + 64 >= maxnc) continue;
-      add_diag_trsm(k0 + 64);
-      off = (int)tasks.size();
-      for (int q = 0; q < ns; q++) {
-        int s = sups[q];
-        const int rc = y.ncol[s] - k0;
-        if (rc <= 64) continue;
-        if (rc >= 128) add_full_update(s, k0, 128);
-        else add_full_update(s, k0 + 64, rc - 64);
-      }
-      if ((int)tasks.size() > off) ops.push_back({UPDATE, off, (int)tasks.size() - off});
     }
     // explicit inverse of every pivot block: the diagonal tiles come from
     // k_diag, the tiles at growing distance from the diagonal follow
@@ -404,7 +399,7 @@ int DeviceReducer::upload(std::string& err) {
                d.chlist.p, d.relptr.p, d.rel.p, d.woff.p};
   std::vector<FTask> ft;
   d.fops.clear();
-  build_factor_schedule(y, ft, d.fops, upd_flops_);
+  build_factor_schedule(y, ft, d.fops, upd_flops_, std::min(256, std::max(64, (opt_.factor_panel / 64) * 64)));
   HIPCHK(d.ftasks.upload(ft));
   upd_launches_ = 0;
   for (auto& op : d.fops)

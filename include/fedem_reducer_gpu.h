@@ -94,7 +94,8 @@ int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags);
  * "lanczosMaxVectors", "lanczosTol" (relative Ritz residual), "eigenshift",
  * "seed" (Lanczos start block), "leafNodes" (nested-dissection leaf size),
  * "timeUpdates" (1: time every trailing-update launch with HIP events),
- * "solveSplitK" (longest contraction range of one solve task, a multiple
+ * "factorPanel" (pivot columns per trailing update: 64, 128, 192 or 256,
+ * default 128), "solveSplitK" (longest contraction range of one solve task, a multiple
  * of 64, default 128; longer ranges are split and reduced in order),
  * "mixedPrecision" (1: FP32 factor of the equilibrated K with FP32 MFMA
  * trailing and Schur updates and FP64 diagonal blocks; B and every K_ii

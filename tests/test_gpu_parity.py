@@ -292,3 +292,22 @@ def test_lanczos_block_sizes_and_unconverged_basis():
     R.analyze(ngen=20)
     with pytest.raises(gpu.ReducerError, match="did not converge"):
         R.reduce()
+
+
+@pytest.mark.parametrize("panel", [64, 192, 256])
+def test_factor_panel_widths_match_oracle(panel):
+    """Panels of 64 to 256 pivot columns per trailing update (sub-panel
+    updates restricted to the panel, then one update of the trailing block):
+    same reduction as the oracle on a block whose separator fronts exceed
+    256 pivot columns."""
+    part = parts.hex_block(16, 12, 10, 0.16, 0.12, 0.10, seed=7, n_ext=6)
+    R = gpu.GpuReducer()
+    R.set_part(part)
+    R.set_option("factorPanel", panel)
+    R.analyze(ngen=4)
+    assert R.info()["max_front"] > 256
+    R.reduce()
+    SK, SM, G = R.reduced()
+    o = orc.reduce(part, ngen=4)
+    assert rel(SK, o["SK"]) <= TOL and rel(SM, o["SM"]) <= TOL and rel(G, o["G"]) <= TOL
+    assert rel(R.B(), o["B"]) <= TOL
