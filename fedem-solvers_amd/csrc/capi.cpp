@@ -212,6 +212,21 @@ int fgpu_get_vgi(const fgpu_reducer* h, double* vgi) {
   return 0;
 }
 
+int fgpu_autotune(fgpu_reducer* h, int* panel, int* split) {
+  GUARD(h)
+  if (!h->analyzed) { h->err = "fgpu_autotune: not analyzed"; return -1; }
+  int P = 0, K = 0;
+  int rc = h->red.autotune(P, K, h->err);
+  h->reduced = false;
+  if (rc) return rc;
+  h->opts.factor_panel = P;
+  h->opts.solve_split = K;
+  if (panel) *panel = P;
+  if (split) *split = K;
+  return 0;
+  END_GUARD(h)
+}
+
 int fgpu_get_fe_mass(const fgpu_reducer* h, double* smass) {
   if (!h || !h->reduced || !smass) return -1;
   *smass = h->red.results().mass_fe;

@@ -1283,6 +1283,53 @@ int DeviceReducer::recover(int nrhs, const double* Ub, const double* Fi, double*
   return 0;
 }
 
+int DeviceReducer::autotune(int& panel, int& split, std::string& err) {
+  Impl& d = *d_;
+  if (!uploaded_ && upload(err)) return -3;
+  auto set_panel = [&](int P) -> int {
+    std::vector<FTask> ft;
+    d.fops.clear();
+    build_factor_schedule(sym_, ft, d.fops, upd_flops_, P);
+    HIPCHK(d.ftasks.upload(ft));
+    upd_launches_ = 0;
+    for (auto& op : d.fops)
+      if (op.kind == UPDATE) upd_launches_++;
+    d.g_factor.reset();
+    opt_.factor_panel = P;
+    return 0;
+  };
+  auto set_split = [&](int KS) {
+    d.solve_split = KS;
+    opt_.solve_split = KS;
+    d.plans.clear();
+    d.g_lanczos.reset();
+  };
+  // warm-up (allocations, first captures), then one timed reduction per candidate
+  if (int rc = reduce(err)) return rc;
+  double best = INFINITY;
+  int bestP = opt_.factor_panel;
+  for (int P : {64, 128, 192, 256}) {
+    if (set_panel(P)) return -3;
+    if (int rc = reduce(err)) return rc;
+    if (int rc = reduce(err)) return rc;  // the second run replays the captured factor graph
+    if (times_.factor < best) { best = times_.factor; bestP = P; }
+  }
+  if (set_panel(bestP)) return -3;
+  best = INFINITY;
+  int bestK = d.solve_split;
+  for (int KS : {64, 128, 256, 512}) {
+    set_split(KS);
+    if (int rc = reduce(err)) return rc;
+    if (int rc = reduce(err)) return rc;
+    const double t = times_.bsolve + times_.modes;
+    if (t < best) { best = t; bestK = KS; }
+  }
+  set_split(bestK);
+  panel = bestP;
+  split = bestK;
+  return 0;
+}
+
 size_t DeviceReducer::footprint_bytes() const {
   const Symbolic& y = sym_;
   const size_t nnz = y.rowptr.empty() ? 0 : (size_t)y.rowptr.back();

@@ -81,6 +81,10 @@ class DeviceReducer {
   int recover(int nrhs, const double* Ub, const double* Fi, double* Ui, std::string& err);
   // device memory one reduction of the analysed part needs (bytes, estimate)
   size_t footprint_bytes() const;
+  // Times this part's reduction for every candidate factor panel width and
+  // solve split and keeps the fastest pair (results then differ from the
+  // defaults' at rounding level only); call after analyze.
+  int autotune(int& panel, int& split, std::string& err);
 
   const Sam& sam() const { return sam_; }
   const Symbolic& sym() const { return sym_; }

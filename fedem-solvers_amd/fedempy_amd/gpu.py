@@ -48,6 +48,7 @@ def load_library(path=None):
     lib.fgpu_get_vgi.argtypes = [P, P]
     lib.fgpu_solve_internal.argtypes = [P, I, P, P]
     lib.fgpu_get_times.argtypes = [P, P]
+    lib.fgpu_autotune.argtypes = [P, P, P]
     lib.fgpu_get_info.argtypes = [P, P]
     lib.fgpu_get_csr.argtypes = [P, P, P, P, P, P]
     lib.fgpu_get_element_matrices.argtypes = [P, P, P, P]
@@ -198,6 +199,13 @@ class GpuReducer:
         X = np.zeros_like(R, order="F")
         self._check(self.lib.fgpu_solve_internal(self.h, R.shape[1], _ptr(R), _ptr(X)), "fgpu_solve_internal")
         return X
+
+    def autotune(self):
+        """Times this part's reduction over the factor panel widths and solve
+        splits and keeps the fastest; returns {"factorPanel", "solveSplitK"}."""
+        p, k = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.fgpu_autotune(self.h, ctypes.byref(p), ctypes.byref(k)), "fgpu_autotune")
+        return {"factorPanel": p.value, "solveSplitK": k.value}
 
     def times(self):
         t = np.zeros(11)

@@ -128,6 +128,14 @@ int fgpu_get_B(fgpu_reducer* h, double* B);
 int fgpu_get_phi(fgpu_reducer* h, double* phi);
 /* vgi = K_ii^-1 g_i (ndof1 x 3), the _V.fmx content (flags bit0). */
 int fgpu_get_vgi(const fgpu_reducer* h, double* vgi);
+/* Tile autotuner (no counterpart in the reference: device tuning).  After
+ * fgpu_analyze, times this part's reduction for the factor panel widths
+ * 64/128/192/256 and the solve splits 64/128/256/512 and keeps the fastest
+ * ("factorPanel", "solveSplitK"); the choice is returned so that a run can
+ * be repeated with the same settings.  Results of tuned settings differ from
+ * the defaults' at rounding level only. */
+int fgpu_autotune(fgpu_reducer* h, int* panel, int* split);
+
 /* Total FE mass: the sum over elements of the element mass EMASS (SMASS of
  * lhsINADD, inadd.f90:258-265 and 319), computed on the device from the
  * element mass matrices as the x-translation mass of each element. */

@@ -311,3 +311,22 @@ def test_factor_panel_widths_match_oracle(panel):
     o = orc.reduce(part, ngen=4)
     assert rel(SK, o["SK"]) <= TOL and rel(SM, o["SM"]) <= TOL and rel(G, o["G"]) <= TOL
     assert rel(R.B(), o["B"]) <= TOL
+
+
+def test_autotuned_settings_match_oracle():
+    """The tile autotuner picks a panel width and a solve split from its
+    candidate sets; the reduction with the picked settings matches the
+    oracle."""
+    part = parts.hex_block(12, 8, 6, 0.12, 0.08, 0.06, seed=9, n_ext=5)
+    R = gpu.GpuReducer()
+    R.set_part(part)
+    R.analyze(ngen=5)
+    choice = R.autotune()
+    print("autotune:", choice, flush=True)
+    assert choice["factorPanel"] in (64, 128, 192, 256) and choice["solveSplitK"] in (64, 128, 256, 512)
+    R.reduce()
+    SK, SM, G = R.reduced()
+    o = orc.reduce(part, ngen=5)
+    assert rel(SK, o["SK"]) <= TOL and rel(SM, o["SM"]) <= TOL and rel(G, o["G"]) <= TOL
+    lam, _ = R.eigen()
+    assert np.max(np.abs(lam - o["lam"]) / o["lam"]) <= 1e-9
