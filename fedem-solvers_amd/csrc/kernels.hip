@@ -934,6 +934,53 @@ __global__ __launch_bounds__(256) void k_sum_fixed(int n, const double* __restri
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// ---------------------------------------------------------------- Lanczos block orthonormalisation
+// Cholesky factor of the symmetrised n x n Gram matrix G = R^T R (R upper,
+// row-major) and R^{-1}, by one workgroup (n <= 64): at step j every thread
+// forms the pivot (same value in each), thread i > j the entry R[j][i]; the
+// inverse is solved column by column, thread t owning column t.  A
+// non-positive pivot sets *bad (the block has broken down).
+__global__ __launch_bounds__(64) void k_chol_inv(int n, const double* __restrict__ G, double* __restrict__ R,
+                                                 double* __restrict__ Ri, int* __restrict__ bad) {
+  __shared__ double A[64][65], U[64][65], V[64][65];
+  const int t = threadIdx.x;
+  if (t < n)
+    for (int i = 0; i < n; i++) {
+      A[i][t] = i == t ? G[i * n + t] : 0.5 * (G[i * n + t] + G[t * n + i]);
+      U[i][t] = 0.0;
+      V[i][t] = 0.0;
+    }
+  __syncthreads();
+  for (int j = 0; j < n; j++) {
+    double s = A[j][j];
+    for (int k = 0; k < j; k++) s -= U[k][j] * U[k][j];
+    if (!(s > 0.0)) {
+      if (t == 0) *bad = 1;
+      s = 1.0;
+    }
+    const double d = sqrt(s);
+    if (t == j) U[j][j] = d;
+    if (t > j && t < n) {
+      double v = A[j][t];
+      for (int k = 0; k < j; k++) v -= U[k][j] * U[k][t];
+      U[j][t] = v / d;
+    }
+    __syncthreads();
+  }
+  if (t < n) {
+    V[t][t] = 1.0 / U[t][t];
+    for (int i = t - 1; i >= 0; i--) {
+      double v = 0.0;
+      for (int k = i + 1; k <= t; k++) v += U[i][k] * V[k][t];
+      V[i][t] = -v / U[i][i];
+    }
+    for (int i = 0; i < n; i++) {
+      R[i * n + t] = U[i][t];
+      Ri[i * n + t] = V[i][t];
+    }
+  }
+}
+
 // ---------------------------------------------------------------- mixed precision
 // Symmetric diagonal equilibration of the stiffness matrix for the FP32
 // factor: s[r] = 1/sqrt(K[r][r]) for every perm row r of the CSR matrix
@@ -1122,6 +1169,9 @@ void launch_emass(hipStream_t st, int n, const int* elist, int nen, int d, const
 }
 void launch_sum_fixed(hipStream_t st, int n, const double* x, double* out) {
   k_sum_fixed<<<1, 256, 0, st>>>(n, x, out);
+}
+void launch_chol_inv(hipStream_t st, int n, const double* G, double* R, double* Ri, int* bad) {
+  k_chol_inv<<<1, 64, 0, st>>>(n, G, R, Ri, bad);
 }
 void launch_diag_scale(hipStream_t st, int n, const int* rowptr, const int* colind, const double* val, double* s) {
   if (n > 0) k_diag_scale<<<nblk(n, 256), 256, 0, st>>>(n, rowptr, colind, val, s);

@@ -57,6 +57,9 @@ void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, i
 // sign: nev doubles of scratch
 void launch_mode_signs(hipStream_t st, int n1, const int* p1, double* Phi, int64_t nrows, int nev, double* sign);
 void launch_identity_rows(hipStream_t st, double* X, int64_t nrows, int nrhs, int ndof1);
+// Lanczos: G (n x n, n <= 64) = R^T R, R and R^-1 row-major; *bad = 1 if
+// G is not positive definite
+void launch_chol_inv(hipStream_t st, int n, const double* G, double* R, double* Ri, int* bad);
 // mixed precision: s = 1/sqrt(diag K) per perm row; equilibrated lower
 // entries of K into the FP32 fronts; X = diag(rs) X diag(cs) (cs may be
 // null); Y = a X + b Y; partial sums of squares of x and y over

@@ -58,6 +58,28 @@ struct DBuf {
   }
 };
 
+// Pinned host memory (hipHostMalloc) for asynchronous device-to-host copies.
+template <class T>
+struct HBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  HBuf() = default;
+  HBuf(const HBuf&) = delete;
+  HBuf& operator=(const HBuf&) = delete;
+  ~HBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    n = count;
+    return hipHostMalloc((void**)&p, std::max<size_t>(1, count) * sizeof(T), hipHostMallocDefault);
+  }
+};
+
 enum OpKind {
   EXTADD, DIAG, TRSM, UPDATE, INV_OFF,
   FW_INIT, FW_INV, FW_UPD, FW_OUT, BW_INIT, BW_UPD, BW_INV,
@@ -141,6 +163,10 @@ struct DeviceReducer::Impl {
   std::vector<double> s22;  // mixed: Schur complement from the refined B (n2 x n2, column-major)
   // component modes
   DBuf<double> Q, P, Wl, Zl, Yl, Cc, Phi, Tmp;
+  DBuf<double> Cacc, Rdev;  // reorthogonalisation coefficients of a step; the two CholQR factors
+  DBuf<int> lzbad;
+  HBuf<double> hC, hR;
+  HBuf<int> hbad;
   std::vector<int> hexl, shl, bml;  // elements by type: HEX8, ANDES4, BEAM2
   int64_t wrows = 0;
   hipEvent_t ev[12]{};
@@ -763,7 +789,7 @@ int DeviceReducer::reduce(std::string& err) {
   std::vector<double> Sres;
   if (nev > 0) {
     int rc = 0;
-    const int bs = std::max(1, std::min(opt_.lanczos_block, n1));
+    const int bs = std::max(1, std::min(std::min(opt_.lanczos_block, 64), n1));  // block Gram matrices <= 64 x 64
     int mcap = opt_.lanczos_maxvec > 0 ? opt_.lanczos_maxvec : std::max(10 * nev + 40, 16 * bs);
     mcap = std::min(mcap, n1);
     mcap = std::max(bs, (mcap / bs) * bs);
@@ -779,6 +805,12 @@ int DeviceReducer::reduce(std::string& err) {
     HIPCHK(d.Cc.alloc((size_t)mcap * std::max(bs, nev) + 64));
     HIPCHK(d.part.alloc((size_t)xty_chunks(neq) * mcap * std::max(bs, nev)));
     HIPCHK(d.small.alloc((size_t)mcap * mcap + 64));
+    HIPCHK(d.Cacc.alloc((size_t)mcap * bs));
+    HIPCHK(d.Rdev.alloc((size_t)2 * bs * bs));
+    HIPCHK(d.lzbad.alloc(1));
+    HIPCHK(d.hC.alloc((size_t)mcap * bs));
+    HIPCHK(d.hR.alloc((size_t)2 * bs * bs));
+    HIPCHK(d.hbad.alloc(1));
     // start block: deterministic pseudo-random values on the internal rows
     {
       std::vector<double> q0((size_t)neq * bs, 0.0);
@@ -794,41 +826,41 @@ int DeviceReducer::reduce(std::string& err) {
         }
       HIPCHK(hipMemcpyAsync(d.Wl.p, q0.data(), q0.size() * sizeof(double), hipMemcpyHostToDevice, st));
     }
-    std::vector<double> G((size_t)bs * bs), R, Ri, Cfull;
+    std::vector<double> Cfull;
     std::vector<double> H((size_t)mcap * mcap, 0.0);
     // M-orthonormalise W into block j of Q (and M W into P); returns R with W = Q_j R
+    // Block M-orthonormalisation of W (CholQR twice): G = W^T M W = R^T R on
+    // the device, W <- W R^-1, M W <- (M W) R^-1; one host synchronisation
+    // per block, after which R_tot = R_2 R_1 and the step's
+    // reorthogonalisation coefficients (hC) are on the host.
     auto orth_block = [&](int j, std::vector<double>& Rtot) -> int {
-      std::vector<double> Racc((size_t)bs * bs, 0.0);
-      for (int i = 0; i < bs; i++) Racc[(size_t)i * bs + i] = 1.0;
+      HIPCHK(hipMemsetAsync(d.lzbad.p, 0, sizeof(int), st));
       for (int pass = 0; pass < 2; pass++) {
         launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs);
         launch_xty(st, neq, bs, bs, d.Wl.p, bs, d.Zl.p, bs, d.part.p, d.Cc.p);
-        HIPCHK(hipMemcpyAsync(G.data(), d.Cc.p, sizeof(double) * bs * bs, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        for (int a = 0; a < bs; a++)
-          for (int b = 0; b < a; b++) G[(size_t)a * bs + b] = G[(size_t)b * bs + a] = 0.5 * (G[(size_t)a * bs + b] + G[(size_t)b * bs + a]);
-        if (!chol_upper(bs, G, R)) { err = "Lanczos: block breakdown (Gram matrix not positive definite)"; return -6; }
-        inv_upper(bs, R, Ri);
-        HIPCHK(hipMemcpyAsync(d.small.p, Ri.data(), sizeof(double) * bs * bs, hipMemcpyHostToDevice, st));
+        launch_chol_inv(st, bs, d.Cc.p, d.Rdev.p + (size_t)pass * bs * bs, d.small.p, d.lzbad.p);
         // Q_j = W R^-1 (via Tmp), P_j = Z R^-1
         launch_gemm_small(st, neq, bs, bs, d.Wl.p, bs, d.small.p, d.Tmp.p, bs, 0.0, 1.0);
         launch_gemm_small(st, neq, bs, bs, d.Zl.p, bs, d.small.p, d.Yl.p, bs, 0.0, 1.0);
         HIPCHK(hipMemcpyAsync(d.Wl.p, d.Tmp.p, sizeof(double) * neq * bs, hipMemcpyDeviceToDevice, st));
-        // Racc = R * Racc
-        std::vector<double> T2((size_t)bs * bs, 0.0);
-        for (int a = 0; a < bs; a++)
-          for (int b = 0; b < bs; b++) {
-            double s = 0;
-            for (int k = 0; k < bs; k++) s += R[(size_t)a * bs + k] * Racc[(size_t)k * bs + b];
-            T2[(size_t)a * bs + b] = s;
-          }
-        Racc.swap(T2);
       }
       HIPCHK(hipMemcpy2DAsync(d.Q.p + (size_t)j * bs, sizeof(double) * mcap, d.Wl.p, sizeof(double) * bs,
                               sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
       HIPCHK(hipMemcpy2DAsync(d.P.p + (size_t)j * bs, sizeof(double) * mcap, d.Yl.p, sizeof(double) * bs,
                               sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
-      Rtot.swap(Racc);
+      HIPCHK(hipMemcpyAsync(d.hR.p, d.Rdev.p, sizeof(double) * 2 * bs * bs, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(d.hbad.p, d.lzbad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (*d.hbad.p) { err = "Lanczos: block breakdown (Gram matrix not positive definite)"; return -6; }
+      const double* R1 = d.hR.p;
+      const double* R2 = d.hR.p + (size_t)bs * bs;
+      Rtot.assign((size_t)bs * bs, 0.0);
+      for (int a = 0; a < bs; a++)
+        for (int b = 0; b < bs; b++) {
+          double s = 0;
+          for (int k = 0; k < bs; k++) s += R2[(size_t)a * bs + k] * R1[(size_t)k * bs + b];
+          Rtot[(size_t)a * bs + b] = s;
+        }
       return 0;
     };
     std::vector<double> R0;
@@ -905,22 +937,29 @@ int DeviceReducer::reduce(std::string& err) {
       else HIPCHK(run_graph(st, d.g_lanczos,
                        {pl, pl->tasks.p, pl->part.p, d.Tmp.p, d.Yl.p, d.Wl.p, d.W.p, d.V.p, d.pool.p, d.ipool.p,
                         d.S.foff, d.S.woff}, solve_step));
-      // two passes of full reorthogonalisation against Q_0..Q_j
-      Cfull.assign((size_t)mj * bs, 0.0);
+      // two passes of full reorthogonalisation against Q_0..Q_j; the
+      // coefficients add up on the device and reach the host with the next
+      // synchronisation
+      launch_fill(st, d.Cacc.p, (int64_t)mj * bs, 0.0);
       for (int pass = 0; pass < 2; pass++) {
         launch_xty(st, neq, mj, bs, d.P.p, mcap, d.Wl.p, bs, d.part.p, d.Cc.p);
         launch_gemm_small(st, neq, mj, bs, d.Q.p, mcap, d.Cc.p, d.Wl.p, bs, 1.0, -1.0);
-        std::vector<double> C((size_t)mj * bs);
-        HIPCHK(hipMemcpyAsync(C.data(), d.Cc.p, sizeof(double) * mj * bs, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        for (size_t q = 0; q < C.size(); q++) Cfull[q] += C[q];
+        launch_axpby(st, (int64_t)mj * bs, 1.0, d.Cc.p, 1.0, d.Cacc.p);
       }
-      for (int a = 0; a < mj; a++)
-        for (int b = 0; b < bs; b++) H[(size_t)a * mcap + j * bs + b] = Cfull[(size_t)a * bs + b];
+      HIPCHK(hipMemcpyAsync(d.hC.p, d.Cacc.p, sizeof(double) * mj * bs, hipMemcpyDeviceToHost, st));
+      auto put_coeffs = [&]() {
+        for (int a = 0; a < mj; a++)
+          for (int b = 0; b < bs; b++) H[(size_t)a * mcap + j * bs + b] = d.hC.p[(size_t)a * bs + b];
+      };
       nb = j + 1;
-      if (mj + bs > mcap) break;
+      if (mj + bs > mcap) {
+        HIPCHK(hipStreamSynchronize(st));
+        put_coeffs();
+        break;
+      }
       std::vector<double> Rn;
       if ((rc = orth_block(j + 1, Rn))) return rc;
+      put_coeffs();
       for (int a = 0; a < bs; a++)
         for (int b = 0; b < bs; b++) H[(size_t)(mj + a) * mcap + j * bs + b] = Rn[(size_t)a * bs + b];
       // Convergence tests on the projected problem run on a host thread while
