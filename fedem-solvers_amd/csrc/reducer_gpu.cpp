@@ -165,8 +165,9 @@ struct DeviceReducer::Impl {
   DBuf<double> Q, P, Wl, Zl, Yl, Cc, Phi, Tmp;
   DBuf<double> Cacc, Rdev;  // reorthogonalisation coefficients of a step; the two CholQR factors
   DBuf<int> lzbad;
-  HBuf<double> hC, hR;
+  HBuf<double> hC, hR;  // two slots each (step parity)
   HBuf<int> hbad;
+  hipEvent_t lz_ev[2]{};
   std::vector<int> hexl, shl, bml;  // elements by type: HEX8, ANDES4, BEAM2
   int64_t wrows = 0;
   hipEvent_t ev[12]{};
@@ -180,6 +181,8 @@ DeviceReducer::~DeviceReducer() {
     for (auto& e : d_->ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : d_->upd_ev) (void)hipEventDestroy(e);
+    for (auto& e : d_->lz_ev)
+      if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(st_);
   }
 }
@@ -808,9 +811,11 @@ int DeviceReducer::reduce(std::string& err) {
     HIPCHK(d.Cacc.alloc((size_t)mcap * bs));
     HIPCHK(d.Rdev.alloc((size_t)2 * bs * bs));
     HIPCHK(d.lzbad.alloc(1));
-    HIPCHK(d.hC.alloc((size_t)mcap * bs));
-    HIPCHK(d.hR.alloc((size_t)2 * bs * bs));
-    HIPCHK(d.hbad.alloc(1));
+    HIPCHK(d.hC.alloc((size_t)2 * mcap * bs));
+    HIPCHK(d.hR.alloc((size_t)4 * bs * bs));
+    HIPCHK(d.hbad.alloc(2));
+    for (auto& e : d.lz_ev)
+      if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     // start block: deterministic pseudo-random values on the internal rows
     {
       std::vector<double> q0((size_t)neq * bs, 0.0);
@@ -830,30 +835,37 @@ int DeviceReducer::reduce(std::string& err) {
     std::vector<double> H((size_t)mcap * mcap, 0.0);
     // M-orthonormalise W into block j of Q (and M W into P); returns R with W = Q_j R
     // Block M-orthonormalisation of W (CholQR twice): G = W^T M W = R^T R on
-    // the device, W <- W R^-1, M W <- (M W) R^-1; one host synchronisation
-    // per block, after which R_tot = R_2 R_1 and the step's
-    // reorthogonalisation coefficients (hC) are on the host.
-    auto orth_block = [&](int j, std::vector<double>& Rtot) -> int {
+    // the device, W <- W R^-1, M W <- (M W) R^-1, then Q_j = W and P_j = M W.
+    // orth_issue queues the work and the copies of both R factors and the
+    // breakdown flag to host slot `par`; orth_finish (after the stream has
+    // passed them) forms R_tot = R_2 R_1.
+    auto orth_issue = [&](int j, int par) -> int {
       HIPCHK(hipMemsetAsync(d.lzbad.p, 0, sizeof(int), st));
+      // M W is formed once: the second pass's M (W R_1^-1) is (M W) R_1^-1
+      launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs);
+      double* Zin = d.Zl.p;
+      double* Zout = d.Yl.p;
       for (int pass = 0; pass < 2; pass++) {
-        launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs);
-        launch_xty(st, neq, bs, bs, d.Wl.p, bs, d.Zl.p, bs, d.part.p, d.Cc.p);
+        launch_xty(st, neq, bs, bs, d.Wl.p, bs, Zin, bs, d.part.p, d.Cc.p);
         launch_chol_inv(st, bs, d.Cc.p, d.Rdev.p + (size_t)pass * bs * bs, d.small.p, d.lzbad.p);
-        // Q_j = W R^-1 (via Tmp), P_j = Z R^-1
         launch_gemm_small(st, neq, bs, bs, d.Wl.p, bs, d.small.p, d.Tmp.p, bs, 0.0, 1.0);
-        launch_gemm_small(st, neq, bs, bs, d.Zl.p, bs, d.small.p, d.Yl.p, bs, 0.0, 1.0);
+        launch_gemm_small(st, neq, bs, bs, Zin, bs, d.small.p, Zout, bs, 0.0, 1.0);
         HIPCHK(hipMemcpyAsync(d.Wl.p, d.Tmp.p, sizeof(double) * neq * bs, hipMemcpyDeviceToDevice, st));
+        std::swap(Zin, Zout);
       }
       HIPCHK(hipMemcpy2DAsync(d.Q.p + (size_t)j * bs, sizeof(double) * mcap, d.Wl.p, sizeof(double) * bs,
                               sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpy2DAsync(d.P.p + (size_t)j * bs, sizeof(double) * mcap, d.Yl.p, sizeof(double) * bs,
+      HIPCHK(hipMemcpy2DAsync(d.P.p + (size_t)j * bs, sizeof(double) * mcap, Zin, sizeof(double) * bs,
                               sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpyAsync(d.hR.p, d.Rdev.p, sizeof(double) * 2 * bs * bs, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipMemcpyAsync(d.hbad.p, d.lzbad.p, sizeof(int), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      if (*d.hbad.p) { err = "Lanczos: block breakdown (Gram matrix not positive definite)"; return -6; }
-      const double* R1 = d.hR.p;
-      const double* R2 = d.hR.p + (size_t)bs * bs;
+      HIPCHK(hipMemcpyAsync(d.hR.p + (size_t)par * 2 * bs * bs, d.Rdev.p, sizeof(double) * 2 * bs * bs,
+                            hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(d.hbad.p + par, d.lzbad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      return 0;
+    };
+    auto orth_finish = [&](int par, std::vector<double>& Rtot) -> int {
+      if (d.hbad.p[par]) { err = "Lanczos: block breakdown (Gram matrix not positive definite)"; return -6; }
+      const double* R1 = d.hR.p + (size_t)par * 2 * bs * bs;
+      const double* R2 = R1 + (size_t)bs * bs;
       Rtot.assign((size_t)bs * bs, 0.0);
       for (int a = 0; a < bs; a++)
         for (int b = 0; b < bs; b++) {
@@ -864,7 +876,9 @@ int DeviceReducer::reduce(std::string& err) {
       return 0;
     };
     std::vector<double> R0;
-    if ((rc = orth_block(0, R0))) return rc;
+    if ((rc = orth_issue(0, 0))) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    if ((rc = orth_finish(0, R0))) return rc;
     int nb = 0;  // number of completed blocks in H
     std::vector<double> theta, Svec, Rlast;
     bool conv = false;
@@ -921,9 +935,13 @@ int DeviceReducer::reduce(std::string& err) {
     std::future<Check> pending;
     int pending_step = 0;
     Check done;
-    for (int j = 0; (j + 1) * bs <= mcap; j++) {
-      const int mj = (j + 1) * bs;
-      // W = K_ii^{-1} (M Q_j); P_j is stored in Yl after orth_block
+    // Step j (device): W = K_ii^{-1} (M Q_j), two passes of full
+    // reorthogonalisation against Q_0..Q_j (coefficients added up on the
+    // device and copied to host slot j % 2), then the next block Q_{j+1}.
+    // Steps are queued one ahead of the host's bookkeeping, so the device
+    // never waits for the host between blocks.
+    auto issue_step = [&](int j) -> int {
+      const int mj = (j + 1) * bs, par = j & 1;
       HIPCHK(hipMemcpy2DAsync(d.Tmp.p, sizeof(double) * bs, d.P.p + (size_t)j * bs, sizeof(double) * mcap,
                               sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
       auto solve_step = [&]() {
@@ -937,29 +955,32 @@ int DeviceReducer::reduce(std::string& err) {
       else HIPCHK(run_graph(st, d.g_lanczos,
                        {pl, pl->tasks.p, pl->part.p, d.Tmp.p, d.Yl.p, d.Wl.p, d.W.p, d.V.p, d.pool.p, d.ipool.p,
                         d.S.foff, d.S.woff}, solve_step));
-      // two passes of full reorthogonalisation against Q_0..Q_j; the
-      // coefficients add up on the device and reach the host with the next
-      // synchronisation
       launch_fill(st, d.Cacc.p, (int64_t)mj * bs, 0.0);
       for (int pass = 0; pass < 2; pass++) {
         launch_xty(st, neq, mj, bs, d.P.p, mcap, d.Wl.p, bs, d.part.p, d.Cc.p);
         launch_gemm_small(st, neq, mj, bs, d.Q.p, mcap, d.Cc.p, d.Wl.p, bs, 1.0, -1.0);
         launch_axpby(st, (int64_t)mj * bs, 1.0, d.Cc.p, 1.0, d.Cacc.p);
       }
-      HIPCHK(hipMemcpyAsync(d.hC.p, d.Cacc.p, sizeof(double) * mj * bs, hipMemcpyDeviceToHost, st));
-      auto put_coeffs = [&]() {
-        for (int a = 0; a < mj; a++)
-          for (int b = 0; b < bs; b++) H[(size_t)a * mcap + j * bs + b] = d.hC.p[(size_t)a * bs + b];
-      };
+      HIPCHK(hipMemcpyAsync(d.hC.p + (size_t)par * mcap * bs, d.Cacc.p, sizeof(double) * mj * bs,
+                            hipMemcpyDeviceToHost, st));
+      if (mj + bs <= mcap)
+        if (int rc2 = orth_issue(j + 1, par)) return rc2;
+      HIPCHK(hipEventRecord(d.lz_ev[par], st));
+      return 0;
+    };
+    if ((rc = issue_step(0))) return rc;
+    for (int j = 0; (j + 1) * bs <= mcap; j++) {
+      const int mj = (j + 1) * bs, par = j & 1;
+      const bool last = mj + bs > mcap;
+      if (!last && (rc = issue_step(j + 1))) return rc;
+      HIPCHK(hipEventSynchronize(d.lz_ev[par]));
+      const double* hc = d.hC.p + (size_t)par * mcap * bs;
+      for (int a = 0; a < mj; a++)
+        for (int b = 0; b < bs; b++) H[(size_t)a * mcap + j * bs + b] = hc[(size_t)a * bs + b];
       nb = j + 1;
-      if (mj + bs > mcap) {
-        HIPCHK(hipStreamSynchronize(st));
-        put_coeffs();
-        break;
-      }
+      if (last) break;
       std::vector<double> Rn;
-      if ((rc = orth_block(j + 1, Rn))) return rc;
-      put_coeffs();
+      if ((rc = orth_finish(par, Rn))) return rc;
       for (int a = 0; a < bs; a++)
         for (int b = 0; b < bs; b++) H[(size_t)(mj + a) * mcap + j * bs + b] = Rn[(size_t)a * bs + b];
       // Convergence tests on the projected problem run on a host thread while
@@ -967,7 +988,12 @@ int DeviceReducer::reduce(std::string& err) {
       // collected after block j + 2 whatever the timing, so where the
       // iteration stops (and so every result bit) depends on the data only.
       if (pending.valid() && j >= pending_step + 2) {
+        const auto tw = std::chrono::steady_clock::now();
         Check c = pending.get();
+        if (std::getenv("FGPU_TRACE_LANCZOS"))
+          std::fprintf(stderr, "lanczos step %d: check of %d vectors joined after %.3f ms wait, worst %.2e\n", j,
+                       c.mj, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count(),
+                       c.worst);
         if (c.ok) {
           done = std::move(c);
           conv = true;
