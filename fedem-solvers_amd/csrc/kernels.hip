@@ -679,13 +679,14 @@ __global__ __launch_bounds__(256) void k_mv_red(const SolveTask* __restrict__ ta
 // Forward-solve front initialisation, task = (front, 64-row chunk r0): pivot
 // rows from the global right-hand side R (row-major, perm rows; the Schur
 // front takes the boundary rows), update rows zero, then the children's
-// update rows falling in the chunk added child after child.
+// update rows falling in the chunk added child after child.  The task's
+// k0/kn index its (child, first, end) triples in crange.
 __global__ __launch_bounds__(256) void k_fw_init(const SolveTask* __restrict__ tasks, DevSym S,
-                                                 const double* __restrict__ R, double* __restrict__ W, int nrhs,
-                                                 int ndof1) {
+                                                 const int* __restrict__ crange, const double* __restrict__ R,
+                                                 double* __restrict__ W, int nrhs, int ndof1) {
   const SolveTask t = tasks[blockIdx.x];
   const int sn = t.sn, nc = S.ncol[sn], c0 = S.col0[sn];
-  const int r0 = t.r0, r1 = t.r0 + t.rn;
+  const int r0 = t.r0;
   double* Wf = W + S.woff[sn] * (int64_t)nrhs;
   const int tid = threadIdx.x;
   const int base = nc > 0 ? c0 : ndof1;
@@ -695,17 +696,11 @@ __global__ __launch_bounds__(256) void k_fw_init(const SolveTask* __restrict__ t
     if (i < nc || nc == 0) v = R[(int64_t)(base + i) * nrhs + j];
     Wf[(int64_t)i * nrhs + j] = v;
   }
-  for (int q = S.chptr[sn]; q < S.chptr[sn + 1]; q++) {
-    const int c = S.chlist[q];
-    const int ncc = S.ncol[c], nu = S.nrow[c] - ncc;
+  for (int q = 0; q < t.kn; q++) {
+    const int* cr = crange + 3 * (t.k0 + q);
+    const int c = cr[0], ka = cr[1], kb = cr[2];
     const int* rel = S.rel + S.relptr[c];
-    int lo = 0, hi = nu;
-    while (lo < hi) { int m = (lo + hi) >> 1; if (rel[m] < r0) lo = m + 1; else hi = m; }
-    const int ka = lo;
-    hi = nu;
-    while (lo < hi) { int m = (lo + hi) >> 1; if (rel[m] < r1) lo = m + 1; else hi = m; }
-    const int kb = lo;
-    const double* Wc = W + (S.woff[c] + ncc) * (int64_t)nrhs;
+    const double* Wc = W + (S.woff[c] + S.ncol[c]) * (int64_t)nrhs;
     __syncthreads();
     for (int idx = tid; idx < (kb - ka) * nrhs; idx += 256) {
       const int k = ka + idx / nrhs, j = idx % nrhs;
@@ -1122,9 +1117,9 @@ void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, d
                    const double* part, int nrhs, int mode) {
   if (n > 0) k_mv_red<<<n, 256, 0, st>>>(t, S, W, V, X, part, nrhs, mode);
 }
-void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* R, double* W,
-                    int nrhs, int ndof1) {
-  if (n > 0) k_fw_init<<<n, 256, 0, st>>>(t, S, R, W, nrhs, ndof1);
+void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const int* crange, const double* R,
+                    double* W, int nrhs, int ndof1) {
+  if (n > 0) k_fw_init<<<n, 256, 0, st>>>(t, S, crange, R, W, nrhs, ndof1);
 }
 void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y,
                     const double* X, double* W, int nrhs) {

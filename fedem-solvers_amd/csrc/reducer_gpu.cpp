@@ -94,7 +94,7 @@ struct SolvePlan {
   int nslots = 0;  // 64x64 partial buffers of the split-K solve tasks
   std::vector<Op> fw_int, fw_all, bw;
   DBuf<SolveTask> tasks;
-  DBuf<int> sns;
+  DBuf<int> crange;  // forward-init tasks: (child, first, end) of the child update rows each adds
   DBuf<double> part;
 };
 
@@ -271,18 +271,38 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
 }
 
 static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<SolveTask>& tasks,
-                             std::vector<int>& sns, int KS) {
+                             std::vector<int>& crange, int KS) {
   const int nrhs = sp.nrhs, schur = y.nsup - 1;
   // (front, 64-row chunk) tasks over all rows (init) or the solved rows (out)
+  // a forward-init task (front s, rows r0..r0+rn-1) gets the update rows of
+  // each child that land in its rows as (child, first, end) triples, found
+  // here once instead of by a search in every solve
   auto add_sns = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list) {
     int off = (int)tasks.size();
     for (int s : list) {
       int nr = (kind == FW_OUT && s != schur) ? y.ncol[s] : y.nrow[s];
-      for (int r0 = 0; r0 < nr; r0 += 64) tasks.push_back({s, r0, std::min(64, nr - r0), 0, 0, 0, -1, 0});
+      for (int r0 = 0; r0 < nr; r0 += 64) {
+        SolveTask t{s, r0, std::min(64, nr - r0), 0, 0, 0, -1, 0};
+        if (kind == FW_INIT) {
+          t.k0 = (int)crange.size() / 3;
+          for (int q = y.chptr[s]; q < y.chptr[s + 1]; q++) {
+            const int c = y.chlist[q], nu = y.nrow[c] - y.ncol[c];
+            const int* rel = y.rel.data() + y.relptr[c];
+            const int ka = (int)(std::lower_bound(rel, rel + nu, r0) - rel);
+            const int kb = (int)(std::lower_bound(rel, rel + nu, r0 + t.rn) - rel);
+            if (kb > ka) {
+              crange.push_back(c);
+              crange.push_back(ka);
+              crange.push_back(kb);
+              t.kn++;
+            }
+          }
+        }
+        tasks.push_back(t);
+      }
     }
     if ((int)tasks.size() > off) ops.push_back({kind, off, (int)tasks.size() - off});
   };
-  (void)sns;
   // GEMM tasks over the row tiles of a front and all right-hand-side tiles;
   // K ranges longer than KS are split and reduced in slot order
   auto add_tiles = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list, bool upd_rows) {
@@ -474,10 +494,10 @@ static int get_plan(const Symbolic& y, DeviceReducer::Impl& d, int nrhs, SolvePl
   auto sp = std::make_unique<SolvePlan>();
   sp->nrhs = nrhs;
   std::vector<SolveTask> tasks;
-  std::vector<int> sns;
-  build_solve_plan(y, *sp, tasks, sns, d.solve_split);
+  std::vector<int> crange;
+  build_solve_plan(y, *sp, tasks, crange, d.solve_split);
   HIPCHK(sp->tasks.upload(tasks));
-  HIPCHK(sp->sns.upload(sns));
+  HIPCHK(sp->crange.upload(crange));
   HIPCHK(sp->part.alloc((size_t)std::max(1, sp->nslots) * 4096));
   out = sp.get();
   d.plans[nrhs] = std::move(sp);
@@ -518,7 +538,7 @@ static void run_forward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan&
   const std::vector<Op>& ops = all ? sp.fw_all : sp.fw_int;
   for (const Op& op : ops) {
     switch (op.kind) {
-      case FW_INIT: launch_fw_init(st, sp.tasks.p + op.off, op.n, d.S, R, d.W.p, sp.nrhs, ndof1); break;
+      case FW_INIT: launch_fw_init(st, sp.tasks.p + op.off, op.n, d.S, sp.crange.p, R, d.W.p, sp.nrhs, ndof1); break;
       case FW_OUT:
         if (d.mixed)
           launch_fw_out(st, sp.tasks.p + op.off, op.n, d.S, d.pool32.p, d.W.p, d.V.p, Yout, sp.nrhs, scale, ndof1);
