@@ -192,26 +192,23 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
 // ---------------------------------------------------------------- extend-add
 // Task: sn = parent front, a = first parent column of a 16-column block.  The
 // workgroup adds, child after child, every child update-matrix column whose
-// parent column falls in its block.  Blocks are disjoint, so no two
-// workgroups touch the same parent entry; children are added in list order.
+// parent column falls in its block; the task's b/kb index the (child,
+// first, end) column ranges in xrange, found when the schedule was built.
+// Blocks are disjoint, so no two workgroups touch the same parent entry;
+// children are added in list order.
 template <class T>
 __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ tasks, DevSym S,
-                                                    T* __restrict__ pool) {
+                                                    const int* __restrict__ xrange, T* __restrict__ pool) {
   const FTask t = tasks[blockIdx.x];
-  const int sn = t.sn, j0 = t.a, j1 = t.a + 16;
+  const int sn = t.sn;
   const int ldp = S.nrow[sn];
   T* P = pool + S.foff[sn];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int q = S.chptr[sn]; q < S.chptr[sn + 1]; q++) {
-    const int c = S.chlist[q];
+  for (int q = 0; q < t.kb; q++) {
+    const int* xr = xrange + 3 * (t.b + q);
+    const int c = xr[0], a = xr[1], b = xr[2];
     const int nc = S.ncol[c], ldc = S.nrow[c], nu = ldc - nc;
     const int* rel = S.rel + S.relptr[c];
-    int lo = 0, hi = nu;
-    while (lo < hi) { int m = (lo + hi) >> 1; if (rel[m] < j0) lo = m + 1; else hi = m; }
-    int a = lo;
-    hi = nu;
-    while (lo < hi) { int m = (lo + hi) >> 1; if (rel[m] < j1) lo = m + 1; else hi = m; }
-    const int b = lo;
     const T* C = pool + S.foff[c] + (int64_t)nc * ldc + nc;
     for (int jj = a + w; jj < b; jj += 4) {
       const int pc = rel[jj];
@@ -1073,8 +1070,9 @@ void launch_fill(hipStream_t st, double* p, int64_t n, double v) {
   if (n > 0) k_fill<<<(unsigned)std::min<int64_t>(nblk(n, 256), 16384), 256, 0, st>>>(p, n, v);
 }
 #define FGPU_FACTOR_LAUNCHERS(T)                                                                           \
-  void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool) {                \
-    if (n > 0) k_extend_add<T><<<n, 256, 0, st>>>(t, S, pool);                                               \
+  void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, const int* xrange,        \
+                         T* pool) {                                                                        \
+    if (n > 0) k_extend_add<T><<<n, 256, 0, st>>>(t, S, xrange, pool);                                       \
   }                                                                                                        \
   void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,   \
                    T* ipool, int* piv) {                                                                   \

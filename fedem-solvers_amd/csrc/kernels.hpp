@@ -22,7 +22,8 @@ void launch_fill(hipStream_t st, double* p, int64_t n, double v);
 // (FP64 path) and float (mixed precision: FP32 fronts and FP32 MFMA updates,
 // diagonal blocks factored in FP64; solve vectors stay FP64).
 #define FGPU_FACTOR_DECLS(T)                                                                              \
-  void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool);                \
+  void launch_extend_add(hipStream_t st, const FTask* t, int n, const DevSym& S, const int* xrange,        \
+                         T* pool);                                                                         \
   void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,  \
                    T* ipool, int* piv);                                                                   \
   void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,  \

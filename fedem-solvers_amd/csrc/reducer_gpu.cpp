@@ -147,6 +147,7 @@ struct DeviceReducer::Impl {
   DBuf<int64_t> foff, woff, cidx, fmap, ioff;
   DevSym S{};
   DBuf<FTask> ftasks;
+  DBuf<int> xrange;  // extend-add tasks: (child, first, end) column ranges
   std::vector<Op> fops;
   std::map<int, std::unique_ptr<SolvePlan>> plans;
   // values
@@ -188,7 +189,8 @@ DeviceReducer::~DeviceReducer() {
 }
 
 static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, std::vector<Op>& ops,
-                                  double& flops, int panel) {
+                                  std::vector<int>& xrange, double& flops, int panel) {
+  xrange.clear();
   flops = 0;
   for (int l = 0; l < y.nlevels; l++) {
     const int* sups = &y.lvsup[y.lvptr[l]];
@@ -197,7 +199,25 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
     for (int q = 0; q < ns; q++) {
       int s = sups[q];
       if (y.chptr[s + 1] == y.chptr[s]) continue;
-      for (int j0 = 0; j0 < y.nrow[s]; j0 += 16) tasks.push_back({s, j0, 0, 0, 0});
+      // extend-add task (front s, 16-column block j0): the (child, first,
+      // end) ranges of child update columns landing in the block, in child
+      // order, at xrange[3 b ..], kb of them
+      for (int j0 = 0; j0 < y.nrow[s]; j0 += 16) {
+        FTask t{s, j0, (int)xrange.size() / 3, 0, 0};
+        for (int q = y.chptr[s]; q < y.chptr[s + 1]; q++) {
+          const int c = y.chlist[q], nu = y.nrow[c] - y.ncol[c];
+          const int* rel = y.rel.data() + y.relptr[c];
+          const int ka = (int)(std::lower_bound(rel, rel + nu, j0) - rel);
+          const int kb = (int)(std::lower_bound(rel, rel + nu, j0 + 16) - rel);
+          if (kb > ka) {
+            xrange.push_back(c);
+            xrange.push_back(ka);
+            xrange.push_back(kb);
+            t.kb++;
+          }
+        }
+        tasks.push_back(t);
+      }
     }
     if ((int)tasks.size() > off) ops.push_back({EXTADD, off, (int)tasks.size() - off});
     int maxnc = 0;
@@ -448,8 +468,10 @@ int DeviceReducer::upload(std::string& err) {
                d.chlist.p, d.relptr.p, d.rel.p, d.woff.p};
   std::vector<FTask> ft;
   d.fops.clear();
-  build_factor_schedule(y, ft, d.fops, upd_flops_, std::min(256, std::max(64, (opt_.factor_panel / 64) * 64)));
+  std::vector<int> xr;
+  build_factor_schedule(y, ft, d.fops, xr, upd_flops_, std::min(256, std::max(64, (opt_.factor_panel / 64) * 64)));
   HIPCHK(d.ftasks.upload(ft));
+  HIPCHK(d.xrange.upload(xr));
   upd_launches_ = 0;
   for (auto& op : d.fops)
     if (op.kind == UPDATE) upd_launches_++;
@@ -712,7 +734,7 @@ int DeviceReducer::reduce(std::string& err) {
   auto factor_with = [&](auto* pool, auto* ipool) {
     for (const Op& op : d.fops) {
       switch (op.kind) {
-        case EXTADD: launch_extend_add(st, d.ftasks.p + op.off, op.n, d.S, pool); break;
+        case EXTADD: launch_extend_add(st, d.ftasks.p + op.off, op.n, d.S, d.xrange.p, pool); break;
         case DIAG: launch_diag(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool, d.piv.p); break;
         case TRSM: launch_trsm(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool); break;
         case UPDATE:
@@ -734,7 +756,8 @@ int DeviceReducer::reduce(std::string& err) {
   } else {
     // the factor schedule is fixed per symbolic structure: one graph
     HIPCHK(run_graph(st, d.g_factor,
-                     {d.ftasks.p, d.pool.p, d.ipool.p, d.pool32.p, d.ipool32.p, d.ioff.p, d.piv.p, d.S.foff, d.S.rows},
+                     {d.ftasks.p, d.xrange.p, d.pool.p, d.ipool.p, d.pool32.p, d.ipool32.p, d.ioff.p, d.piv.p,
+                      d.S.foff, d.S.rows},
                      factor_ops));
   }
   HIPCHK(hipGetLastError());
@@ -1373,9 +1396,11 @@ int DeviceReducer::autotune(int& panel, int& split, std::string& err) {
   if (!uploaded_ && upload(err)) return -3;
   auto set_panel = [&](int P) -> int {
     std::vector<FTask> ft;
+    std::vector<int> xr;
     d.fops.clear();
-    build_factor_schedule(sym_, ft, d.fops, upd_flops_, P);
+    build_factor_schedule(sym_, ft, d.fops, xr, upd_flops_, P);
     HIPCHK(d.ftasks.upload(ft));
+    HIPCHK(d.xrange.upload(xr));
     upd_launches_ = 0;
     for (auto& op : d.fops)
       if (op.kind == UPDATE) upd_launches_++;
