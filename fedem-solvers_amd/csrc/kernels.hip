@@ -784,6 +784,7 @@ __global__ __launch_bounds__(256) void k_xty_part(int n, int na, int nb, const d
   const int r0 = blockIdx.x * XTY_ROWS, a0 = blockIdx.y * 64, b0 = blockIdx.z * 64;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nbt = min(64, nb - b0), nat = min(64, na - a0);
+  const bool narrow = nb <= 16;
   double acc[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) acc[q] = 0.0;
@@ -805,15 +806,47 @@ __global__ __launch_bounds__(256) void k_xty_part(int n, int na, int nb, const d
       sY[idx >> 6][idx & 63] = ry[q];
     }
     __syncthreads();
-    // sY is zero beyond nbt: same unguarded loop in every thread
-#pragma unroll 4
-    for (int rr = 0; rr < 32; rr++) {
-      const double xv = sX[rr][lane];
+    // sY is zero beyond nbt: same unguarded loop in every thread.  Up to 16
+    // columns of Y (the Lanczos block) the four waves split the rows of the
+    // chunk instead of the columns, so none of them multiplies zeros.
+    if (narrow) {
+#pragma unroll 2
+      for (int rr = wv; rr < 32; rr += 4) {
+        const double xv = sX[rr][lane];
 #pragma unroll
-      for (int q = 0; q < 16; q++) acc[q] += xv * sY[rr][wv * 16 + q];
+        for (int q = 0; q < 16; q++) acc[q] += xv * sY[rr][q];
+      }
+    } else {
+#pragma unroll 4
+      for (int rr = 0; rr < 32; rr++) {
+        const double xv = sX[rr][lane];
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc[q] += xv * sY[rr][wv * 16 + q];
+      }
     }
   }
   double* P = part + (int64_t)blockIdx.x * na * nb;
+  if (narrow) {
+    // the waves' partial sums, added in wave order by wave 0: waves 1 and 2
+    // leave theirs in rows 0-15 and 16-31 of sX, wave 3 in rows 0-15 of sY
+    __syncthreads();
+    if (wv == 1 || wv == 2) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) sX[(wv - 1) * 16 + q][lane] = acc[q];
+    } else if (wv == 3) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) sY[q][lane] = acc[q];
+    }
+    __syncthreads();
+    if (wv == 0 && lane < nat) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const double v = ((acc[q] + sX[q][lane]) + sX[16 + q][lane]) + sY[q][lane];
+        if (q < nbt) P[(int64_t)(a0 + lane) * nb + b0 + q] = v;
+      }
+    }
+    return;
+  }
   if (lane < nat) {
 #pragma unroll
     for (int q = 0; q < 16; q++) {
