@@ -919,6 +919,46 @@ __global__ __launch_bounds__(256) void k_gemm_small(int n, int na, int nb, const
   Y[(int64_t)r * ldy + j] = y + beta * s;
 }
 
+// Y = alpha Y + beta X S for S with at most 16 columns: one thread per row of
+// X keeps the row's 16 sums in registers (S staged in LDS, padded to 16
+// columns), so X is read once, in row order, instead of once per column.
+// Same summation order as k_gemm_small.
+__global__ __launch_bounds__(256) void k_gemm_rows16(int n, int na, int nb, const double* __restrict__ X, int ldx,
+                                                     const double* __restrict__ Sm, double* __restrict__ Y,
+                                                     int ldy, double alpha, double beta) {
+  extern __shared__ double sS[];
+  for (int i = threadIdx.x; i < na * 16; i += blockDim.x) {
+    const int a = i >> 4, j = i & 15;
+    sS[i] = j < nb ? Sm[a * nb + j] : 0.0;
+  }
+  __syncthreads();
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double* x = X + (int64_t)r * ldx;
+  double acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) acc[j] = 0.0;
+  int a = 0;
+  for (; a + 4 <= na; a += 4) {
+    double xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) xv[u] = x[a + u];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int j = 0; j < 16; j++) acc[j] += xv[u] * sS[(a + u) * 16 + j];
+  }
+  for (; a < na; a++) {
+    const double xv = x[a];
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[j] += xv * sS[a * 16 + j];
+  }
+  double* y = Y + (int64_t)r * ldy;
+#pragma unroll
+  for (int j = 0; j < 16; j++)
+    if (j < nb) y[j] = ((alpha == 0.0) ? 0.0 : alpha * y[j]) + beta * acc[j];
+}
+
 // X[(ndof1 + b) * nrhs + b] = 1, all other entries 0 (the [B; I] identity rows)
 __global__ void k_identity_rows(double* __restrict__ X, int64_t nrows, int nrhs, int ndof1) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1174,6 +1214,15 @@ void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx,
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta) {
   if (n <= 0 || nb <= 0) return;
+  if (nb <= 16) {  // one thread per row, S padded to 16 columns, in chunks of 512 rows (64 KiB)
+    for (int a0 = 0; a0 < na || a0 == 0; a0 += 512) {
+      const int ka = std::min(512, na - a0);
+      k_gemm_rows16<<<nblk(n, 256), 256, sizeof(double) * 16 * std::max(ka, 1), st>>>(
+          n, ka, nb, X + a0, ldx, Sm + (size_t)a0 * nb, Y, ldy, a0 == 0 ? alpha : 1.0, beta);
+      if (na <= 0) break;
+    }
+    return;
+  }
   // S is staged in LDS in row chunks of at most 64 KiB; chunk k adds
   // beta X_k S_k to the result of the chunks before it (fixed order)
   const int rows = std::max(1, (int)(65536 / (sizeof(double) * nb)));
