@@ -559,7 +559,7 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
                                             double* __restrict__ V, double* __restrict__ X,
                                             double* __restrict__ part, int nrhs) {
   __shared__ double sA[64][65];
-  __shared__ double sB[64][NJ];
+  __shared__ double sB[NJ][65];  // B tile transposed: sB[j][k]
   const SolveTask t = tasks[blockIdx.x];
   const int sn = t.sn, ld = S.nrow[sn], nc = S.ncol[sn];
   const T* F = pool + S.foff[sn];
@@ -571,11 +571,11 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
   const int r0 = t.r0, rn = t.rn;
   const double* Bsrc = MODE == 1 ? Vf : (MODE == 2 ? Wf + (int64_t)nc * nrhs : Wf);
   constexpr bool rowmajorA = (MODE == 0 || MODE == 1);
-  constexpr int CPT = NJ / 4;   // right-hand sides per thread: wv*CPT .. wv*CPT+CPT-1
+  constexpr int NT = NJ / 16;   // 16x16 MFMA result tiles per wave: rows 16 wv .. 16 wv + 15, all NJ columns
   constexpr int BPT = NJ / 4;   // B-tile values loaded per thread (64 x NJ tile)
-  double acc[CPT];
+  f64x4 acc[NT];
 #pragma unroll
-  for (int u = 0; u < CPT; u++) acc[u] = 0.0;
+  for (int y = 0; y < NT; y++) acc[y] = f64x4{0.0, 0.0, 0.0, 0.0};
   const int kend = t.k0 + t.kn;
   double ra[16], rb[BPT];
   // global loads of the 64-wide chunk starting at kc (zero outside the task)
@@ -616,31 +616,40 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
 #pragma unroll
     for (int q = 0; q < BPT; q++) {
       const int idx = tid + 256 * q;
-      sB[idx / NJ][idx % NJ] = rb[q];
+      sB[idx % NJ][idx / NJ] = rb[q];
     }
     __syncthreads();
     // the next chunk's loads are in flight while this one is multiplied
     if (kc + 64 < kend) load(kc + 64);
-    // sB is zero beyond nj and sA beyond kn, so every thread runs the same unguarded loop
-#pragma unroll 4
-    for (int k = 0; k < kn; k++) {
-      const double a = sA[lane][k];
+    // v_mfma_f64_16x16x4f64 over k in steps of 4 (sA and sB are zero beyond
+    // kn and nj): the A operand takes the sB rows (columns j), the B operand
+    // the sA rows (target rows), as in mfma_tile
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int m0 = 0; m0 < kn; m0 += 4) {
+      const double aa = sA[16 * wv + lr][m0 + lk];
 #pragma unroll
-      for (int u = 0; u < CPT; u++) acc[u] += a * sB[k][wv * CPT + u];
+      for (int y = 0; y < NT; y++)
+        acc[y] = __builtin_amdgcn_mfma_f64_16x16x4f64(sB[16 * y + lr][m0 + lk], aa, acc[y], 0, 0, 0);
     }
   }
+  // lane holds target row 16 wv + (lane & 15), columns 16 y + (lane >> 4) + 4 r
+  const int i = 16 * wv + (lane & 15), lk = lane >> 4;
   if (t.slot >= 0) {
     double* P = part + (int64_t)t.slot * 4096;
 #pragma unroll
-    for (int u = 0; u < CPT; u++) P[lane * 64 + wv * CPT + u] = acc[u];
+    for (int y = 0; y < NT; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) P[i * 64 + 16 * y + lk + 4 * r] = acc[y][r];
     return;
   }
-  if (lane >= rn) return;
+  if (i >= rn) return;
 #pragma unroll
-  for (int u = 0; u < CPT; u++) {
-    const int jj = wv * CPT + u;
-    if (jj < nj) mv_apply(MODE, S, sn, nc, r0 + lane, t.j0 + jj, nrhs, acc[u], Wf, Vf, X);
-  }
+  for (int y = 0; y < NT; y++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int jj = 16 * y + lk + 4 * r;
+      if (jj < nj) mv_apply(MODE, S, sn, nc, r0 + i, t.j0 + jj, nrhs, acc[y][r], Wf, Vf, X);
+    }
 }
 
 // Reduce task: slots k0 .. k0+kn-1 hold the partials of rows r0.. (rn) and
