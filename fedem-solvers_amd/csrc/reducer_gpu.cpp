@@ -158,7 +158,7 @@ struct DeviceReducer::Impl {
   // mixed precision: FP32 fronts and pivot-block inverses, equilibration
   // scale, refinement work (correction, residual, K X) and norm partials
   bool mixed = false;
-  int solve_split = 128;
+  int solve_split = 256;
   DBuf<float> pool32, ipool32;
   DBuf<double> dscale, dcs, Dl, Rr, Zr, irpart;
   std::vector<double> s22;  // mixed: Schur complement from the refined B (n2 x n2, column-major)
