@@ -763,9 +763,16 @@ __global__ __launch_bounds__(256) void k_spmm(int n, int nrhs, const int* __rest
                                               const double* __restrict__ val,
                                               const double* __restrict__ X, int ldx,
                                               double* __restrict__ Z, int ldz) {
-  int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int r = (int)(g / nrhs), j = (int)(g % nrhs);
-  if (r >= n) return;
+  // threads sweep all rows for one block of columns (all of them up to 128,
+  // else 64) before the next block, so the rows of X that a sweep gathers
+  // stay in cache
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int wcol = nrhs <= 128 ? nrhs : 64;
+  const int64_t per = (int64_t)n * wcol;
+  const int jb = (int)(g / per);
+  const int64_t rem = g - (int64_t)jb * per;
+  const int r = (int)(rem / wcol), j = jb * wcol + (int)(rem % wcol);
+  if (r >= n || j >= nrhs) return;
   const int kb = rowptr[r], ke = rowptr[r + 1];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   int k = kb;
@@ -781,23 +788,23 @@ __global__ __launch_bounds__(256) void k_spmm(int n, int nrhs, const int* __rest
   Z[(int64_t)r * ldz + j] = (s0 + s1) + (s2 + s3);
 }
 
-// Partial C = X^T Y: blockIdx.x = chunk of XTY_ROWS rows, blockIdx.y = 64
+// Partial C = X^T Y: blockIdx.x = chunk of crows rows, blockIdx.y = 64
 // columns of X (a), blockIdx.z = 64 columns of Y (b); lane -> a, wave -> b
 // (b = 16*wave + u).  part[chunk][a][b] (na x nb row-major per chunk).
 #define XTY_ROWS 128
-__global__ __launch_bounds__(256) void k_xty_part(int n, int na, int nb, const double* __restrict__ X,
+__global__ __launch_bounds__(256) void k_xty_part(int n, int crows, int na, int nb, const double* __restrict__ X,
                                                   int ldx, const double* __restrict__ Y, int ldy,
                                                   double* __restrict__ part) {
   __shared__ double sX[32][64];
   __shared__ double sY[32][64];
-  const int r0 = blockIdx.x * XTY_ROWS, a0 = blockIdx.y * 64, b0 = blockIdx.z * 64;
+  const int r0 = blockIdx.x * crows, a0 = blockIdx.y * 64, b0 = blockIdx.z * 64;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nbt = min(64, nb - b0), nat = min(64, na - a0);
   const bool narrow = nb <= 16;
   double acc[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) acc[q] = 0.0;
-  const int rend = min(n, r0 + XTY_ROWS);
+  const int rend = min(n, r0 + crows);
   for (int rc = r0; rc < rend; rc += 32) {
     double rx[8], ry[8];
 #pragma unroll
@@ -1207,17 +1214,24 @@ void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, 
 }
 void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* colind, const double* val,
                  const double* X, int ldx, double* Z, int ldz) {
-  if (n > 0 && nrhs > 0)
-    k_spmm<<<nblk((int64_t)n * nrhs, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
+  if (n > 0 && nrhs > 0) {
+    const int wcol = nrhs <= 128 ? nrhs : 64;
+    const int64_t threads = (int64_t)n * wcol * ((nrhs + wcol - 1) / wcol);
+    k_spmm<<<nblk(threads, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
+  }
 }
 int xty_chunks(int n) { return (n + XTY_ROWS - 1) / XTY_ROWS; }
 void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Y, int ldy,
                 double* part, double* C) {
   if (na <= 0 || nb <= 0) return;
-  int nch = xty_chunks(n);
+  // wide products (many 64x64 output tiles) take longer row chunks: fewer
+  // partial matrices to write and add, still thousands of workgroups
+  const int tiles = ((na + 63) / 64) * ((nb + 63) / 64);
+  const int crows = tiles >= 16 ? 8 * XTY_ROWS : XTY_ROWS;
+  const int nch = (n + crows - 1) / crows;
   if (nch == 0) { launch_fill(st, C, (int64_t)na * nb, 0.0); return; }
   dim3 g(nch, (na + 63) / 64, (nb + 63) / 64);
-  k_xty_part<<<g, 256, 0, st>>>(n, na, nb, X, ldx, Y, ldy, part);
+  k_xty_part<<<g, 256, 0, st>>>(n, crows, na, nb, X, ldx, Y, ldy, part);
   k_xty_sum<<<nblk((int64_t)na * nb, 4), 256, 0, st>>>(nch, na * nb, part, C);
 }
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
