@@ -481,9 +481,10 @@ __global__ __launch_bounds__(256) void k_inv_off(const FTask* __restrict__ tasks
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
   typename MF<T>::acc acc[2][2];
   zero_acc(acc);
-  for (int kt = t.a; kt < t.b; kt++) {
+  T ra[16], rb[16];
+  // global loads of tile k (sA rows of tile i, sB the transposed column tile of X)
+  auto load = [&](int kt) {
     const int rk = 64 * kt, nk = min(64, nc - rk);
-    T ra[16], rb[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q, r = idx & 63, m = idx >> 6;
@@ -494,7 +495,10 @@ __global__ __launch_bounds__(256) void k_inv_off(const FTask* __restrict__ tasks
       const bool ok = c < nj && mm < nk && (kt > t.a || mm >= c);
       rb[q] = ok ? X[(int64_t)(rk + mm) + (int64_t)(rj + c) * nc] : (T)0;
     }
-    __syncthreads();
+  };
+  load(t.a);
+  for (int kt = t.a; kt < t.b; kt++) {
+    __syncthreads();  // the previous tile's products are done with the LDS tiles
 #pragma unroll
     for (int q = 0; q < 16; q++) {
       const int idx = tid + 256 * q;
@@ -502,6 +506,7 @@ __global__ __launch_bounds__(256) void k_inv_off(const FTask* __restrict__ tasks
       sB[idx >> 6][idx & 63] = rb[q];
     }
     __syncthreads();
+    if (kt + 1 < t.b) load(kt + 1);  // the next tile's loads are in flight during the products
     mfma_tile<T>(sA, sB, 64, acc, lane, wr, wc);
   }
   // T = acc (rows i, cols j) -> LDS as sB[c][m] = T[m][c] for the second product
