@@ -15,6 +15,16 @@ namespace fgpu {
 
 using f64x4 = __attribute__((__vector_size__(4 * sizeof(double)))) double;
 
+// Workgroups of a launch are dispatched round-robin over the eight XCDs of
+// the MI355X (workgroup b on XCD b mod 8), each with its own L2.  xcd_task
+// maps workgroup b to a task index so that every XCD takes one contiguous
+// range of the task list: neighbouring tasks, which share panel columns or
+// source rows, then share an L2.  A bijection of [0, n).
+__device__ __forceinline__ int xcd_task(int b, int n) {
+  const int q = n >> 3, r = n & 7, c = b & 7, m = b >> 3;
+  return c < r ? c * (q + 1) + m : r * (q + 1) + (c - r) * q + m;
+}
+
 // ---------------------------------------------------------------- HEX8
 // One 64-lane workgroup per element.  Lanes 0..26 evaluate the 27 Gauss
 // points (Femlib/hexa.f HEXA31 with NIP=3 and HEXA35/PRIMAS share the same
@@ -199,7 +209,7 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
 template <class T>
 __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ tasks, DevSym S,
                                                     const int* __restrict__ xrange, T* __restrict__ pool) {
-  const FTask t = tasks[blockIdx.x];
+  const FTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int sn = t.sn;
   const int ldp = S.nrow[sn];
   T* P = pool + S.foff[sn];
@@ -351,7 +361,7 @@ __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, D
                                               const T* __restrict__ ipool) {
   __shared__ T sA[64][65];
   __shared__ T sB[64][65];
-  const FTask t = tasks[blockIdx.x];
+  const FTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
   const int kb = min(64, nc - k0);
   const int R0 = k0 + kb + 64 * t.a;
@@ -403,7 +413,7 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
                                                 T* __restrict__ pool) {
   __shared__ T sI[64][65];
   __shared__ T sJ[64][65];
-  const FTask t = tasks[blockIdx.x];
+  const FTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn];
   const int kb = t.kb;
   const int t0 = k0 + kb;
@@ -565,7 +575,7 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
                                             double* __restrict__ part, int nrhs) {
   __shared__ double sA[64][65];
   __shared__ double sB[NJ][65];  // B tile transposed: sB[j][k]
-  const SolveTask t = tasks[blockIdx.x];
+  const SolveTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int sn = t.sn, ld = S.nrow[sn], nc = S.ncol[sn];
   const T* F = pool + S.foff[sn];
   const T* Xi = ipool + ioff[sn];
