@@ -259,8 +259,8 @@ template <> struct MF<float> {
   static __device__ __forceinline__ int col(int lk, int r) { return 4 * lk + r; }
 };
 
-template <class T>
-__device__ __forceinline__ void mfma_tile(const T (*sA)[65], const T (*sB)[65], int kbr,
+template <class T, int LD>
+__device__ __forceinline__ void mfma_tile(const T (*sA)[LD], const T (*sB)[LD], int kbr,
                                           typename MF<T>::acc (&acc)[2][2], int lane, int wr, int wc) {
   const int lr = lane & 15, lk = lane >> 4;
   for (int m0 = 0; m0 < kbr; m0 += 4) {
@@ -411,8 +411,8 @@ __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, D
 template <class T>
 __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks, DevSym S,
                                                 T* __restrict__ pool) {
-  __shared__ T sI[64][65];
-  __shared__ T sJ[64][65];
+  __shared__ T sI[64][33];
+  __shared__ T sJ[64][33];
   const FTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn];
   const int kb = t.kb;
@@ -424,12 +424,12 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
   typename MF<T>::acc acc[2][2];
   zero_acc(acc);
-  T ri[16], rj[16], rd[16];
-  // global loads of the 64-column chunk at c0 (zero beyond the panel)
+  T ri[8], rj[8], rd[8];
+  // global loads of the 32-column chunk at c0 (zero beyond the panel)
   auto load = [&](int c0) {
-    const int kc = min(64, kb - c0), p0 = k0 + c0;
+    const int kc = min(32, kb - c0), p0 = k0 + c0;
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < 8; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
       const bool okm = m < kc;
       rd[q] = okm ? F[(int64_t)(p0 + m) * (ld + 1)] : (T)0;
@@ -438,17 +438,17 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
     }
   };
   load(0);
-  for (int c0 = 0; c0 < kb; c0 += 64) {
-    const int kc = min(64, kb - c0);
+  for (int c0 = 0; c0 < kb; c0 += 32) {
+    const int kc = min(32, kb - c0);
     if (c0 > 0) __syncthreads();  // the previous chunk's products are done with the tiles
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < 8; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
       sI[i][m] = ri[q] * rd[q];
       sJ[i][m] = rj[q];
     }
     __syncthreads();
-    if (c0 + 64 < kb) load(c0 + 64);  // the next chunk's loads are in flight during the products
+    if (c0 + 32 < kb) load(c0 + 32);  // the next chunk's loads are in flight during the products
     mfma_tile<T>(sI, sJ, (kc + 3) & ~3, acc, lane, wr, wc);
   }
   const int lr = lane & 15, lk = lane >> 4;
