@@ -573,8 +573,9 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
                                             const T* __restrict__ ipool, double* __restrict__ W,
                                             double* __restrict__ V, double* __restrict__ X,
                                             double* __restrict__ part, int nrhs) {
-  __shared__ double sA[64][65];
-  __shared__ double sB[NJ][65];  // B tile transposed: sB[j][k]
+  constexpr int CH = 32;            // K chunk staged in LDS (small tiles: more workgroups per CU)
+  __shared__ double sA[64][CH + 1];
+  __shared__ double sB[NJ][CH + 1];  // B tile transposed: sB[j][k]
   const SolveTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int sn = t.sn, ld = S.nrow[sn], nc = S.ncol[sn];
   const T* F = pool + S.foff[sn];
@@ -587,20 +588,20 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
   const double* Bsrc = MODE == 1 ? Vf : (MODE == 2 ? Wf + (int64_t)nc * nrhs : Wf);
   constexpr bool rowmajorA = (MODE == 0 || MODE == 1);
   constexpr int NT = NJ / 16;   // 16x16 MFMA result tiles per wave: rows 16 wv .. 16 wv + 15, all NJ columns
-  constexpr int BPT = NJ / 4;   // B-tile values loaded per thread (64 x NJ tile)
+  constexpr int BPT = NJ * CH / 256;  // B-tile values loaded per thread (CH x NJ tile)
   f64x4 acc[NT];
 #pragma unroll
   for (int y = 0; y < NT; y++) acc[y] = f64x4{0.0, 0.0, 0.0, 0.0};
   const int kend = t.k0 + t.kn;
-  double ra[16], rb[BPT];
-  // global loads of the 64-wide chunk starting at kc (zero outside the task)
+  double ra[64 * CH / 256], rb[BPT];
+  // global loads of the CH-wide chunk starting at kc (zero outside the task)
   auto load = [&](int kc) {
-    const int kn = min(64, kend - kc);
+    const int kn = min(CH, kend - kc);
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < 64 * CH / 256; q++) {
       const int idx = tid + 256 * q;
-      const int i = rowmajorA ? (idx & 63) : (idx >> 6);
-      const int k = rowmajorA ? (idx >> 6) : (idx & 63);
+      const int i = rowmajorA ? (idx & 63) : (idx / CH);
+      const int k = rowmajorA ? (idx >> 6) : (idx % CH);
       const int r = r0 + i, kk = kc + k;
       double v = 0.0;
       if (i < rn && k < kn) {
@@ -618,14 +619,14 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
     }
   };
   if (t.k0 < kend) load(t.k0);
-  for (int kc = t.k0; kc < kend; kc += 64) {
-    const int kn = min(64, kend - kc);
+  for (int kc = t.k0; kc < kend; kc += CH) {
+    const int kn = min(CH, kend - kc);
     __syncthreads();  // the previous chunk's products are done with the LDS tiles
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < 64 * CH / 256; q++) {
       const int idx = tid + 256 * q;
-      const int i = rowmajorA ? (idx & 63) : (idx >> 6);
-      const int k = rowmajorA ? (idx >> 6) : (idx & 63);
+      const int i = rowmajorA ? (idx & 63) : (idx / CH);
+      const int k = rowmajorA ? (idx >> 6) : (idx % CH);
       sA[i][k] = ra[q];
     }
 #pragma unroll
@@ -635,7 +636,7 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
     }
     __syncthreads();
     // the next chunk's loads are in flight while this one is multiplied
-    if (kc + 64 < kend) load(kc + 64);
+    if (kc + CH < kend) load(kc + CH);
     // v_mfma_f64_16x16x4f64 over k in steps of 4 (sA and sB are zero beyond
     // kn and nj): the A operand takes the sB rows (columns j), the B operand
     // the sA rows (target rows), as in mfma_tile
