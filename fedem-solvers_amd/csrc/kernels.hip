@@ -294,61 +294,145 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
   return __hiloint2double(hi, lo);
 }
 
-// k_diag task: sn, k0 = first panel column.  One wave: lane i keeps row i
-// of the kb x kb (kb <= 64) diagonal block and row i of the inverse of its
-// unit lower factor in registers.  Right-looking LDL^T: at step j column j
-// of L is broadcast lane by lane to update the trailing rows, and row j of
-// X = L^{-1} (complete at step j) updates the rows below it
-// (x_i -= L_ij x_j).  L and D go back into the front, X into the diagonal
-// tile of the pivot-block inverse pool.  The arithmetic is FP64 for both
-// storage types (the FP32 factor of the mixed-precision path keeps its
-// diagonal blocks' factorisation in FP64 and rounds only the stored result).
+// k_diag task: sn, k0 = first panel column.  Blocked LDL^T of the kb x kb
+// (kb <= 64) diagonal block in LDS by four waves, in 16-column steps:
+// (1) wave 0 factors the 16 x 16 diagonal sub-block with its rows in
+// registers and readlane broadcasts (L11 unit lower, D1, X11 = L11^-1);
+// (2) the rows below form L21 = A21 X11^T D1^-1 on FP64 MFMA, one 16-row
+// tile per wave; (3) the lower trailing block takes A22 -= (L21 D1) L21^T,
+// 16 x 16 MFMA tiles shared by the waves.  The off-diagonal tiles of
+// X = L^-1 then follow by distance from the diagonal:
+// X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.  L and D go back into the front,
+// X into the diagonal tile of the pivot-block inverse pool.  The arithmetic
+// is FP64 for both storage types (the FP32 factor of the mixed-precision
+// path keeps its diagonal blocks' factorisation in FP64 and rounds only the
+// stored result).  Rows beyond kb are carried as identity rows.
 template <class T>
-__global__ __launch_bounds__(64) void k_diag(const FTask* __restrict__ tasks, DevSym S,
-                                             T* __restrict__ pool, const int64_t* __restrict__ ioff,
-                                             T* __restrict__ ipool, int* __restrict__ piv) {
+__global__ __launch_bounds__(256) void k_diag(const FTask* __restrict__ tasks, DevSym S,
+                                              T* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                              T* __restrict__ ipool, int* __restrict__ piv) {
+  __shared__ double sA[64][65], sX[64][65], sD[64], sT[4][16][17];
   const FTask t = tasks[blockIdx.x];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
   const int kb = min(64, nc - k0);
   T* F = pool + S.foff[sn];
   T* Xg = ipool + ioff[sn];
-  const int lane = threadIdx.x;
-  const bool act = lane < kb;
-  double a[64], x[64];
-  double myd = 1.0;
-#pragma unroll
-  for (int m = 0; m < 64; m++) {
-    a[m] = (act && m <= lane && m < kb) ? (double)F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] : 0.0;
-    x[m] = (m == lane) ? 1.0 : 0.0;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    const int i = idx & 63, m = idx >> 6;
+    sA[i][m] = (i < kb && m <= i) ? (double)F[(int64_t)(k0 + i) + (int64_t)(k0 + m) * ld] : (i == m ? 1.0 : 0.0);
+    sX[i][m] = (i == m) ? 1.0 : 0.0;
   }
+  __syncthreads();
+  const int nsb = (kb + 15) >> 4;
+  for (int p = 0; p < nsb; p++) {
+    const int c0 = 16 * p;
+    // (1) the 16 x 16 diagonal sub-block: lane i < 16 holds row c0 + i
+    if (wv == 0) {
+      double a[16], x[16];
 #pragma unroll
-  for (int j = 0; j < 64; j++) {
-    if (j < kb) {
-      const double dj = lane_bcast(a[j], j);
-      if (lane == j) myd = dj;
-      const double l = (lane > j) ? a[j] / dj : 0.0;
-      if (lane > j) a[j] = l;
-      const double ldj = l * dj;
-#pragma unroll
-      for (int m = j + 1; m < 64; m++) {
-        const double lm = lane_bcast(l, m);  // L[m][j]
-        if (m <= lane) a[m] -= ldj * lm;
+      for (int m = 0; m < 16; m++) {
+        a[m] = (lane < 16 && m <= lane) ? sA[c0 + lane][c0 + m] : 0.0;
+        x[m] = (m == lane) ? 1.0 : 0.0;
       }
 #pragma unroll
-      for (int m = 0; m <= j; m++) x[m] -= l * lane_bcast(x[m], j);  // X[j][m]
+      for (int j = 0; j < 16; j++) {
+        const double dj = lane_bcast(a[j], j);
+        const bool below = lane > j && lane < 16;
+        const double l = below ? a[j] / dj : 0.0;
+        if (below) a[j] = l;
+        const double ldj = l * dj;
+#pragma unroll
+        for (int m = j + 1; m < 16; m++) {
+          const double lm = lane_bcast(l, m);  // L[m][j]
+          if (m <= lane) a[m] -= ldj * lm;
+        }
+#pragma unroll
+        for (int m = 0; m <= j; m++) x[m] -= l * lane_bcast(x[m], j);  // X[j][m]
+      }
+      if (lane < 16) {
+        double dd = 0.0;
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+          if (m <= lane) {
+            sA[c0 + lane][c0 + m] = a[m];
+            sX[c0 + lane][c0 + m] = x[m];
+          }
+          dd = (m == lane) ? a[m] : dd;
+        }
+        sD[c0 + lane] = dd;
+      }
+    }
+    __syncthreads();
+    // (2) L21 = A21 X11^T D1^-1, wave w the row tile at c0 + 16 (w + 1)
+    const int rb0 = c0 + 16, nrt = nsb - p - 1;
+    if (wv < nrt) {
+      const int r0 = rb0 + 16 * wv;
+      f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int m0 = 0; m0 < 16; m0 += 4)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sX[c0 + lr][c0 + m0 + lk], sA[r0 + lr][c0 + m0 + lk], acc, 0, 0, 0);
+      // acc[r]: row r0 + lr, column c0 + lk + 4 r
+#pragma unroll
+      for (int r = 0; r < 4; r++) sA[r0 + lr][c0 + lk + 4 * r] = acc[r] / sD[c0 + lk + 4 * r];
+    }
+    __syncthreads();
+    // (3) A22 -= (L21 D1) L21^T over the lower 16 x 16 tiles (ti >= tj)
+    for (int q = wv; q < nrt * (nrt + 1) / 2; q += 4) {
+      const int ti = q == 0 ? 0 : (q < 3 ? 1 : (q < 6 ? 2 : 3));
+      const int tj = q - ti * (ti + 1) / 2;
+      const int ri = rb0 + 16 * ti, rj = rb0 + 16 * tj;
+      f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int m0 = 0; m0 < 16; m0 += 4) {
+        const int m = c0 + m0 + lk;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sA[rj + lr][m], sA[ri + lr][m] * sD[m], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = ri + lr, j = rj + lk + 4 * r;
+        if (i >= j) sA[i][j] -= acc[r];
+      }
+    }
+    __syncthreads();
+  }
+  // off-diagonal 16 x 16 tiles of X = L^-1, by distance d from the diagonal
+  for (int d = 1; d < nsb; d++) {
+    const int j = wv, i = wv + d;
+    if (i < nsb) {
+      f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+      for (int kt = j; kt < i; kt++)
+#pragma unroll
+        for (int m0 = 0; m0 < 16; m0 += 4)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sX[16 * kt + m0 + lk][16 * j + lr],
+                                                     sA[16 * i + lr][16 * kt + m0 + lk], acc, 0, 0, 0);
+      // T (rows of tile i, columns of tile j) through this wave's LDS tile
+#pragma unroll
+      for (int r = 0; r < 4; r++) sT[wv][lr][lk + 4 * r] = acc[r];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      f64x4 acc2 = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int m0 = 0; m0 < 16; m0 += 4)
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(sT[wv][m0 + lk][lr], sX[16 * i + lr][16 * i + m0 + lk], acc2, 0,
+                                                    0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; r++) sX[16 * i + lr][16 * j + lk + 4 * r] = -acc2[r];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    const int i = idx & 63, m = idx >> 6;
+    if (i < kb && m <= i) {
+      F[(int64_t)(k0 + i) + (int64_t)(k0 + m) * ld] = (T)sA[i][m];
+      Xg[(int64_t)(k0 + i) + (int64_t)(k0 + m) * nc] = (T)sX[i][m];
     }
   }
-  if (act) {
-    if (!(myd > 0.0)) {  // K_ii must be positive definite
-      atomicAdd(&piv[0], 1);
-      atomicMin(&piv[1], S.col0[sn] + k0 + lane);
-    }
-#pragma unroll
-    for (int m = 0; m < 64; m++)
-      if (m <= lane) {
-        F[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * ld] = (T)a[m];
-        Xg[(int64_t)(k0 + lane) + (int64_t)(k0 + m) * nc] = (T)x[m];
-      }
+  if (tid < kb && !(sA[tid][tid] > 0.0)) {  // K_ii must be positive definite
+    atomicAdd(&piv[0], 1);
+    atomicMin(&piv[1], S.col0[sn] + k0 + tid);
   }
 }
 
@@ -1181,7 +1265,7 @@ void launch_fill(hipStream_t st, double* p, int64_t n, double v) {
   }                                                                                                        \
   void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,   \
                    T* ipool, int* piv) {                                                                   \
-    if (n > 0) k_diag<T><<<n, 64, 0, st>>>(t, S, pool, ioff, ipool, piv);                                    \
+    if (n > 0) k_diag<T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, piv);                                   \
   }                                                                                                        \
   void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,   \
                    const T* ipool) {                                                                       \
