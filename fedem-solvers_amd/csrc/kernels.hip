@@ -972,6 +972,59 @@ __global__ __launch_bounds__(256) void k_xty_part(int n, int crows, int na, int 
   }
 }
 
+// Partial C = X^T Y for wide products (both sides >= 64 columns), on FP64
+// MFMA: the workgroup's 64 x 64 output tile accumulates over its chunk of
+// crows rows in steps of 32; X and Y rows are read along their columns
+// (coalesced) and stored transposed in LDS (sA[a][r], sB[b][r]) for
+// mfma_tile.  Same grid and partial layout as k_xty_part.
+__global__ __launch_bounds__(256) void k_xty_mfma(int n, int crows, int na, int nb, const double* __restrict__ X,
+                                                  int ldx, const double* __restrict__ Y, int ldy,
+                                                  double* __restrict__ part) {
+  __shared__ double sA[64][33];
+  __shared__ double sB[64][33];
+  const int r0 = blockIdx.x * crows, a0 = blockIdx.y * 64, b0 = blockIdx.z * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nbt = min(64, nb - b0), nat = min(64, na - a0);
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  f64x4 acc[2][2];
+  zero_acc(acc);
+  const int rend = min(n, r0 + crows);
+  double rx[8], ry[8];
+  auto load = [&](int rc) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int idx = tid + 256 * q, rr = idx >> 6, c = idx & 63, r = rc + rr;
+      const bool ok = r < rend;
+      rx[q] = (ok && c < nat) ? X[(int64_t)r * ldx + a0 + c] : 0.0;
+      ry[q] = (ok && c < nbt) ? Y[(int64_t)r * ldy + b0 + c] : 0.0;
+    }
+  };
+  if (r0 < rend) load(r0);
+  for (int rc = r0; rc < rend; rc += 32) {
+    if (rc > r0) __syncthreads();  // the previous rows' products are done with the tiles
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int idx = tid + 256 * q;
+      sA[idx & 63][idx >> 6] = rx[q];
+      sB[idx & 63][idx >> 6] = ry[q];
+    }
+    __syncthreads();
+    if (rc + 32 < rend) load(rc + 32);
+    mfma_tile<double, 33>(sA, sB, 32, acc, lane, wr, wc);
+  }
+  double* P = part + (int64_t)blockIdx.x * na * nb;
+  const int lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int a = wr + 16 * x + lr, b = wc + 16 * y + lk + 4 * r;
+        if (a < nat && b < nbt) P[(int64_t)(a0 + a) * nb + b0 + b] = acc[x][y][r];
+      }
+}
+
 // One wave per output element: lane l sums chunks l, l+64, ... then a fixed
 // butterfly combines the lanes (same order on every run).
 __global__ __launch_bounds__(256) void k_xty_sum(int nchunk, int nab, const double* __restrict__ part,
@@ -1331,7 +1384,8 @@ void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx,
   const int nch = (n + crows - 1) / crows;
   if (nch == 0) { launch_fill(st, C, (int64_t)na * nb, 0.0); return; }
   dim3 g(nch, (na + 63) / 64, (nb + 63) / 64);
-  k_xty_part<<<g, 256, 0, st>>>(n, crows, na, nb, X, ldx, Y, ldy, part);
+  if (tiles >= 16) k_xty_mfma<<<g, 256, 0, st>>>(n, crows, na, nb, X, ldx, Y, ldy, part);
+  else k_xty_part<<<g, 256, 0, st>>>(n, crows, na, nb, X, ldx, Y, ldy, part);
   k_xty_sum<<<nblk((int64_t)na * nb, 4), 256, 0, st>>>(nch, na * nb, part, C);
 }
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,

@@ -330,3 +330,18 @@ def test_autotuned_settings_match_oracle():
     assert rel(SK, o["SK"]) <= TOL and rel(SM, o["SM"]) <= TOL and rel(G, o["G"]) <= TOL
     lam, _ = R.eigen()
     assert np.max(np.abs(lam - o["lam"]) / o["lam"]) <= 1e-9
+
+
+def test_many_boundary_dofs_match_oracle():
+    """A plate with 48 edge nodes (288 boundary DOFs): the boundary-sized
+    products (mass reduction B^T M B, Schur block) take the wide MFMA path."""
+    part = parts.config2(seed=5, n=40, n_ext=48)
+    R = gpu.GpuReducer()
+    R.set_part(part)
+    z = R.analyze(ngen=0)
+    assert z["ndof2"] == 288
+    R.reduce()
+    SK, SM, G = R.reduced()
+    o = orc.reduce(part, ngen=0)
+    assert rel(SK, o["SK"]) <= TOL and rel(SM, o["SM"]) <= TOL and rel(G, o["G"]) <= TOL
+    assert rel(R.B(), o["B"]) <= TOL
