@@ -858,6 +858,7 @@ __global__ __launch_bounds__(256) void k_bw_init(const SolveTask* __restrict__ t
 
 // ---------------------------------------------------------------- products
 // Z = A X for the full symmetric CSR matrix A (perm space), X, Z row-major.
+template <bool SKIP>
 __global__ __launch_bounds__(256) void k_spmm(int n, int nrhs, const int* __restrict__ rowptr,
                                               const int* __restrict__ colind,
                                               const double* __restrict__ val,
@@ -874,17 +875,22 @@ __global__ __launch_bounds__(256) void k_spmm(int n, int nrhs, const int* __rest
   const int r = (int)(rem / wcol), j = jb * wcol + (int)(rem % wcol);
   if (r >= n || j >= nrhs) return;
   const int kb = rowptr[r], ke = rowptr[r + 1];
+  // SKIP: entries that are zero in value (the off-diagonal pattern of a
+  // lumped mass) skip their gather of X
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   int k = kb;
   for (; k + 4 <= ke; k += 4) {
     const int c0 = colind[k], c1 = colind[k + 1], c2 = colind[k + 2], c3 = colind[k + 3];
     const double v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
-    s0 += v0 * X[(int64_t)c0 * ldx + j];
-    s1 += v1 * X[(int64_t)c1 * ldx + j];
-    s2 += v2 * X[(int64_t)c2 * ldx + j];
-    s3 += v3 * X[(int64_t)c3 * ldx + j];
+    if (!SKIP || v0 != 0.0) s0 += v0 * X[(int64_t)c0 * ldx + j];
+    if (!SKIP || v1 != 0.0) s1 += v1 * X[(int64_t)c1 * ldx + j];
+    if (!SKIP || v2 != 0.0) s2 += v2 * X[(int64_t)c2 * ldx + j];
+    if (!SKIP || v3 != 0.0) s3 += v3 * X[(int64_t)c3 * ldx + j];
   }
-  for (; k < ke; k++) s0 += val[k] * X[(int64_t)colind[k] * ldx + j];
+  for (; k < ke; k++) {
+    const double v = val[k];
+    if (!SKIP || v != 0.0) s0 += v * X[(int64_t)colind[k] * ldx + j];
+  }
   Z[(int64_t)r * ldz + j] = (s0 + s1) + (s2 + s3);
 }
 
@@ -1366,11 +1372,12 @@ void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, 
   if (n > 0) k_bw_init<<<n, 256, 0, st>>>(t, S, Y, X, W, nrhs);
 }
 void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* colind, const double* val,
-                 const double* X, int ldx, double* Z, int ldz) {
+                 const double* X, int ldx, double* Z, int ldz, bool skip_zeros) {
   if (n > 0 && nrhs > 0) {
     const int wcol = nrhs <= 128 ? nrhs : 64;
     const int64_t threads = (int64_t)n * wcol * ((nrhs + wcol - 1) / wcol);
-    k_spmm<<<nblk(threads, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
+    if (skip_zeros) k_spmm<true><<<nblk(threads, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
+    else k_spmm<false><<<nblk(threads, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
   }
 }
 int xty_chunks(int n) { return (n + XTY_ROWS - 1) / XTY_ROWS; }

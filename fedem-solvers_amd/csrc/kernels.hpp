@@ -47,7 +47,8 @@ void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, 
 void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y,
                     const double* X, double* W, int nrhs);
 void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* colind, const double* val,
-                 const double* X, int ldx, double* Z, int ldz);
+                 const double* X, int ldx, double* Z, int ldz,
+                 bool skip_zeros = false);
 int xty_chunks(int n);
 void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Y, int ldy,
                 double* part, double* C);

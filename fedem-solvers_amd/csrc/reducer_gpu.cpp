@@ -158,6 +158,7 @@ struct DeviceReducer::Impl {
   // mixed precision: FP32 fronts and pivot-block inverses, equilibration
   // scale, refinement work (correction, residual, K X) and norm partials
   bool mixed = false;
+  bool m_lumped = false;  // only lumped-mass (ANDES4) elements: M is diagonal in a K-shaped pattern
   int solve_split = 256;
   DBuf<float> pool32, ipool32;
   DBuf<double> dscale, dcs, Dl, Rr, Zr, irpart;
@@ -450,6 +451,7 @@ int DeviceReducer::upload(std::string& err) {
   for (int s = 0; s < y.nsup; s++) ioff[s + 1] = ioff[s] + (int64_t)y.ncol[s] * y.ncol[s];
   HIPCHK(d.ioff.upload(ioff));
   d.mixed = opt_.mixed;
+  d.m_lumped = d.hexl.empty() && d.bml.empty();
   d.solve_split = std::max(64, (opt_.solve_split / 64) * 64);
   if (d.mixed) {
     d.ipool.release();
@@ -885,7 +887,7 @@ int DeviceReducer::reduce(std::string& err) {
     auto orth_issue = [&](int j, int par) -> int {
       HIPCHK(hipMemsetAsync(d.lzbad.p, 0, sizeof(int), st));
       // M W is formed once: the second pass's M (W R_1^-1) is (M W) R_1^-1
-      launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs);
+      launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs, d.m_lumped);
       double* Zin = d.Zl.p;
       double* Zout = d.Yl.p;
       for (int pass = 0; pass < 2; pass++) {
@@ -1058,7 +1060,7 @@ int DeviceReducer::reduce(std::string& err) {
     // without a converged test the Ritz pairs come from the full basis; the
     // M-Gram matrix of the last (reorthogonalised) block gives their residuals
     if (!conv) {
-      launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs);
+      launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs, d.m_lumped);
       launch_xty(st, neq, bs, bs, d.Wl.p, bs, d.Zl.p, bs, d.part.p, d.Cc.p);
       std::vector<double> Gl((size_t)bs * bs);
       HIPCHK(hipMemcpyAsync(Gl.data(), d.Cc.p, sizeof(double) * bs * bs, hipMemcpyDeviceToHost, st));
@@ -1111,7 +1113,7 @@ int DeviceReducer::reduce(std::string& err) {
     return 0;
   };
   if (!opt_.no_mass && n2 > 0) {
-    launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Mv.p, d.X.p, n2, d.Z.p, n2);
+    launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Mv.p, d.X.p, n2, d.Z.p, n2, d.m_lumped);
     if (xty_host(n2, n2, d.X.p, n2, d.Z.p, n2, M22)) return -3;
   }
   if (ngen > 0) {
@@ -1119,7 +1121,7 @@ int DeviceReducer::reduce(std::string& err) {
     launch_spmm(st, neq, ngen, d.rowptr.p, d.colind.p, d.Kv.p, d.Phi.p, nev, d.Yl.p, ngen);
     if (xty_host(ngen, ngen, d.Phi.p, nev, d.Yl.p, ngen, K11)) return -3;
     if (!opt_.no_mass) {
-      launch_spmm(st, neq, ngen, d.rowptr.p, d.colind.p, d.Mv.p, d.Phi.p, nev, d.Yl.p, ngen);
+      launch_spmm(st, neq, ngen, d.rowptr.p, d.colind.p, d.Mv.p, d.Phi.p, nev, d.Yl.p, ngen, d.m_lumped);
       if (xty_host(ngen, ngen, d.Phi.p, nev, d.Yl.p, ngen, M11)) return -3;
       if (n2 > 0 && xty_host(ngen, n2, d.Phi.p, nev, d.Z.p, n2, M12)) return -3;
     }
@@ -1128,7 +1130,7 @@ int DeviceReducer::reduce(std::string& err) {
   // ---- gravity: g = M a ; G_e = [B; I]^T g ; G_g = Phi^T g
   if (!opt_.no_mass) {
     HIPCHK(d.Tmp.alloc((size_t)neq * std::max(3, std::max(opt_.lanczos_block, nev))));
-    launch_spmm(st, neq, 3, d.rowptr.p, d.colind.p, d.Mv.p, d.grav.p, 3, d.Tmp.p, 3);
+    launch_spmm(st, neq, 3, d.rowptr.p, d.colind.p, d.Mv.p, d.grav.p, 3, d.Tmp.p, 3, d.m_lumped);
     if (n2 > 0 && xty_host(n2, 3, d.X.p, n2, d.Tmp.p, 3, Ge)) return -3;
     if (ngen > 0 && xty_host(ngen, 3, d.Phi.p, nev, d.Tmp.p, 3, Gg)) return -3;
     if (opt_.want_vgi && n1 > 0) {
@@ -1305,7 +1307,7 @@ int DeviceReducer::gravity_internal(std::vector<double>& gi, std::string& err) {
   if (opt_.no_mass) { err = "gravity forces need the mass matrix"; return -1; }
   DBuf<double> g;
   HIPCHK(g.alloc((size_t)neq * 3));
-  launch_spmm(st_, neq, 3, d.rowptr.p, d.colind.p, d.Mv.p, d.grav.p, 3, g.p, 3);
+  launch_spmm(st_, neq, 3, d.rowptr.p, d.colind.p, d.Mv.p, d.grav.p, 3, g.p, 3, d.m_lumped);
   std::vector<double> h((size_t)neq * 3);
   HIPCHK(hipMemcpyAsync(h.data(), g.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st_));
   HIPCHK(hipStreamSynchronize(st_));
