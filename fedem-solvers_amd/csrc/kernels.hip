@@ -629,6 +629,98 @@ __global__ __launch_bounds__(256) void k_inv_off(const FTask* __restrict__ tasks
       }
 }
 
+// Split inverse tiles.  k_inv_part task: sn, a = tile column j, b = tile
+// row i, k0 = first k tile of the chunk, kb = partial slot:
+//   part[slot] = sum_{k=k0}^{min(k0+INV_CHUNK,i)-1} L_ik X_kj   (64 x 64, row-major)
+// k_inv_fin task: sn, a = j, b = i, k0 = first slot, kb = number of slots:
+//   X_ij = - X_ii * (sum of the partials in slot order).
+template <class T>
+__global__ __launch_bounds__(256) void k_inv_part(const FTask* __restrict__ tasks, DevSym S,
+                                                  const T* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                                  const T* __restrict__ ipool, T* __restrict__ part) {
+  __shared__ T sA[64][65];
+  __shared__ T sB[64][65];
+  const FTask t = tasks[blockIdx.x];
+  const int sn = t.sn, ld = S.nrow[sn], nc = S.ncol[sn];
+  const int rj = 64 * t.a, nj = min(64, nc - rj);
+  const int ri = 64 * t.b, ni = min(64, nc - ri);
+  const T* F = pool + S.foff[sn];
+  const T* X = ipool + ioff[sn];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  const int kend = min(t.k0 + INV_CHUNK, t.b);
+  typename MF<T>::acc acc[2][2];
+  zero_acc(acc);
+  T ra[16], rb[16];
+  auto load = [&](int kt) {
+    const int rk = 64 * kt, nk = min(64, nc - rk);
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q, r = idx & 63, m = idx >> 6;
+      ra[q] = (r < ni && m < nk) ? F[(int64_t)(ri + r) + (int64_t)(rk + m) * ld] : (T)0;
+      const int c = idx >> 6, mm = idx & 63;
+      const bool ok = c < nj && mm < nk && (kt > t.a || mm >= c);
+      rb[q] = ok ? X[(int64_t)(rk + mm) + (int64_t)(rj + c) * nc] : (T)0;
+    }
+  };
+  load(t.k0);
+  for (int kt = t.k0; kt < kend; kt++) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q;
+      sA[idx & 63][idx >> 6] = ra[q];
+      sB[idx >> 6][idx & 63] = rb[q];
+    }
+    __syncthreads();
+    if (kt + 1 < kend) load(kt + 1);
+    mfma_tile<T>(sA, sB, 64, acc, lane, wr, wc);
+  }
+  T* P = part + (int64_t)t.kb * 4096;
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) P[(wr + 16 * x + lr) * 64 + wc + 16 * y + MF<T>::col(lk, r)] = acc[x][y][r];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_inv_fin(const FTask* __restrict__ tasks, DevSym S,
+                                                 const int64_t* __restrict__ ioff, T* __restrict__ ipool,
+                                                 const T* __restrict__ part) {
+  __shared__ T sA[64][65];
+  __shared__ T sB[64][65];
+  const FTask t = tasks[blockIdx.x];
+  const int sn = t.sn, nc = S.ncol[sn];
+  const int rj = 64 * t.a, nj = min(64, nc - rj);
+  const int ri = 64 * t.b, ni = min(64, nc - ri);
+  T* X = ipool + ioff[sn];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    const int m = idx >> 6, c = idx & 63;  // T[m][c] -> sB[c][m]
+    T v = (T)0;
+    for (int q = 0; q < t.kb; q++) v += part[(int64_t)(t.k0 + q) * 4096 + idx];
+    sB[c][m] = v;
+    const int r = idx & 63, mm = idx >> 6;  // X_ii lower -> sA[r][mm]
+    sA[r][mm] = (r < ni && mm < ni && r >= mm) ? X[(int64_t)(ri + r) + (int64_t)(ri + mm) * nc] : (T)0;
+  }
+  __syncthreads();
+  typename MF<T>::acc acc[2][2];
+  zero_acc(acc);
+  mfma_tile<T>(sA, sB, 64, acc, lane, wr, wc);
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = wr + 16 * x + lr, c = wc + 16 * y + MF<T>::col(lk, r);
+        if (i < ni && c < nj) X[(int64_t)(ri + i) + (int64_t)(rj + c) * nc] = -acc[x][y][r];
+      }
+}
+
 // ---------------------------------------------------------------- solves
 // Multi-vectors are row-major (rows x nrhs).  Work buffer W holds, for front
 // s, rows woff[s] .. woff[s]+nrow[s]-1; V holds the solved pivot rows before
@@ -1336,6 +1428,14 @@ void launch_fill(hipStream_t st, double* p, int64_t n, double v) {
   void launch_inv_off(hipStream_t st, const FTask* t, int n, const DevSym& S, const T* pool,               \
                       const int64_t* ioff, T* ipool) {                                                     \
     if (n > 0) k_inv_off<T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool);                                     \
+  }                                                                                                        \
+  void launch_inv_part(hipStream_t st, const FTask* t, int n, const DevSym& S, const T* pool,              \
+                       const int64_t* ioff, const T* ipool, T* ipart) {                                    \
+    if (n > 0) k_inv_part<T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, ipart);                             \
+  }                                                                                                        \
+  void launch_inv_fin(hipStream_t st, const FTask* t, int n, const DevSym& S, const int64_t* ioff,         \
+                      T* ipool, const T* ipart) {                                                          \
+    if (n > 0) k_inv_fin<T><<<n, 256, 0, st>>>(t, S, ioff, ipool, ipart);                                    \
   }                                                                                                        \
   void launch_mv(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const T* pool,                \
                  const int64_t* ioff, const T* ipool, double* W, double* V, double* X, double* part,       \

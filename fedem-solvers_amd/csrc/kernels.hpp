@@ -18,6 +18,10 @@ void launch_beam(hipStream_t st, int nbeam, const int* elist, const int* eptr, c
 void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* ccoef,
                      const double* Kp, const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool);
 void launch_fill(hipStream_t st, double* p, int64_t n, double v);
+// Pivot-block inverse tiles whose sum X_ij = -X_ii sum_k L_ik X_kj has more
+// than INV_CHUNK terms are summed in INV_CHUNK-tile chunks in parallel.
+constexpr int INV_CHUNK = 4;
+
 // Factorisation and solve launchers for both factor storage types: double
 // (FP64 path) and float (mixed precision: FP32 fronts and FP32 MFMA updates,
 // diagonal blocks factored in FP64; solve vectors stay FP64).
@@ -31,6 +35,10 @@ void launch_fill(hipStream_t st, double* p, int64_t n, double v);
   void launch_update(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool);                    \
   void launch_inv_off(hipStream_t st, const FTask* t, int n, const DevSym& S, const T* pool,              \
                       const int64_t* ioff, T* ipool);                                                     \
+  void launch_inv_part(hipStream_t st, const FTask* t, int n, const DevSym& S, const T* pool,             \
+                       const int64_t* ioff, const T* ipool, T* ipart);                                    \
+  void launch_inv_fin(hipStream_t st, const FTask* t, int n, const DevSym& S, const int64_t* ioff,        \
+                      T* ipool, const T* ipart);                                                          \
   /* solve GEMM tasks, mode 0 FW_INV, 1 FW_UPD, 2 BW_UPD, 3 BW_INV */                                     \
   void launch_mv(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const T* pool,               \
                  const int64_t* ioff, const T* ipool, double* W, double* V, double* X, double* part,      \

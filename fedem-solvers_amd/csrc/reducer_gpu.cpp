@@ -81,7 +81,7 @@ struct HBuf {
 };
 
 enum OpKind {
-  EXTADD, DIAG, TRSM, UPDATE, INV_OFF,
+  EXTADD, DIAG, TRSM, UPDATE, INV_OFF, INV_PART, INV_FIN,
   FW_INIT, FW_INV, FW_UPD, FW_OUT, BW_INIT, BW_UPD, BW_INV,
   FW_INV_R, FW_UPD_R, BW_UPD_R, BW_INV_R  // ordered reductions of split-K partials
 };
@@ -148,6 +148,7 @@ struct DeviceReducer::Impl {
   DevSym S{};
   DBuf<FTask> ftasks;
   DBuf<int> xrange;  // extend-add tasks: (child, first, end) column ranges
+  DBuf<double> ipart;  // partial sums of split pivot-block inverse tiles (64 x 64 each; FP32 ones use half)
   std::vector<Op> fops;
   std::map<int, std::unique_ptr<SolvePlan>> plans;
   // values
@@ -190,8 +191,9 @@ DeviceReducer::~DeviceReducer() {
 }
 
 static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, std::vector<Op>& ops,
-                                  std::vector<int>& xrange, double& flops, int panel) {
+                                  std::vector<int>& xrange, double& flops, int panel, int& inv_slots) {
   xrange.clear();
+  inv_slots = 0;
   flops = 0;
   for (int l = 0; l < y.nlevels; l++) {
     const int* sups = &y.lvsup[y.lvptr[l]];
@@ -280,13 +282,35 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
     // k_diag, the tiles at growing distance from the diagonal follow
     int maxT = 0;
     for (int q = 0; q < ns; q++) maxT = std::max(maxT, (y.ncol[sups[q]] + 63) / 64);
+    // A tile whose sum has more than INV_CHUNK terms is split: partial sums
+    // over INV_CHUNK-tile chunks in parallel (k_inv_part), then their sum in
+    // chunk order and the product with X_ii (k_inv_fin).
     for (int dist = 1; dist < maxT; dist++) {
       off = (int)tasks.size();
+      std::vector<FTask> part, fin;
+      int slot = 0;
       for (int q = 0; q < ns; q++) {
         int s = sups[q], T = (y.ncol[s] + 63) / 64;
-        for (int j = 0; j + dist < T; j++) tasks.push_back({s, j, j + dist, 0, 0});
+        for (int j = 0; j + dist < T; j++) {
+          if (dist <= INV_CHUNK) {
+            tasks.push_back({s, j, j + dist, 0, 0});
+            continue;
+          }
+          const int first = slot;
+          for (int k = j; k < j + dist; k += INV_CHUNK) part.push_back({s, j, j + dist, k, slot++});
+          fin.push_back({s, j, j + dist, first, slot - first});
+        }
       }
       if ((int)tasks.size() > off) ops.push_back({INV_OFF, off, (int)tasks.size() - off});
+      if (!part.empty()) {
+        off = (int)tasks.size();
+        tasks.insert(tasks.end(), part.begin(), part.end());
+        ops.push_back({INV_PART, off, (int)part.size()});
+        off = (int)tasks.size();
+        tasks.insert(tasks.end(), fin.begin(), fin.end());
+        ops.push_back({INV_FIN, off, (int)fin.size()});
+      }
+      inv_slots = std::max(inv_slots, slot);
     }
   }
 }
@@ -471,9 +495,12 @@ int DeviceReducer::upload(std::string& err) {
   std::vector<FTask> ft;
   d.fops.clear();
   std::vector<int> xr;
-  build_factor_schedule(y, ft, d.fops, xr, upd_flops_, std::min(256, std::max(64, (opt_.factor_panel / 64) * 64)));
+  int inv_slots = 0;
+  build_factor_schedule(y, ft, d.fops, xr, upd_flops_, std::min(256, std::max(64, (opt_.factor_panel / 64) * 64)),
+                        inv_slots);
   HIPCHK(d.ftasks.upload(ft));
   HIPCHK(d.xrange.upload(xr));
+  HIPCHK(d.ipart.alloc((size_t)std::max(1, inv_slots) * 4096));
   upd_launches_ = 0;
   for (auto& op : d.fops)
     if (op.kind == UPDATE) upd_launches_++;
@@ -746,6 +773,14 @@ int DeviceReducer::reduce(std::string& err) {
           ui++;
           break;
         case INV_OFF: launch_inv_off(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool); break;
+        case INV_PART:
+          launch_inv_part(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool,
+                          reinterpret_cast<decltype(pool)>(d.ipart.p));
+          break;
+        case INV_FIN:
+          launch_inv_fin(st, d.ftasks.p + op.off, op.n, d.S, d.ioff.p, ipool,
+                         reinterpret_cast<decltype(pool)>(d.ipart.p));
+          break;
       }
     }
   };
@@ -758,8 +793,8 @@ int DeviceReducer::reduce(std::string& err) {
   } else {
     // the factor schedule is fixed per symbolic structure: one graph
     HIPCHK(run_graph(st, d.g_factor,
-                     {d.ftasks.p, d.xrange.p, d.pool.p, d.ipool.p, d.pool32.p, d.ipool32.p, d.ioff.p, d.piv.p,
-                      d.S.foff, d.S.rows},
+                     {d.ftasks.p, d.xrange.p, d.ipart.p, d.pool.p, d.ipool.p, d.pool32.p, d.ipool32.p, d.ioff.p,
+                      d.piv.p, d.S.foff, d.S.rows},
                      factor_ops));
   }
   HIPCHK(hipGetLastError());
@@ -1400,7 +1435,9 @@ int DeviceReducer::autotune(int& panel, int& split, std::string& err) {
     std::vector<FTask> ft;
     std::vector<int> xr;
     d.fops.clear();
-    build_factor_schedule(sym_, ft, d.fops, xr, upd_flops_, P);
+    int inv_slots = 0;
+    build_factor_schedule(sym_, ft, d.fops, xr, upd_flops_, P, inv_slots);
+    HIPCHK(d.ipart.alloc((size_t)std::max(1, inv_slots) * 4096));
     HIPCHK(d.ftasks.upload(ft));
     HIPCHK(d.xrange.upload(xr));
     upd_launches_ = 0;
