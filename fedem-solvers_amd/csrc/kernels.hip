@@ -728,7 +728,7 @@ __global__ __launch_bounds__(256) void k_inv_fin(const FTask* __restrict__ tasks
 // right-hand sides j0..j0+63 of front sn over the source slice k0..k0+kn-1:
 //   mode 0 FW_INV : V[r]      =  sum_{k<=r}   Linv[r][k]   W[k]
 //   mode 1 FW_UPD : W[nc+r]  -=  sum_{k<nc}   L[nc+r][k]   V[k]
-//   mode 2 BW_UPD : W[r]     -=  sum_{k<nu}   L[nc+k][r]   W[nc+k]
+//   mode 2 BW_UPD : W[r]     -=  sum_{k<nu}   L[nc+k][r]   X[rows[nc+k]]
 //   mode 3 BW_INV : X[c0+r]   =  sum_{k>=r}   Linv[k][r]   W[k]
 // Long K ranges are split over several tasks (slot >= 0) whose 64x64
 // partials k_mv_red then sums in slot order, so the result does not depend
@@ -761,7 +761,10 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int nj = min(NJ, nrhs - t.j0);
   const int r0 = t.r0, rn = t.rn;
-  const double* Bsrc = MODE == 1 ? Vf : (MODE == 2 ? Wf + (int64_t)nc * nrhs : Wf);
+  // B rows: mode 1 the solved pivot rows V, mode 2 the final solution rows
+  // of the front's update rows (X[rows[nc + k]]), modes 0 and 3 W
+  const double* Bsrc = MODE == 1 ? Vf : Wf;
+  const int* urows = S.rows + S.rptr[sn] + nc;
   constexpr bool rowmajorA = (MODE == 0 || MODE == 1);
   constexpr int NT = NJ / 16;   // 16x16 MFMA result tiles per wave: rows 16 wv .. 16 wv + 15, all NJ columns
   constexpr int BPT = NJ * CH / 256;  // B-tile values loaded per thread (CH x NJ tile)
@@ -791,7 +794,8 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
 #pragma unroll
     for (int q = 0; q < BPT; q++) {
       const int idx = tid + 256 * q, kb = idx / NJ, jb = idx % NJ;
-      rb[q] = (kb < kn && jb < nj) ? Bsrc[(int64_t)(kc + kb) * nrhs + t.j0 + jb] : 0.0;
+      if (MODE == 2) rb[q] = (kb < kn && jb < nj) ? X[(int64_t)urows[kc + kb] * nrhs + t.j0 + jb] : 0.0;
+      else rb[q] = (kb < kn && jb < nj) ? Bsrc[(int64_t)(kc + kb) * nrhs + t.j0 + jb] : 0.0;
     }
   };
   if (t.k0 < kend) load(t.k0);
@@ -876,34 +880,25 @@ __global__ __launch_bounds__(256) void k_mv_red(const SolveTask* __restrict__ ta
 
 // Forward-solve front initialisation, task = (front, 64-row chunk r0): pivot
 // rows from the global right-hand side R (row-major, perm rows; the Schur
-// front takes the boundary rows), update rows zero, then the children's
-// update rows falling in the chunk added child after child.  The task's
-// k0/kn index its (child, first, end) triples in crange.
+// front takes the boundary rows), update rows zero, plus the children's
+// update rows that land in each row.  Work row w adds the rows
+// gsrc[gptr[w]] .. gsrc[gptr[w+1]-1] of W in child order.
 __global__ __launch_bounds__(256) void k_fw_init(const SolveTask* __restrict__ tasks, DevSym S,
-                                                 const int* __restrict__ crange, const double* __restrict__ R,
-                                                 double* __restrict__ W, int nrhs, int ndof1) {
+                                                 const int* __restrict__ gptr, const int* __restrict__ gsrc,
+                                                 const double* __restrict__ R, double* __restrict__ W, int nrhs,
+                                                 int ndof1) {
   const SolveTask t = tasks[blockIdx.x];
   const int sn = t.sn, nc = S.ncol[sn], c0 = S.col0[sn];
-  const int r0 = t.r0;
-  double* Wf = W + S.woff[sn] * (int64_t)nrhs;
-  const int tid = threadIdx.x;
+  const int64_t wo = S.woff[sn];
+  double* Wf = W + wo * nrhs;
   const int base = nc > 0 ? c0 : ndof1;
-  for (int idx = tid; idx < t.rn * nrhs; idx += 256) {
-    const int i = r0 + idx / nrhs, j = idx % nrhs;
+  for (int idx = threadIdx.x; idx < t.rn * nrhs; idx += 256) {
+    const int i = t.r0 + idx / nrhs, j = idx % nrhs;
     double v = 0.0;
     if (i < nc || nc == 0) v = R[(int64_t)(base + i) * nrhs + j];
+    const int e1 = gptr[wo + i + 1];
+    for (int e = gptr[wo + i]; e < e1; e++) v += W[(int64_t)gsrc[e] * nrhs + j];
     Wf[(int64_t)i * nrhs + j] = v;
-  }
-  for (int q = 0; q < t.kn; q++) {
-    const int* cr = crange + 3 * (t.k0 + q);
-    const int c = cr[0], ka = cr[1], kb = cr[2];
-    const int* rel = S.rel + S.relptr[c];
-    const double* Wc = W + (S.woff[c] + S.ncol[c]) * (int64_t)nrhs;
-    __syncthreads();
-    for (int idx = tid; idx < (kb - ka) * nrhs; idx += 256) {
-      const int k = ka + idx / nrhs, j = idx % nrhs;
-      Wf[(int64_t)rel[k] * nrhs + j] += Wc[(int64_t)k * nrhs + j];
-    }
   }
 }
 
@@ -928,23 +923,16 @@ __global__ __launch_bounds__(256) void k_fw_out(const SolveTask* __restrict__ ta
   }
 }
 
-// Backward-solve front initialisation, task = (front, 64-row chunk): pivot
-// rows from Y (zero when Y is null), update rows gathered from the already
-// final rows of X (ancestors and boundary).
+// Backward-solve front initialisation, task = (front, 64-row chunk of the
+// pivot rows): pivot rows from Y (zero when Y is null).
 __global__ __launch_bounds__(256) void k_bw_init(const SolveTask* __restrict__ tasks, DevSym S,
-                                                 const double* __restrict__ Y,
-                                                 const double* __restrict__ X,
-                                                 double* __restrict__ W, int nrhs) {
+                                                 const double* __restrict__ Y, double* __restrict__ W, int nrhs) {
   const SolveTask t = tasks[blockIdx.x];
-  const int sn = t.sn, nc = S.ncol[sn], c0 = S.col0[sn];
-  const int* rows = S.rows + S.rptr[sn];
+  const int sn = t.sn, c0 = S.col0[sn];
   double* Wf = W + S.woff[sn] * (int64_t)nrhs;
   for (int idx = threadIdx.x; idx < t.rn * nrhs; idx += 256) {
     const int i = t.r0 + idx / nrhs, j = idx % nrhs;
-    double v;
-    if (i < nc) v = Y ? Y[(int64_t)(c0 + i) * nrhs + j] : 0.0;
-    else v = X[(int64_t)rows[i] * nrhs + j];
-    Wf[(int64_t)i * nrhs + j] = v;
+    Wf[(int64_t)i * nrhs + j] = Y ? Y[(int64_t)(c0 + i) * nrhs + j] : 0.0;
   }
 }
 
@@ -1463,13 +1451,13 @@ void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, d
                    const double* part, int nrhs, int mode) {
   if (n > 0) k_mv_red<<<n, 256, 0, st>>>(t, S, W, V, X, part, nrhs, mode);
 }
-void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const int* crange, const double* R,
-                    double* W, int nrhs, int ndof1) {
-  if (n > 0) k_fw_init<<<n, 256, 0, st>>>(t, S, crange, R, W, nrhs, ndof1);
+void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const int* gptr, const int* gsrc,
+                    const double* R, double* W, int nrhs, int ndof1) {
+  if (n > 0) k_fw_init<<<n, 256, 0, st>>>(t, S, gptr, gsrc, R, W, nrhs, ndof1);
 }
-void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y,
-                    const double* X, double* W, int nrhs) {
-  if (n > 0) k_bw_init<<<n, 256, 0, st>>>(t, S, Y, X, W, nrhs);
+void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y, double* W,
+                    int nrhs) {
+  if (n > 0) k_bw_init<<<n, 256, 0, st>>>(t, S, Y, W, nrhs);
 }
 void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* colind, const double* val,
                  const double* X, int ldx, double* Z, int ldz, bool skip_zeros) {

@@ -94,7 +94,7 @@ struct SolvePlan {
   int nslots = 0;  // 64x64 partial buffers of the split-K solve tasks
   std::vector<Op> fw_int, fw_all, bw;
   DBuf<SolveTask> tasks;
-  DBuf<int> crange;  // forward-init tasks: (child, first, end) of the child update rows each adds
+  DBuf<int> gptr, gsrc;  // forward init: the child update rows added to each work row, in child order
   DBuf<double> part;
 };
 
@@ -316,35 +316,40 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
 }
 
 static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<SolveTask>& tasks,
-                             std::vector<int>& crange, int KS) {
+                             std::vector<int>& gptr, std::vector<int>& gsrc, int KS) {
   const int nrhs = sp.nrhs, schur = y.nsup - 1;
-  // (front, 64-row chunk) tasks over all rows (init) or the solved rows (out)
-  // a forward-init task (front s, rows r0..r0+rn-1) gets the update rows of
-  // each child that land in its rows as (child, first, end) triples, found
-  // here once instead of by a search in every solve
+  // gather lists of the forward init: work row woff[s] + i of front s adds
+  // the update rows of s's children that land in its row i, child after
+  // child (a child adds at most one row to a parent row)
+  std::vector<int64_t> woff(y.nsup + 1, 0);
+  for (int s = 0; s < y.nsup; s++) woff[s + 1] = woff[s] + y.nrow[s];
+  gptr.assign(woff[y.nsup] + 1, 0);
+  for (int s = 0; s < y.nsup; s++)
+    for (int q = y.chptr[s]; q < y.chptr[s + 1]; q++) {
+      const int c = y.chlist[q], nu = y.nrow[c] - y.ncol[c];
+      const int* rel = y.rel.data() + y.relptr[c];
+      for (int k = 0; k < nu; k++) gptr[woff[s] + rel[k] + 1]++;
+    }
+  for (size_t i = 1; i < gptr.size(); i++) gptr[i] += gptr[i - 1];
+  gsrc.assign(gptr.back(), 0);
+  {
+    std::vector<int> fill(gptr.begin(), gptr.end() - 1);
+    for (int s = 0; s < y.nsup; s++)
+      for (int q = y.chptr[s]; q < y.chptr[s + 1]; q++) {
+        const int c = y.chlist[q], nu = y.nrow[c] - y.ncol[c];
+        const int* rel = y.rel.data() + y.relptr[c];
+        for (int k = 0; k < nu; k++) gsrc[fill[woff[s] + rel[k]]++] = (int)(woff[c] + y.ncol[c] + k);
+      }
+  }
+  // (front, 64-row chunk) tasks over all rows (forward init), the solved
+  // rows (forward output, the Schur front: all its rows) or the pivot rows
+  // (backward init)
   auto add_sns = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list) {
     int off = (int)tasks.size();
     for (int s : list) {
-      int nr = (kind == FW_OUT && s != schur) ? y.ncol[s] : y.nrow[s];
-      for (int r0 = 0; r0 < nr; r0 += 64) {
-        SolveTask t{s, r0, std::min(64, nr - r0), 0, 0, 0, -1, 0};
-        if (kind == FW_INIT) {
-          t.k0 = (int)crange.size() / 3;
-          for (int q = y.chptr[s]; q < y.chptr[s + 1]; q++) {
-            const int c = y.chlist[q], nu = y.nrow[c] - y.ncol[c];
-            const int* rel = y.rel.data() + y.relptr[c];
-            const int ka = (int)(std::lower_bound(rel, rel + nu, r0) - rel);
-            const int kb = (int)(std::lower_bound(rel, rel + nu, r0 + t.rn) - rel);
-            if (kb > ka) {
-              crange.push_back(c);
-              crange.push_back(ka);
-              crange.push_back(kb);
-              t.kn++;
-            }
-          }
-        }
-        tasks.push_back(t);
-      }
+      const bool piv = (kind == FW_OUT && s != schur) || kind == BW_INIT;
+      const int nr = piv ? y.ncol[s] : y.nrow[s];
+      for (int r0 = 0; r0 < nr; r0 += 64) tasks.push_back({s, r0, std::min(64, nr - r0), 0, 0, 0, -1, 0});
     }
     if ((int)tasks.size() > off) ops.push_back({kind, off, (int)tasks.size() - off});
   };
@@ -386,9 +391,12 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
       ops.push_back({rkind, off, (int)red.size()});
     }
   };
-  // forward: levels bottom-up; fw_all also carries the Schur front
+  // forward: levels bottom-up; fw_all also carries the Schur front.  The
+  // solved pivot rows of a front stay in V, so one output launch after the
+  // last level copies them all.
   for (int pass = 0; pass < 2; pass++) {
     std::vector<Op>& ops = pass == 0 ? sp.fw_int : sp.fw_all;
+    std::vector<int> all;
     for (int l = 0; l < y.nlevels; l++) {
       std::vector<int> list, ilist;
       for (int q = y.lvptr[l]; q < y.lvptr[l + 1]; q++) {
@@ -400,16 +408,24 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
       add_sns(ops, FW_INIT, list);
       add_tiles(ops, FW_INV, ilist, false);
       add_tiles(ops, FW_UPD, ilist, true);
-      add_sns(ops, FW_OUT, list);
+      all.insert(all.end(), list.begin(), list.end());
     }
+    add_sns(ops, FW_OUT, all);
   }
-  // backward: levels top-down, internal fronts only
+  // backward: the pivot rows of every internal front from Y in one launch,
+  // then levels top-down (the update-row products read the final rows of X
+  // directly)
+  {
+    std::vector<int> all;
+    for (int s = 0; s < y.nsup; s++)
+      if (s != schur) all.push_back(s);
+    add_sns(sp.bw, BW_INIT, all);
+  }
   for (int l = y.nlevels - 1; l >= 0; l--) {
     std::vector<int> list;
     for (int q = y.lvptr[l]; q < y.lvptr[l + 1]; q++)
       if (y.lvsup[q] != schur) list.push_back(y.lvsup[q]);
     if (list.empty()) continue;
-    add_sns(sp.bw, BW_INIT, list);
     add_tiles(sp.bw, BW_UPD, list, false);
     add_tiles(sp.bw, BW_INV, list, false);
   }
@@ -545,10 +561,11 @@ static int get_plan(const Symbolic& y, DeviceReducer::Impl& d, int nrhs, SolvePl
   auto sp = std::make_unique<SolvePlan>();
   sp->nrhs = nrhs;
   std::vector<SolveTask> tasks;
-  std::vector<int> crange;
-  build_solve_plan(y, *sp, tasks, crange, d.solve_split);
+  std::vector<int> gptr, gsrc;
+  build_solve_plan(y, *sp, tasks, gptr, gsrc, d.solve_split);
   HIPCHK(sp->tasks.upload(tasks));
-  HIPCHK(sp->crange.upload(crange));
+  HIPCHK(sp->gptr.upload(gptr));
+  HIPCHK(sp->gsrc.upload(gsrc.empty() ? std::vector<int>(1, 0) : gsrc));
   HIPCHK(sp->part.alloc((size_t)std::max(1, sp->nslots) * 4096));
   out = sp.get();
   d.plans[nrhs] = std::move(sp);
@@ -589,7 +606,7 @@ static void run_forward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan&
   const std::vector<Op>& ops = all ? sp.fw_all : sp.fw_int;
   for (const Op& op : ops) {
     switch (op.kind) {
-      case FW_INIT: launch_fw_init(st, sp.tasks.p + op.off, op.n, d.S, sp.crange.p, R, d.W.p, sp.nrhs, ndof1); break;
+      case FW_INIT: launch_fw_init(st, sp.tasks.p + op.off, op.n, d.S, sp.gptr.p, sp.gsrc.p, R, d.W.p, sp.nrhs, ndof1); break;
       case FW_OUT:
         if (d.mixed)
           launch_fw_out(st, sp.tasks.p + op.off, op.n, d.S, d.pool32.p, d.W.p, d.V.p, Yout, sp.nrhs, scale, ndof1);
@@ -604,7 +621,7 @@ static void run_forward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan&
 // X = L^{-T} Y on the internal rows; X's boundary rows must be set by the caller
 static void run_backward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan& sp, const double* Yin, double* X) {
   for (const Op& op : sp.bw) {
-    if (op.kind == BW_INIT) launch_bw_init(st, sp.tasks.p + op.off, op.n, d.S, Yin, X, d.W.p, sp.nrhs);
+    if (op.kind == BW_INIT) launch_bw_init(st, sp.tasks.p + op.off, op.n, d.S, Yin, d.W.p, sp.nrhs);
     else run_mv_op(st, d, sp, op, X);
   }
 }
