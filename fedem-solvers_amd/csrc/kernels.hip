@@ -850,7 +850,7 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
 
 // Reduce task: slots k0 .. k0+kn-1 hold the partials of rows r0.. (rn) and
 // right-hand sides j0.. of front sn; sum them in slot order and apply.
-__global__ __launch_bounds__(256) void k_mv_red(const SolveTask* __restrict__ tasks, DevSym S,
+__global__ __launch_bounds__(1024) void k_mv_red(const SolveTask* __restrict__ tasks, DevSym S,
                                                 double* __restrict__ W, double* __restrict__ V,
                                                 double* __restrict__ X, const double* __restrict__ part,
                                                 int nrhs, int mode) {
@@ -859,7 +859,7 @@ __global__ __launch_bounds__(256) void k_mv_red(const SolveTask* __restrict__ ta
   double* Wf = W + S.woff[sn] * (int64_t)nrhs;
   double* Vf = V + S.woff[sn] * (int64_t)nrhs;
   const int nj = min(64, nrhs - t.j0);
-  for (int idx = threadIdx.x; idx < t.rn * nj; idx += 256) {
+  for (int idx = threadIdx.x; idx < t.rn * nj; idx += blockDim.x) {
     const int i = idx / nj, jj = idx % nj;
     const double* p = part + (int64_t)t.k0 * 4096 + i * 64 + jj;
     // slots summed in order; the loads of four slots are issued together
@@ -883,7 +883,7 @@ __global__ __launch_bounds__(256) void k_mv_red(const SolveTask* __restrict__ ta
 // front takes the boundary rows), update rows zero, plus the children's
 // update rows that land in each row.  Work row w adds the rows
 // gsrc[gptr[w]] .. gsrc[gptr[w+1]-1] of W in child order.
-__global__ __launch_bounds__(256) void k_fw_init(const SolveTask* __restrict__ tasks, DevSym S,
+__global__ __launch_bounds__(1024) void k_fw_init(const SolveTask* __restrict__ tasks, DevSym S,
                                                  const int* __restrict__ gptr, const int* __restrict__ gsrc,
                                                  const double* __restrict__ R, double* __restrict__ W, int nrhs,
                                                  int ndof1) {
@@ -892,7 +892,7 @@ __global__ __launch_bounds__(256) void k_fw_init(const SolveTask* __restrict__ t
   const int64_t wo = S.woff[sn];
   double* Wf = W + wo * nrhs;
   const int base = nc > 0 ? c0 : ndof1;
-  for (int idx = threadIdx.x; idx < t.rn * nrhs; idx += 256) {
+  for (int idx = threadIdx.x; idx < t.rn * nrhs; idx += blockDim.x) {
     const int i = t.r0 + idx / nrhs, j = idx % nrhs;
     double v = 0.0;
     if (i < nc || nc == 0) v = R[(int64_t)(base + i) * nrhs + j];
@@ -1449,11 +1449,11 @@ FGPU_FACTOR_LAUNCHERS(float)
 #undef FGPU_FACTOR_LAUNCHERS
 void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, double* W, double* V, double* X,
                    const double* part, int nrhs, int mode) {
-  if (n > 0) k_mv_red<<<n, 256, 0, st>>>(t, S, W, V, X, part, nrhs, mode);
+  if (n > 0) k_mv_red<<<n, nrhs <= 16 ? 64 * nrhs : 1024, 0, st>>>(t, S, W, V, X, part, nrhs, mode);
 }
 void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const int* gptr, const int* gsrc,
                     const double* R, double* W, int nrhs, int ndof1) {
-  if (n > 0) k_fw_init<<<n, 256, 0, st>>>(t, S, gptr, gsrc, R, W, nrhs, ndof1);
+  if (n > 0) k_fw_init<<<n, nrhs <= 16 ? 64 * nrhs : 1024, 0, st>>>(t, S, gptr, gsrc, R, W, nrhs, ndof1);
 }
 void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y, double* W,
                     int nrhs) {
