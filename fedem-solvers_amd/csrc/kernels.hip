@@ -773,6 +773,17 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
   for (int y = 0; y < NT; y++) acc[y] = f64x4{0.0, 0.0, 0.0, 0.0};
   const int kend = t.k0 + t.kn;
   double ra[64 * CH / 256], rb[BPT];
+  // mode 2: the X row of each B value this thread loads, fetched one chunk
+  // ahead of the B loads that use them
+  int ur[BPT];
+  auto load_rows = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+      const int kb = (tid + 256 * q) / NJ;
+      ur[q] = kc + kb < kend ? urows[kc + kb] : 0;
+    }
+  };
+  if (MODE == 2) load_rows(t.k0);
   // global loads of the CH-wide chunk starting at kc (zero outside the task)
   auto load = [&](int kc) {
     const int kn = min(CH, kend - kc);
@@ -794,9 +805,10 @@ __global__ __launch_bounds__(256) void k_mv(const SolveTask* __restrict__ tasks,
 #pragma unroll
     for (int q = 0; q < BPT; q++) {
       const int idx = tid + 256 * q, kb = idx / NJ, jb = idx % NJ;
-      if (MODE == 2) rb[q] = (kb < kn && jb < nj) ? X[(int64_t)urows[kc + kb] * nrhs + t.j0 + jb] : 0.0;
+      if (MODE == 2) rb[q] = (kb < kn && jb < nj) ? X[(int64_t)ur[q] * nrhs + t.j0 + jb] : 0.0;
       else rb[q] = (kb < kn && jb < nj) ? Bsrc[(int64_t)(kc + kb) * nrhs + t.j0 + jb] : 0.0;
     }
+    if (MODE == 2) load_rows(kc + CH);
   };
   if (t.k0 < kend) load(t.k0);
   for (int kc = t.k0; kc < kend; kc += CH) {

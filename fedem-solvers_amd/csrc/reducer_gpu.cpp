@@ -278,40 +278,42 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
         if ((int)tasks.size() > off) ops.push_back({UPDATE, off, (int)tasks.size() - off});
       }
     }
-    // explicit inverse of every pivot block: the diagonal tiles come from
-    // k_diag, the tiles at growing distance from the diagonal follow
-    int maxT = 0;
-    for (int q = 0; q < ns; q++) maxT = std::max(maxT, (y.ncol[sups[q]] + 63) / 64);
-    // A tile whose sum has more than INV_CHUNK terms is split: partial sums
-    // over INV_CHUNK-tile chunks in parallel (k_inv_part), then their sum in
-    // chunk order and the product with X_ii (k_inv_fin).
-    for (int dist = 1; dist < maxT; dist++) {
-      off = (int)tasks.size();
-      std::vector<FTask> part, fin;
-      int slot = 0;
-      for (int q = 0; q < ns; q++) {
-        int s = sups[q], T = (y.ncol[s] + 63) / 64;
-        for (int j = 0; j + dist < T; j++) {
-          if (dist <= INV_CHUNK) {
-            tasks.push_back({s, j, j + dist, 0, 0});
-            continue;
-          }
-          const int first = slot;
-          for (int k = j; k < j + dist; k += INV_CHUNK) part.push_back({s, j, j + dist, k, slot++});
-          fin.push_back({s, j, j + dist, first, slot - first});
+  }
+  // Explicit inverse of every pivot block: the diagonal tiles come from
+  // k_diag, the tiles at growing distance from the diagonal follow.  Only the
+  // solves read the inverses, so they are formed after the whole
+  // factorisation, distance by distance over all fronts at once.
+  int maxT = 0;
+  for (int s = 0; s < y.nsup; s++) maxT = std::max(maxT, (y.ncol[s] + 63) / 64);
+  // A tile whose sum has more than INV_CHUNK terms is split: partial sums
+  // over INV_CHUNK-tile chunks in parallel (k_inv_part), then their sum in
+  // chunk order and the product with X_ii (k_inv_fin).
+  for (int dist = 1; dist < maxT; dist++) {
+    int off = (int)tasks.size();
+    std::vector<FTask> part, fin;
+    int slot = 0;
+    for (int s = 0; s < y.nsup; s++) {
+      const int T = (y.ncol[s] + 63) / 64;
+      for (int j = 0; j + dist < T; j++) {
+        if (dist <= INV_CHUNK) {
+          tasks.push_back({s, j, j + dist, 0, 0});
+          continue;
         }
+        const int first = slot;
+        for (int k = j; k < j + dist; k += INV_CHUNK) part.push_back({s, j, j + dist, k, slot++});
+        fin.push_back({s, j, j + dist, first, slot - first});
       }
-      if ((int)tasks.size() > off) ops.push_back({INV_OFF, off, (int)tasks.size() - off});
-      if (!part.empty()) {
-        off = (int)tasks.size();
-        tasks.insert(tasks.end(), part.begin(), part.end());
-        ops.push_back({INV_PART, off, (int)part.size()});
-        off = (int)tasks.size();
-        tasks.insert(tasks.end(), fin.begin(), fin.end());
-        ops.push_back({INV_FIN, off, (int)fin.size()});
-      }
-      inv_slots = std::max(inv_slots, slot);
     }
+    if ((int)tasks.size() > off) ops.push_back({INV_OFF, off, (int)tasks.size() - off});
+    if (!part.empty()) {
+      off = (int)tasks.size();
+      tasks.insert(tasks.end(), part.begin(), part.end());
+      ops.push_back({INV_PART, off, (int)part.size()});
+      off = (int)tasks.size();
+      tasks.insert(tasks.end(), fin.begin(), fin.end());
+      ops.push_back({INV_FIN, off, (int)fin.size()});
+    }
+    inv_slots = std::max(inv_slots, slot);
   }
 }
 
