@@ -686,9 +686,9 @@ __global__ __launch_bounds__(256) void k_inv_part(const FTask* __restrict__ task
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_inv_fin(const FTask* __restrict__ tasks, DevSym S,
-                                                 const int64_t* __restrict__ ioff, T* __restrict__ ipool,
-                                                 const T* __restrict__ part) {
+__global__ __launch_bounds__(1024) void k_inv_fin(const FTask* __restrict__ tasks, DevSym S,
+                                                  const int64_t* __restrict__ ioff, T* __restrict__ ipool,
+                                                  const T* __restrict__ part) {
   __shared__ T sA[64][65];
   __shared__ T sB[64][65];
   const FTask t = tasks[blockIdx.x];
@@ -697,16 +697,42 @@ __global__ __launch_bounds__(256) void k_inv_fin(const FTask* __restrict__ tasks
   const int ri = 64 * t.b, ni = min(64, nc - ri);
   T* X = ipool + ioff[sn];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
-  for (int idx = tid; idx < 64 * 64; idx += 256) {
-    const int m = idx >> 6, c = idx & 63;  // T[m][c] -> sB[c][m]
-    T v = (T)0;
-    for (int q = 0; q < t.kb; q++) v += part[(int64_t)(t.k0 + q) * 4096 + idx];
-    sB[c][m] = v;
-    const int r = idx & 63, mm = idx >> 6;  // X_ii lower -> sA[r][mm]
-    sA[r][mm] = (r < ni && mm < ni && r >= mm) ? X[(int64_t)(ri + r) + (int64_t)(ri + mm) * nc] : (T)0;
+  // sum of the partials (slot order) for four entries per thread, every
+  // slot's loads issued before the additions; X_ii lower -> sA
+  {
+    T v[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) v[e] = (T)0;
+    const T* p = part + (int64_t)t.k0 * 4096 + tid;
+    int q = 0;
+    for (; q + 1 < t.kb; q += 2) {
+      T a0[4], a1[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        a0[e] = p[(int64_t)q * 4096 + 1024 * e];
+        a1[e] = p[(int64_t)(q + 1) * 4096 + 1024 * e];
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        v[e] += a0[e];
+        v[e] += a1[e];
+      }
+    }
+    if (q < t.kb) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) v[e] += p[(int64_t)q * 4096 + 1024 * e];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int idx = tid + 1024 * e, m = idx >> 6, c = idx & 63;  // T[m][c] -> sB[c][m]
+      sB[c][m] = v[e];
+      const int r = idx & 63, mm = idx >> 6;  // X_ii lower -> sA[r][mm]
+      sA[r][mm] = (r < ni && mm < ni && r >= mm) ? X[(int64_t)(ri + r) + (int64_t)(ri + mm) * nc] : (T)0;
+    }
   }
   __syncthreads();
+  if (wv >= 4) return;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
   typename MF<T>::acc acc[2][2];
   zero_acc(acc);
   mfma_tile<T>(sA, sB, 64, acc, lane, wr, wc);
@@ -1435,7 +1461,7 @@ void launch_fill(hipStream_t st, double* p, int64_t n, double v) {
   }                                                                                                        \
   void launch_inv_fin(hipStream_t st, const FTask* t, int n, const DevSym& S, const int64_t* ioff,         \
                       T* ipool, const T* ipart) {                                                          \
-    if (n > 0) k_inv_fin<T><<<n, 256, 0, st>>>(t, S, ioff, ipool, ipart);                                    \
+    if (n > 0) k_inv_fin<T><<<n, 1024, 0, st>>>(t, S, ioff, ipool, ipart);                                    \
   }                                                                                                        \
   void launch_mv(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const T* pool,                \
                  const int64_t* ioff, const T* ipool, double* W, double* V, double* X, double* part,       \
