@@ -307,16 +307,12 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 // is FP64 for both storage types (the FP32 factor of the mixed-precision
 // path keeps its diagonal blocks' factorisation in FP64 and rounds only the
 // stored result).  Rows beyond kb are carried as identity rows.
+//
+// diag_block: that factorisation in LDS; sA holds L (unit lower) with D on
+// its diagonal, sD = D, sX = L^-1, all in FP64.
 template <class T>
-__global__ __launch_bounds__(256) void k_diag(const FTask* __restrict__ tasks, DevSym S,
-                                              T* __restrict__ pool, const int64_t* __restrict__ ioff,
-                                              T* __restrict__ ipool, int* __restrict__ piv) {
-  __shared__ double sA[64][65], sX[64][65], sD[64], sT[4][16][17];
-  const FTask t = tasks[blockIdx.x];
-  const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
-  const int kb = min(64, nc - k0);
-  T* F = pool + S.foff[sn];
-  T* Xg = ipool + ioff[sn];
+__device__ __forceinline__ void diag_block(const T* __restrict__ F, int ld, int k0, int kb, double (&sA)[64][65],
+                                           double (&sX)[64][65], double (&sD)[64], double (&sT)[4][16][17]) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
@@ -423,6 +419,15 @@ __global__ __launch_bounds__(256) void k_diag(const FTask* __restrict__ tasks, D
     }
     __syncthreads();
   }
+}
+
+// L, D back into the front and X into the pivot-block inverse pool, plus the
+// positive-definiteness check of the pivots.
+template <class T>
+__device__ __forceinline__ void diag_store(T* __restrict__ F, T* __restrict__ Xg, int ld, int nc, int k0, int kb,
+                                           int col0, const double (&sA)[64][65], const double (&sX)[64][65],
+                                           int* __restrict__ piv) {
+  const int tid = threadIdx.x;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     const int i = idx & 63, m = idx >> 6;
     if (i < kb && m <= i) {
@@ -432,8 +437,90 @@ __global__ __launch_bounds__(256) void k_diag(const FTask* __restrict__ tasks, D
   }
   if (tid < kb && !(sA[tid][tid] > 0.0)) {  // K_ii must be positive definite
     atomicAdd(&piv[0], 1);
-    atomicMin(&piv[1], S.col0[sn] + k0 + tid);
+    atomicMin(&piv[1], col0 + k0 + tid);
   }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_diag(const FTask* __restrict__ tasks, DevSym S,
+                                              T* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                              T* __restrict__ ipool, int* __restrict__ piv) {
+  __shared__ double sA[64][65], sX[64][65], sD[64], sT[4][16][17];
+  const FTask t = tasks[blockIdx.x];
+  const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
+  const int kb = min(64, nc - k0);
+  T* F = pool + S.foff[sn];
+  T* Xg = ipool + ioff[sn];
+  diag_block<T>(F, ld, k0, kb, sA, sX, sD, sT);
+  diag_store<T>(F, Xg, ld, nc, k0, kb, S.col0[sn], sA, sX, piv);
+}
+
+// Diagonal block and triangular solve in one launch, for panels with few
+// row tiles below the diagonal block (the upper levels of the tree, where a
+// separate diagonal-block launch would run on a handful of CUs).  Task: sn,
+// a = row tile (-1: none below), k0 = panel.  Every workgroup factors the
+// diagonal block in LDS (the same FP64 arithmetic as k_diag, so the same
+// bits); the one with a <= 0 stores it.  The row tile then takes
+// L_r = F_r L_kk^{-T} D^{-1} with the factor values rounded to the storage
+// type, exactly as k_trsm reads them back from memory.  The row tile's
+// loads are issued before the factorisation.
+template <class T>
+__global__ __launch_bounds__(256) void k_diag_trsm(const FTask* __restrict__ tasks, DevSym S,
+                                                   T* __restrict__ pool, const int64_t* __restrict__ ioff,
+                                                   T* __restrict__ ipool, int* __restrict__ piv) {
+  __shared__ double sA[64][65], sX[64][65], sD[64], sT[4][16][17];
+  const FTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
+  const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn], nc = S.ncol[sn];
+  const int kb = min(64, nc - k0);
+  T* F = pool + S.foff[sn];
+  T* Xg = ipool + ioff[sn];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int R0 = k0 + kb + 64 * max(t.a, 0);
+  T ra[16];
+  if (t.a >= 0) {
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
+      ra[q] = (m < kb && R0 + i < ld) ? F[(int64_t)(R0 + i) + (int64_t)(k0 + m) * ld] : (T)0;
+    }
+  }
+  diag_block<T>(F, ld, k0, kb, sA, sX, sD, sT);
+  if (t.a <= 0) diag_store<T>(F, Xg, ld, nc, k0, kb, S.col0[sn], sA, sX, piv);
+  if (t.a < 0) return;
+  __syncthreads();  // every read of sA by the store is done
+  // operand tiles in the storage type: the row tile in sA's space, L^-1 in
+  // sX's space (in place for double: each thread rewrites only its entries)
+  T(*sF)[65] = reinterpret_cast<T(*)[65]>(&sA[0][0]);
+  T(*sL)[65] = reinterpret_cast<T(*)[65]>(&sX[0][0]);
+  T xv[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
+    xv[q] = (m < kb && i < kb && i >= m) ? (T)sX[i][m] : (T)0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
+    sF[i][m] = ra[q];
+    sL[i][m] = xv[q];
+  }
+  __syncthreads();
+  const int kbr = (kb + 3) & ~3;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  typename MF<T>::acc acc[2][2];
+  zero_acc(acc);
+  mfma_tile<T>(sF, sL, kbr, acc, lane, wr, wc);
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = R0 + wr + 16 * x + lr;
+        const int j = wc + 16 * y + MF<T>::col(lk, r);
+        if (i < ld && j < kb) F[(int64_t)i + (int64_t)(k0 + j) * ld] = acc[x][y][r] / (T)sD[j];
+      }
 }
 
 // k_trsm task: sn, a = 64-row tile below the diagonal block, k0 = panel.
@@ -1443,6 +1530,10 @@ void launch_fill(hipStream_t st, double* p, int64_t n, double v) {
   void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,   \
                    T* ipool, int* piv) {                                                                   \
     if (n > 0) k_diag<T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, piv);                                   \
+  }                                                                                                        \
+  void launch_diag_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool,                   \
+                        const int64_t* ioff, T* ipool, int* piv) {                                          \
+    if (n > 0) k_diag_trsm<T><<<n, 256, 0, st>>>(t, S, pool, ioff, ipool, piv);                             \
   }                                                                                                        \
   void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,   \
                    const T* ipool) {                                                                       \

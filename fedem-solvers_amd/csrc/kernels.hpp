@@ -30,6 +30,8 @@ constexpr int INV_CHUNK = 4;
                          T* pool);                                                                         \
   void launch_diag(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,  \
                    T* ipool, int* piv);                                                                   \
+  void launch_diag_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool,                  \
+                        const int64_t* ioff, T* ipool, int* piv);                                          \
   void launch_trsm(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool, const int64_t* ioff,  \
                    const T* ipool);                                                                       \
   void launch_update(hipStream_t st, const FTask* t, int n, const DevSym& S, T* pool);                    \

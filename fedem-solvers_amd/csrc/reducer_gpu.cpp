@@ -81,7 +81,7 @@ struct HBuf {
 };
 
 enum OpKind {
-  EXTADD, DIAG, TRSM, UPDATE, INV_OFF, INV_PART, INV_FIN,
+  EXTADD, DIAG, TRSM, DIAG_TRSM, UPDATE, INV_OFF, INV_PART, INV_FIN,
   FW_INIT, FW_INV, FW_UPD, FW_OUT, BW_INIT, BW_UPD, BW_INV,
   FW_INV_R, FW_UPD_R, BW_UPD_R, BW_INV_R  // ordered reductions of split-K partials
 };
@@ -231,7 +231,27 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
     // sub-panel one trailing update over the whole panel, so the trailing
     // block is read and written once per panel.  A front with fewer than P
     // pivot columns left takes plain 64-column panels.
+    // Panels with at most FUSE_TRSM row tiles below their diagonal blocks
+    // (summed over the level) take one fused launch instead.
+    constexpr int FUSE_TRSM = 256;
     auto add_diag_trsm = [&](int k0) {
+      int ntr = 0;
+      for (int q = 0; q < ns; q++) {
+        int s = sups[q];
+        if (y.ncol[s] > k0) ntr += (y.nrow[s] - k0 - std::min(64, y.ncol[s] - k0) + 63) / 64;
+      }
+      if (ntr <= FUSE_TRSM) {
+        int off = (int)tasks.size();
+        for (int q = 0; q < ns; q++) {
+          int s = sups[q];
+          if (y.ncol[s] <= k0) continue;
+          int kb = std::min(64, y.ncol[s] - k0), rem = y.nrow[s] - k0 - kb;
+          if (rem <= 0) tasks.push_back({s, -1, 0, k0, 0});
+          for (int a = 0; 64 * a < rem; a++) tasks.push_back({s, a, 0, k0, 0});
+        }
+        if ((int)tasks.size() > off) ops.push_back({DIAG_TRSM, off, (int)tasks.size() - off});
+        return;
+      }
       int off = (int)tasks.size();
       for (int q = 0; q < ns; q++) {
         int s = sups[q];
@@ -785,6 +805,7 @@ int DeviceReducer::reduce(std::string& err) {
         case EXTADD: launch_extend_add(st, d.ftasks.p + op.off, op.n, d.S, d.xrange.p, pool); break;
         case DIAG: launch_diag(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool, d.piv.p); break;
         case TRSM: launch_trsm(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool); break;
+        case DIAG_TRSM: launch_diag_trsm(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool, d.piv.p); break;
         case UPDATE:
           if (opt_.time_updates) (void)hipEventRecord(d.upd_ev[2 * ui], st);
           launch_update(st, d.ftasks.p + op.off, op.n, d.S, pool);
