@@ -224,7 +224,23 @@ __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ ta
       const int pc = rel[jj];
       T* dst = P + (int64_t)pc * ldp;
       const T* src = C + (int64_t)jj * ldc;
-      for (int ii = jj + lane; ii < nu; ii += 64) dst[rel[ii]] += src[ii];
+      // the rows of one child column land in distinct parent rows, so eight
+      // iterations' loads go out before their stores
+      for (int i0 = jj + lane; i0 < nu; i0 += 64 * 8) {
+        int pr[8];
+        T v[8], o[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int ii = i0 + 64 * u;
+          pr[u] = ii < nu ? rel[ii] : -1;
+          v[u] = ii < nu ? src[ii] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) o[u] = pr[u] >= 0 ? dst[pr[u]] : (T)0;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (pr[u] >= 0) dst[pr[u]] = o[u] + v[u];
+      }
     }
     __syncthreads();
   }
