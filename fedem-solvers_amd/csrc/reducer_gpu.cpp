@@ -160,7 +160,7 @@ struct DeviceReducer::Impl {
   // scale, refinement work (correction, residual, K X) and norm partials
   bool mixed = false;
   bool m_lumped = false;  // only lumped-mass (ANDES4) elements: M is diagonal in a K-shaped pattern
-  int solve_split = 256;
+  int solve_split = 128;
   DBuf<float> pool32, ipool32;
   DBuf<double> dscale, dcs, Dl, Rr, Zr, irpart;
   std::vector<double> s22;  // mixed: Schur complement from the refined B (n2 x n2, column-major)

@@ -24,7 +24,7 @@ struct ReduceOptions {
   // K_ii refined against the FP64 matrix until the correction falls below
   // ir_tol relative (at most ir_maxit steps; no convergence is an error).
   int factor_panel = 128;   // pivot columns per trailing update (64..256, multiple of 64)
-  int solve_split = 256;    // longest K range of one solve GEMM task (longer ones are split and reduced)
+  int solve_split = 128;    // longest K range of one solve GEMM task (longer ones are split and reduced)
   bool mixed = false;
   double ir_tol = 1e-14;
   int ir_maxit = 40;
