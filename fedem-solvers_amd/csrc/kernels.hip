@@ -1315,44 +1315,59 @@ __global__ __launch_bounds__(256) void k_gemm_small(int n, int na, int nb, const
   Y[(int64_t)r * ldy + j] = y + beta * s;
 }
 
-// Y = alpha Y + beta X S for S with at most 16 columns: one thread per row of
-// X keeps the row's 16 sums in registers (S staged in LDS, padded to 16
-// columns), so X is read once, in row order, instead of once per column.
-// Same summation order as k_gemm_small.
+// Y = alpha Y + beta X S for S with at most 16 columns, on FP64 MFMA: a
+// workgroup takes 64 rows of X, staged through LDS in 32-column chunks read
+// in row order (coalesced), the next chunk's loads in flight during the
+// current chunk's products; wave w accumulates rows 16 w .. 16 w + 15.
 __global__ __launch_bounds__(256) void k_gemm_rows16(int n, int na, int nb, const double* __restrict__ X, int ldx,
                                                      const double* __restrict__ Sm, double* __restrict__ Y,
                                                      int ldy, double alpha, double beta) {
-  extern __shared__ double sS[];
-  for (int i = threadIdx.x; i < na * 16; i += blockDim.x) {
-    const int a = i >> 4, j = i & 15;
-    sS[i] = j < nb ? Sm[a * nb + j] : 0.0;
+  __shared__ double sX[64][33];
+  __shared__ double sS[16][33];  // S chunk transposed: sS[j][k]
+  const int r0 = blockIdx.x * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+  double rx[8], rs[2];
+  auto load = [&](int kc) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int idx = tid + 256 * q, k = idx & 31, r = r0 + (idx >> 5);
+      rx[q] = (r < n && kc + k < na) ? X[(int64_t)r * ldx + kc + k] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int idx = tid + 256 * q, k = idx >> 4, j = idx & 15;
+      rs[q] = (kc + k < na && j < nb) ? Sm[(int64_t)(kc + k) * nb + j] : 0.0;
+    }
+  };
+  if (na > 0) load(0);
+  for (int kc = 0; kc < na; kc += 32) {
+    __syncthreads();  // the previous chunk's products are done with the LDS tiles
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int idx = tid + 256 * q;
+      sX[idx >> 5][idx & 31] = rx[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int idx = tid + 256 * q;
+      sS[idx & 15][idx >> 4] = rs[q];
+    }
+    __syncthreads();
+    if (kc + 32 < na) load(kc + 32);
+#pragma unroll
+    for (int m0 = 0; m0 < 32; m0 += 4)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(sS[lr][m0 + lk], sX[16 * wv + lr][m0 + lk], acc, 0, 0, 0);
   }
-  __syncthreads();
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  // acc[q]: row r0 + 16 wv + lr, column lk + 4 q
+  const int r = r0 + 16 * wv + lr;
   if (r >= n) return;
-  const double* x = X + (int64_t)r * ldx;
-  double acc[16];
-#pragma unroll
-  for (int j = 0; j < 16; j++) acc[j] = 0.0;
-  int a = 0;
-  for (; a + 4 <= na; a += 4) {
-    double xv[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) xv[u] = x[a + u];
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int j = 0; j < 16; j++) acc[j] += xv[u] * sS[(a + u) * 16 + j];
-  }
-  for (; a < na; a++) {
-    const double xv = x[a];
-#pragma unroll
-    for (int j = 0; j < 16; j++) acc[j] += xv * sS[a * 16 + j];
-  }
   double* y = Y + (int64_t)r * ldy;
 #pragma unroll
-  for (int j = 0; j < 16; j++)
-    if (j < nb) y[j] = ((alpha == 0.0) ? 0.0 : alpha * y[j]) + beta * acc[j];
+  for (int q = 0; q < 4; q++) {
+    const int j = lk + 4 * q;
+    if (j < nb) y[j] = ((alpha == 0.0) ? 0.0 : alpha * y[j]) + beta * acc[q];
+  }
 }
 
 // X[(ndof1 + b) * nrhs + b] = 1, all other entries 0 (the [B; I] identity rows)
@@ -1631,13 +1646,8 @@ void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx,
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta) {
   if (n <= 0 || nb <= 0) return;
-  if (nb <= 16) {  // one thread per row, S padded to 16 columns, in chunks of 512 rows (64 KiB)
-    for (int a0 = 0; a0 < na || a0 == 0; a0 += 512) {
-      const int ka = std::min(512, na - a0);
-      k_gemm_rows16<<<nblk(n, 256), 256, sizeof(double) * 16 * std::max(ka, 1), st>>>(
-          n, ka, nb, X + a0, ldx, Sm + (size_t)a0 * nb, Y, ldy, a0 == 0 ? alpha : 1.0, beta);
-      if (na <= 0) break;
-    }
+  if (nb <= 16) {  // MFMA over 64-row tiles
+    k_gemm_rows16<<<(unsigned)((n + 63) / 64), 256, 0, st>>>(n, na, nb, X, ldx, Sm, Y, ldy, alpha, beta);
     return;
   }
   // S is staged in LDS in row chunks of at most 64 KiB; chunk k adds

@@ -62,6 +62,9 @@ void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* 
 int xty_chunks(int n);
 void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Y, int ldy,
                 double* part, double* C);
+// Y = alpha Y + beta X S (X n x na, S na x nb, row-major).  For nb <= 16, Y
+// may be X itself (same leading dimension): each row tile reads its rows
+// before it writes them.
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta);
 // sign convention of the modes Phi (nrows x nev row-major): largest-magnitude

@@ -963,20 +963,22 @@ int DeviceReducer::reduce(std::string& err) {
       HIPCHK(hipMemsetAsync(d.lzbad.p, 0, sizeof(int), st));
       // M W is formed once: the second pass's M (W R_1^-1) is (M W) R_1^-1
       launch_spmm(st, neq, bs, d.rowptr.p, d.colind.p, d.Mv.p, d.Wl.p, bs, d.Zl.p, bs, d.m_lumped);
-      double* Zin = d.Zl.p;
-      double* Zout = d.Yl.p;
+      // the first pass rescales W and M W (in place for blocks of up to 16
+      // columns), the second writes block j of Q and P
+      const bool inplace = bs <= 16;
       for (int pass = 0; pass < 2; pass++) {
-        launch_xty(st, neq, bs, bs, d.Wl.p, bs, Zin, bs, d.part.p, d.Cc.p);
+        launch_xty(st, neq, bs, bs, d.Wl.p, bs, d.Zl.p, bs, d.part.p, d.Cc.p);
         launch_chol_inv(st, bs, d.Cc.p, d.Rdev.p + (size_t)pass * bs * bs, d.small.p, d.lzbad.p);
-        launch_gemm_small(st, neq, bs, bs, d.Wl.p, bs, d.small.p, d.Tmp.p, bs, 0.0, 1.0);
-        launch_gemm_small(st, neq, bs, bs, Zin, bs, d.small.p, Zout, bs, 0.0, 1.0);
-        HIPCHK(hipMemcpyAsync(d.Wl.p, d.Tmp.p, sizeof(double) * neq * bs, hipMemcpyDeviceToDevice, st));
-        std::swap(Zin, Zout);
+        double* qo = pass ? d.Q.p + (size_t)j * bs : (inplace ? d.Wl.p : d.Tmp.p);
+        double* po = pass ? d.P.p + (size_t)j * bs : (inplace ? d.Zl.p : d.Yl.p);
+        const int ldo = pass ? mcap : bs;
+        launch_gemm_small(st, neq, bs, bs, d.Wl.p, bs, d.small.p, qo, ldo, 0.0, 1.0);
+        launch_gemm_small(st, neq, bs, bs, d.Zl.p, bs, d.small.p, po, ldo, 0.0, 1.0);
+        if (pass == 0 && !inplace) {
+          HIPCHK(hipMemcpyAsync(d.Wl.p, d.Tmp.p, sizeof(double) * neq * bs, hipMemcpyDeviceToDevice, st));
+          HIPCHK(hipMemcpyAsync(d.Zl.p, d.Yl.p, sizeof(double) * neq * bs, hipMemcpyDeviceToDevice, st));
+        }
       }
-      HIPCHK(hipMemcpy2DAsync(d.Q.p + (size_t)j * bs, sizeof(double) * mcap, d.Wl.p, sizeof(double) * bs,
-                              sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
-      HIPCHK(hipMemcpy2DAsync(d.P.p + (size_t)j * bs, sizeof(double) * mcap, Zin, sizeof(double) * bs,
-                              sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
       HIPCHK(hipMemcpyAsync(d.hR.p + (size_t)par * 2 * bs * bs, d.Rdev.p, sizeof(double) * 2 * bs * bs,
                             hipMemcpyDeviceToHost, st));
       HIPCHK(hipMemcpyAsync(d.hbad.p + par, d.lzbad.p, sizeof(int), hipMemcpyDeviceToHost, st));
