@@ -1098,8 +1098,28 @@ __global__ __launch_bounds__(256) void k_spmm(int n, int nrhs, const int* __rest
   const int kb = rowptr[r], ke = rowptr[r + 1];
   // SKIP: entries that are zero in value (the off-diagonal pattern of a
   // lumped mass) skip their gather of X
+  // entry k goes to sum k mod 4 (relative to kb), each sum in entry order;
+  // sixteen entries' index, value and gather loads are issued together
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   int k = kb;
+  for (; k + 16 <= ke; k += 16) {
+    int c[16];
+    double v[16], x[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      c[u] = colind[k + u];
+      v[u] = val[k + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) x[u] = (!SKIP || v[u] != 0.0) ? X[(int64_t)c[u] * ldx + j] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; u += 4) {
+      if (!SKIP || v[u] != 0.0) s0 += v[u] * x[u];
+      if (!SKIP || v[u + 1] != 0.0) s1 += v[u + 1] * x[u + 1];
+      if (!SKIP || v[u + 2] != 0.0) s2 += v[u + 2] * x[u + 2];
+      if (!SKIP || v[u + 3] != 0.0) s3 += v[u + 3] * x[u + 3];
+    }
+  }
   for (; k + 4 <= ke; k += 4) {
     const int c0 = colind[k], c1 = colind[k + 1], c2 = colind[k + 2], c3 = colind[k + 3];
     const double v0 = val[k], v1 = val[k + 1], v2 = val[k + 2], v3 = val[k + 3];
