@@ -839,14 +839,28 @@ int DeviceReducer::reduce(std::string& err) {
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[3], st));
-  // ---- B = -K_ii^{-1} K_ib: back-substitution of [0; I]
+  // ---- B = -K_ii^{-1} K_ib: back-substitution of [0; I].  With the FP64
+  // factor and component modes it is queued after the Lanczos steps instead,
+  // where it runs on the device while the host solves the final projected
+  // eigenproblem (events 8 and 9 bracket it there).
   SolvePlan* pb = nullptr;
+  const bool defer_b = n2 > 0 && !d.mixed && nev > 0;
   if (n2 > 0) {
     if (get_plan(y, d, n2, pb, err)) return -3;
     if (ensure_work(d, n2, err)) return -3;
     HIPCHK(d.X.alloc((size_t)neq * n2));
     HIPCHK(d.Z.alloc((size_t)neq * n2));
     HIPCHK(d.Y.alloc((size_t)neq * n2));
+  }
+  auto issue_bsolve = [&]() -> int {
+    HIPCHK(hipEventRecord(d.ev[8], st));
+    launch_identity_rows(st, d.X.p, neq, n2, n1);
+    launch_fill(st, d.Y.p, (int64_t)neq * n2, 0.0);
+    run_backward(st, d, *pb, d.Y.p, d.X.p);
+    HIPCHK(hipEventRecord(d.ev[9], st));
+    return 0;
+  };
+  if (n2 > 0 && !defer_b) {
     launch_identity_rows(st, d.X.p, neq, n2, n1);
     launch_fill(st, d.Y.p, (int64_t)neq * n2, 0.0);
     run_backward(st, d, *pb, d.Y.p, d.X.p);
@@ -1141,8 +1155,17 @@ int DeviceReducer::reduce(std::string& err) {
       launch_xty(st, neq, bs, bs, d.Wl.p, bs, d.Zl.p, bs, d.part.p, d.Cc.p);
       std::vector<double> Gl((size_t)bs * bs);
       HIPCHK(hipMemcpyAsync(Gl.data(), d.Cc.p, sizeof(double) * bs * bs, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipEventRecord(d.ev[10], st));
+      if (defer_b && (rc = issue_bsolve())) return rc;
+      HIPCHK(hipEventSynchronize(d.ev[10]));
+      const auto tw = std::chrono::steady_clock::now();
       done = run_check(nb * bs, projected(nb * bs), Gl);
+      if (std::getenv("FGPU_TRACE_LANCZOS"))
+        std::fprintf(stderr, "lanczos: final check of %d vectors on the host took %.3f ms, worst %.2e\n", done.mj,
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tw).count(),
+                     done.worst);
+    } else if (defer_b && (rc = issue_bsolve())) {
+      return rc;
     }
     res_.lanczos_resid = done.worst;
     if (!done.ok) {
@@ -1239,8 +1262,8 @@ int DeviceReducer::reduce(std::string& err) {
   times_.elements = ev_ms(d.ev[0], d.ev[1]);
   times_.assembly = ev_ms(d.ev[1], d.ev[2]);
   times_.factor = ev_ms(d.ev[2], d.ev[3]);
-  times_.bsolve = ev_ms(d.ev[3], d.ev[4]);
-  times_.modes = ev_ms(d.ev[4], d.ev[5]);
+  times_.bsolve = defer_b ? ev_ms(d.ev[8], d.ev[9]) : ev_ms(d.ev[3], d.ev[4]);
+  times_.modes = ev_ms(d.ev[4], d.ev[5]) - (defer_b ? times_.bsolve : 0.0);
   times_.mass = ev_ms(d.ev[5], d.ev[6]);
   times_.gravity = ev_ms(d.ev[6], d.ev[7]);
   times_.total = ev_ms(d.ev[0], d.ev[7]);
