@@ -852,12 +852,18 @@ int DeviceReducer::reduce(std::string& err) {
     HIPCHK(d.Z.alloc((size_t)neq * n2));
     HIPCHK(d.Y.alloc((size_t)neq * n2));
   }
+  // the deferred pass also forms the mass phase's Z = M [B; I] (events 9-11)
+  const bool defer_z = defer_b && !opt_.no_mass;
   auto issue_bsolve = [&]() -> int {
     HIPCHK(hipEventRecord(d.ev[8], st));
     launch_identity_rows(st, d.X.p, neq, n2, n1);
     launch_fill(st, d.Y.p, (int64_t)neq * n2, 0.0);
     run_backward(st, d, *pb, d.Y.p, d.X.p);
     HIPCHK(hipEventRecord(d.ev[9], st));
+    if (defer_z) {
+      launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Mv.p, d.X.p, n2, d.Z.p, n2, d.m_lumped);
+      HIPCHK(hipEventRecord(d.ev[11], st));
+    }
     return 0;
   };
   if (n2 > 0 && !defer_b) {
@@ -1213,7 +1219,7 @@ int DeviceReducer::reduce(std::string& err) {
     return 0;
   };
   if (!opt_.no_mass && n2 > 0) {
-    launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Mv.p, d.X.p, n2, d.Z.p, n2, d.m_lumped);
+    if (!defer_z) launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Mv.p, d.X.p, n2, d.Z.p, n2, d.m_lumped);
     if (xty_host(n2, n2, d.X.p, n2, d.Z.p, n2, M22)) return -3;
   }
   if (ngen > 0) {
@@ -1263,8 +1269,9 @@ int DeviceReducer::reduce(std::string& err) {
   times_.assembly = ev_ms(d.ev[1], d.ev[2]);
   times_.factor = ev_ms(d.ev[2], d.ev[3]);
   times_.bsolve = defer_b ? ev_ms(d.ev[8], d.ev[9]) : ev_ms(d.ev[3], d.ev[4]);
-  times_.modes = ev_ms(d.ev[4], d.ev[5]) - (defer_b ? times_.bsolve : 0.0);
-  times_.mass = ev_ms(d.ev[5], d.ev[6]);
+  const double zms = defer_z ? ev_ms(d.ev[9], d.ev[11]) : 0.0;
+  times_.modes = ev_ms(d.ev[4], d.ev[5]) - (defer_b ? times_.bsolve : 0.0) - zms;
+  times_.mass = ev_ms(d.ev[5], d.ev[6]) + zms;
   times_.gravity = ev_ms(d.ev[6], d.ev[7]);
   times_.total = ev_ms(d.ev[0], d.ev[7]);
   times_.update_kernel_ms = 0;
