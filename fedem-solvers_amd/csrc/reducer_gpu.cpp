@@ -692,7 +692,7 @@ static void approx_solve(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan
                          double* Rin, double* Yw, double* D) {
   launch_scale_rc(st, ndof1, sp.nrhs, d.dscale.p, nullptr, Rin);
   run_forward(st, d, sp, Rin, Yw, ndof1, true, false);
-  launch_fill(st, D, (int64_t)neq * sp.nrhs, 0.0);
+  launch_fill(st, D + (size_t)ndof1 * sp.nrhs, (int64_t)(neq - ndof1) * sp.nrhs, 0.0);  // boundary rows
   run_backward(st, d, sp, Yw, D);
   launch_scale_rc(st, ndof1, sp.nrhs, d.dscale.p, nullptr, D);
 }
@@ -708,7 +708,7 @@ static int solve_int(hipStream_t st, DeviceReducer::Impl& d, const Symbolic& y, 
   const int64_t nall = (int64_t)neq * nr, nint = (int64_t)n1 * nr;
   if (!d.mixed) {
     run_forward(st, d, sp, R, Yw, n1, true, false);
-    launch_fill(st, X, nall, 0.0);
+    launch_fill(st, X + nint, nall - nint, 0.0);  // boundary rows; the backward pass writes the others
     run_backward(st, d, sp, Yw, X);
     if (steps) *steps = 0;
     return 0;
@@ -856,9 +856,8 @@ int DeviceReducer::reduce(std::string& err) {
   const bool defer_z = defer_b && !opt_.no_mass;
   auto issue_bsolve = [&]() -> int {
     HIPCHK(hipEventRecord(d.ev[8], st));
-    launch_identity_rows(st, d.X.p, neq, n2, n1);
-    launch_fill(st, d.Y.p, (int64_t)neq * n2, 0.0);
-    run_backward(st, d, *pb, d.Y.p, d.X.p);
+    launch_identity_rows(st, d.X.p + (size_t)n1 * n2, neq - n1, n2, 0);  // the backward pass writes rows < n1
+    run_backward(st, d, *pb, nullptr, d.X.p);
     HIPCHK(hipEventRecord(d.ev[9], st));
     if (defer_z) {
       launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Mv.p, d.X.p, n2, d.Z.p, n2, d.m_lumped);
@@ -867,9 +866,8 @@ int DeviceReducer::reduce(std::string& err) {
     return 0;
   };
   if (n2 > 0 && !defer_b) {
-    launch_identity_rows(st, d.X.p, neq, n2, n1);
-    launch_fill(st, d.Y.p, (int64_t)neq * n2, 0.0);
-    run_backward(st, d, *pb, d.Y.p, d.X.p);
+    launch_identity_rows(st, d.X.p + (size_t)n1 * n2, neq - n1, n2, 0);  // the backward pass writes rows < n1
+    run_backward(st, d, *pb, nullptr, d.X.p);
     if (d.mixed) {
       // the backward pass of the equilibrated factor gives S_i^-1 B S_b
       launch_recip(st, n2, d.dscale.p + n1, d.dcs.p);
@@ -1088,7 +1086,7 @@ int DeviceReducer::reduce(std::string& err) {
                               sizeof(double) * bs, neq, hipMemcpyDeviceToDevice, st));
       auto solve_step = [&]() {
         run_forward(st, d, *pl, d.Tmp.p, d.Yl.p, n1, true, false);
-        launch_fill(st, d.Wl.p, (int64_t)neq * bs, 0.0);  // boundary rows stay zero
+        launch_fill(st, d.Wl.p + (size_t)n1 * bs, (int64_t)(neq - n1) * bs, 0.0);  // boundary rows stay zero
         run_backward(st, d, *pl, d.Yl.p, d.Wl.p);
       };
       if (d.mixed) {
