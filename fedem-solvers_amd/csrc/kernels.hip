@@ -1023,11 +1023,15 @@ __global__ __launch_bounds__(1024) void k_mv_red(const SolveTask* __restrict__ t
 // rows from the global right-hand side R (row-major, perm rows; the Schur
 // front takes the boundary rows), update rows zero, plus the children's
 // update rows that land in each row.  Work row w adds the rows
-// gsrc[gptr[w]] .. gsrc[gptr[w+1]-1] of W in child order.
+// gsrc[gptr[w]] .. gsrc[gptr[w+1]-1] of W in child order.  A gathered row
+// whose update product was split (ugrp[row] = g >= 0) is completed here:
+// W row minus the sum of its group's partials in slot order (group g:
+// first slot, slot count, first work row of the tile at ugroups[3 g ..]).
 __global__ __launch_bounds__(1024) void k_fw_init(const SolveTask* __restrict__ tasks, DevSym S,
                                                  const int* __restrict__ gptr, const int* __restrict__ gsrc,
-                                                 const double* __restrict__ R, double* __restrict__ W, int nrhs,
-                                                 int ndof1) {
+                                                 const int* __restrict__ ugrp, const int* __restrict__ ugroups,
+                                                 const double* __restrict__ part, const double* __restrict__ R,
+                                                 double* __restrict__ W, int nrhs, int ndof1) {
   const SolveTask t = tasks[blockIdx.x];
   const int sn = t.sn, nc = S.ncol[sn], c0 = S.col0[sn];
   const int64_t wo = S.woff[sn];
@@ -1038,7 +1042,28 @@ __global__ __launch_bounds__(1024) void k_fw_init(const SolveTask* __restrict__ 
     double v = 0.0;
     if (i < nc || nc == 0) v = R[(int64_t)(base + i) * nrhs + j];
     const int e1 = gptr[wo + i + 1];
-    for (int e = gptr[wo + i]; e < e1; e++) v += W[(int64_t)gsrc[e] * nrhs + j];
+    for (int e = gptr[wo + i]; e < e1; e++) {
+      const int w = gsrc[e];
+      double t = W[(int64_t)w * nrhs + j];
+      const int g = ugrp ? ugrp[w] : -1;
+      if (g >= 0) {
+        const int first = ugroups[3 * g], ns = ugroups[3 * g + 1], w0 = ugroups[3 * g + 2];
+        const double* p = part + (int64_t)first * 4096 + (w - w0) * 64 + j;
+        double sum = 0.0;
+        int q = 0;
+        for (; q + 3 < ns; q += 4) {
+          const double a0 = p[(int64_t)q * 4096], a1 = p[(int64_t)(q + 1) * 4096];
+          const double a2 = p[(int64_t)(q + 2) * 4096], a3 = p[(int64_t)(q + 3) * 4096];
+          sum += a0;
+          sum += a1;
+          sum += a2;
+          sum += a3;
+        }
+        for (; q < ns; q++) sum += p[(int64_t)q * 4096];
+        t -= sum;
+      }
+      v += t;
+    }
     Wf[(int64_t)i * nrhs + j] = v;
   }
 }
@@ -1632,8 +1657,10 @@ void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, d
   if (n > 0) k_mv_red<<<n, nrhs <= 16 ? 64 * nrhs : 1024, 0, st>>>(t, S, W, V, X, part, nrhs, mode);
 }
 void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const int* gptr, const int* gsrc,
-                    const double* R, double* W, int nrhs, int ndof1) {
-  if (n > 0) k_fw_init<<<n, nrhs <= 16 ? 64 * nrhs : 1024, 0, st>>>(t, S, gptr, gsrc, R, W, nrhs, ndof1);
+                    const int* ugrp, const int* ugroups, const double* part, const double* R, double* W, int nrhs,
+                    int ndof1) {
+  if (n > 0)
+    k_fw_init<<<n, nrhs <= 16 ? 64 * nrhs : 1024, 0, st>>>(t, S, gptr, gsrc, ugrp, ugroups, part, R, W, nrhs, ndof1);
 }
 void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y, double* W,
                     int nrhs) {

@@ -53,7 +53,8 @@ FGPU_FACTOR_DECLS(float)
 void launch_mv_red(hipStream_t st, const SolveTask* t, int n, const DevSym& S, double* W, double* V, double* X,
                    const double* part, int nrhs, int mode);
 void launch_fw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const int* gptr, const int* gsrc,
-                    const double* R, double* W, int nrhs, int ndof1);
+                    const int* ugrp, const int* ugroups, const double* part, const double* R, double* W, int nrhs,
+                    int ndof1);
 void launch_bw_init(hipStream_t st, const SolveTask* t, int n, const DevSym& S, const double* Y, double* W,
                     int nrhs);
 void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* colind, const double* val,

@@ -95,6 +95,11 @@ struct SolvePlan {
   std::vector<Op> fw_int, fw_all, bw;
   DBuf<SolveTask> tasks;
   DBuf<int> gptr, gsrc;  // forward init: the child update rows added to each work row, in child order
+  // Split-K update-row products of the forward pass (nrhs <= 64) are reduced
+  // by the parent's forward init as it gathers the rows: ugrp_int/ugrp_all
+  // (per work row, -1: none) index the groups (first slot, slots, first
+  // work row of the tile) of the internal-only and the full forward pass.
+  DBuf<int> ugrp_int, ugrp_all, ugroups;
   DBuf<double> part;
 };
 
@@ -338,7 +343,8 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
 }
 
 static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<SolveTask>& tasks,
-                             std::vector<int>& gptr, std::vector<int>& gsrc, int KS) {
+                             std::vector<int>& gptr, std::vector<int>& gsrc, std::vector<int> (&ugrp)[2],
+                             std::vector<int>& ugroups, int KS) {
   const int nrhs = sp.nrhs, schur = y.nsup - 1;
   // gather lists of the forward init: work row woff[s] + i of front s adds
   // the update rows of s's children that land in its row i, child after
@@ -377,6 +383,10 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
   };
   // GEMM tasks over the row tiles of a front and all right-hand-side tiles;
   // K ranges longer than KS are split and reduced in slot order
+  const bool fuse_upd = nrhs <= 64;
+  ugroups.clear();
+  for (auto& u : ugrp) u.assign(woff[y.nsup], -1);
+  int pass = 0;  // forward pass being built (0: internal fronts only, 1: all)
   auto add_tiles = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list, bool upd_rows) {
     const int mode = kind == FW_INV ? 0 : kind == FW_UPD ? 1 : kind == BW_UPD ? 2 : 3;
     const int rkind = kind == FW_INV ? FW_INV_R : kind == FW_UPD ? FW_UPD_R : kind == BW_UPD ? BW_UPD_R : BW_INV_R;
@@ -402,6 +412,12 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
           }
           int first = sp.nslots;
           for (int k = kb; k < ke; k += KS) tasks.push_back({s, r0, rn, k, std::min(KS, ke - k), j0, sp.nslots++, 0});
+          if (mode == 1 && fuse_upd) {
+            const int w0 = (int)(woff[s] + nc + r0), g = (int)ugroups.size() / 3;
+            ugroups.insert(ugroups.end(), {first, sp.nslots - first, w0});
+            for (int i = 0; i < rn; i++) ugrp[pass][w0 + i] = g;
+            continue;
+          }
           red.push_back({s, r0, rn, first, sp.nslots - first, j0, -2, 0});
         }
       }
@@ -416,7 +432,7 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
   // forward: levels bottom-up; fw_all also carries the Schur front.  The
   // solved pivot rows of a front stay in V, so one output launch after the
   // last level copies them all.
-  for (int pass = 0; pass < 2; pass++) {
+  for (pass = 0; pass < 2; pass++) {
     std::vector<Op>& ops = pass == 0 ? sp.fw_int : sp.fw_all;
     std::vector<int> all;
     for (int l = 0; l < y.nlevels; l++) {
@@ -583,11 +599,14 @@ static int get_plan(const Symbolic& y, DeviceReducer::Impl& d, int nrhs, SolvePl
   auto sp = std::make_unique<SolvePlan>();
   sp->nrhs = nrhs;
   std::vector<SolveTask> tasks;
-  std::vector<int> gptr, gsrc;
-  build_solve_plan(y, *sp, tasks, gptr, gsrc, d.solve_split);
+  std::vector<int> gptr, gsrc, ugrp[2], ugroups;
+  build_solve_plan(y, *sp, tasks, gptr, gsrc, ugrp, ugroups, d.solve_split);
   HIPCHK(sp->tasks.upload(tasks));
   HIPCHK(sp->gptr.upload(gptr));
   HIPCHK(sp->gsrc.upload(gsrc.empty() ? std::vector<int>(1, 0) : gsrc));
+  HIPCHK(sp->ugrp_int.upload(ugrp[0]));
+  HIPCHK(sp->ugrp_all.upload(ugrp[1]));
+  HIPCHK(sp->ugroups.upload(ugroups.empty() ? std::vector<int>(3, 0) : ugroups));
   HIPCHK(sp->part.alloc((size_t)std::max(1, sp->nslots) * 4096));
   out = sp.get();
   d.plans[nrhs] = std::move(sp);
@@ -628,7 +647,10 @@ static void run_forward(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan&
   const std::vector<Op>& ops = all ? sp.fw_all : sp.fw_int;
   for (const Op& op : ops) {
     switch (op.kind) {
-      case FW_INIT: launch_fw_init(st, sp.tasks.p + op.off, op.n, d.S, sp.gptr.p, sp.gsrc.p, R, d.W.p, sp.nrhs, ndof1); break;
+      case FW_INIT:
+        launch_fw_init(st, sp.tasks.p + op.off, op.n, d.S, sp.gptr.p, sp.gsrc.p, all ? sp.ugrp_all.p : sp.ugrp_int.p,
+                       sp.ugroups.p, sp.part.p, R, d.W.p, sp.nrhs, ndof1);
+        break;
       case FW_OUT:
         if (d.mixed)
           launch_fw_out(st, sp.tasks.p + op.off, op.n, d.S, d.pool32.p, d.W.p, d.V.p, Yout, sp.nrhs, scale, ndof1);
