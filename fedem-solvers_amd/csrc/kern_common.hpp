@@ -38,6 +38,7 @@ struct DevSym {
   const int* relptr;
   const int* rel;
   const int64_t* woff;  // per-front offsets of solve work buffers (rows)
+  double* dpiv;         // pivots D of the LDL^T factor by pivot column (col0[front] + column)
 };
 
 }  // namespace fgpu

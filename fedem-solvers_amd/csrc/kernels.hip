@@ -437,23 +437,26 @@ __device__ __forceinline__ void diag_block(const T* __restrict__ F, int ld, int 
   }
 }
 
-// L, D back into the front and X into the pivot-block inverse pool, plus the
-// positive-definiteness check of the pivots.
+// X = L^-1 into the diagonal tile of the pivot-block inverse pool and D into
+// the pivot array, plus the positive-definiteness check of the pivots.  The
+// front's diagonal block itself is left alone (later kernels read D from the
+// pivot array, the solves L^-1 from the inverse pool), so that other
+// workgroups of the same launch can still read it.
 template <class T>
-__device__ __forceinline__ void diag_store(T* __restrict__ F, T* __restrict__ Xg, int ld, int nc, int k0, int kb,
+__device__ __forceinline__ void diag_store(T* __restrict__ Xg, double* __restrict__ dpiv, int nc, int k0, int kb,
                                            int col0, const double (&sA)[64][65], const double (&sX)[64][65],
                                            int* __restrict__ piv) {
   const int tid = threadIdx.x;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     const int i = idx & 63, m = idx >> 6;
-    if (i < kb && m <= i) {
-      F[(int64_t)(k0 + i) + (int64_t)(k0 + m) * ld] = (T)sA[i][m];
-      Xg[(int64_t)(k0 + i) + (int64_t)(k0 + m) * nc] = (T)sX[i][m];
-    }
+    if (i < kb && m <= i) Xg[(int64_t)(k0 + i) + (int64_t)(k0 + m) * nc] = (T)sX[i][m];
   }
-  if (tid < kb && !(sA[tid][tid] > 0.0)) {  // K_ii must be positive definite
-    atomicAdd(&piv[0], 1);
-    atomicMin(&piv[1], col0 + k0 + tid);
+  if (tid < kb) {
+    dpiv[col0 + k0 + tid] = sA[tid][tid];
+    if (!(sA[tid][tid] > 0.0)) {  // K_ii must be positive definite
+      atomicAdd(&piv[0], 1);
+      atomicMin(&piv[1], col0 + k0 + tid);
+    }
   }
 }
 
@@ -468,7 +471,7 @@ __global__ __launch_bounds__(256) void k_diag(const FTask* __restrict__ tasks, D
   T* F = pool + S.foff[sn];
   T* Xg = ipool + ioff[sn];
   diag_block<T>(F, ld, k0, kb, sA, sX, sD, sT);
-  diag_store<T>(F, Xg, ld, nc, k0, kb, S.col0[sn], sA, sX, piv);
+  diag_store<T>(Xg, S.dpiv, nc, k0, kb, S.col0[sn], sA, sX, piv);
 }
 
 // Diagonal block and triangular solve in one launch, for panels with few
@@ -501,7 +504,7 @@ __global__ __launch_bounds__(256) void k_diag_trsm(const FTask* __restrict__ tas
     }
   }
   diag_block<T>(F, ld, k0, kb, sA, sX, sD, sT);
-  if (t.a <= 0) diag_store<T>(F, Xg, ld, nc, k0, kb, S.col0[sn], sA, sX, piv);
+  if (t.a <= 0) diag_store<T>(Xg, S.dpiv, nc, k0, kb, S.col0[sn], sA, sX, piv);
   if (t.a < 0) return;
   __syncthreads();  // every read of sA by the store is done
   // operand tiles in the storage type: the row tile in sA's space, L^-1 in
@@ -585,7 +588,7 @@ __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, D
       for (int r = 0; r < 4; r++) {
         const int i = R0 + wr + 16 * x + lr;
         const int j = wc + 16 * y + MF<T>::col(lk, r);
-        if (i < ld && j < kb) F[(int64_t)i + (int64_t)(k0 + j) * ld] = acc[x][y][r] / F[(int64_t)(k0 + j) * (ld + 1)];
+        if (i < ld && j < kb) F[(int64_t)i + (int64_t)(k0 + j) * ld] = acc[x][y][r] / (T)S.dpiv[S.col0[sn] + k0 + j];
       }
 }
 
@@ -606,6 +609,7 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
   const int t0 = k0 + kb;
   const int I0 = t0 + 64 * t.a, J0 = t0 + 64 * t.b;
   T* F = pool + S.foff[sn];
+  const int c0s = S.col0[sn];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
@@ -619,7 +623,7 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
     for (int q = 0; q < 8; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
       const bool okm = m < kc;
-      rd[q] = okm ? F[(int64_t)(p0 + m) * (ld + 1)] : (T)0;
+      rd[q] = okm ? (T)S.dpiv[c0s + p0 + m] : (T)0;
       ri[q] = (okm && I0 + i < ld) ? F[(int64_t)(I0 + i) + (int64_t)(p0 + m) * ld] : (T)0;
       rj[q] = (okm && J0 + i < ld) ? F[(int64_t)(J0 + i) + (int64_t)(p0 + m) * ld] : (T)0;
     }
@@ -1077,14 +1081,14 @@ __global__ __launch_bounds__(256) void k_fw_out(const SolveTask* __restrict__ ta
                                                 const double* __restrict__ W, const double* __restrict__ V,
                                                 double* __restrict__ Y, int nrhs, int scale, int ndof1) {
   const SolveTask t = tasks[blockIdx.x];
-  const int sn = t.sn, nc = S.ncol[sn], nr = S.nrow[sn], c0 = S.col0[sn];
-  const T* F = pool + S.foff[sn];
+  const int sn = t.sn, nc = S.ncol[sn], c0 = S.col0[sn];
+  (void)pool;  // the storage type of the factor sets the rounding of the pivots
   const double* Wf = (nc > 0 ? V : W) + S.woff[sn] * (int64_t)nrhs;
   const int base = nc > 0 ? c0 : ndof1;
   for (int idx = threadIdx.x; idx < t.rn * nrhs; idx += 256) {
     const int i = t.r0 + idx / nrhs, j = idx % nrhs;
     double v = Wf[(int64_t)i * nrhs + j];
-    if (scale && nc > 0) v /= (double)F[(int64_t)i * (nr + 1)];
+    if (scale && nc > 0) v /= (double)(T)S.dpiv[c0 + i];
     Y[(int64_t)(base + i) * nrhs + j] = v;
   }
 }

@@ -153,6 +153,7 @@ struct DeviceReducer::Impl {
   DevSym S{};
   DBuf<FTask> ftasks;
   DBuf<int> xrange;  // extend-add tasks: (child, first, end) column ranges
+  DBuf<double> dpiv;   // pivots D of the factor by pivot column (FP64 for both factor types)
   DBuf<double> ipart;  // partial sums of split pivot-block inverse tiles (64 x 64 each; FP32 ones use half)
   std::vector<Op> fops;
   std::map<int, std::unique_ptr<SolvePlan>> plans;
@@ -544,8 +545,9 @@ int DeviceReducer::upload(std::string& err) {
   HIPCHK(d.cidx.upload(y.cidx));
   if (!y.ccoef.empty()) HIPCHK(d.ccoef.upload(y.ccoef));
   HIPCHK(d.fmap.upload(y.fmap));
+  HIPCHK(d.dpiv.alloc((size_t)std::max(1, y.ndof1)));
   d.S = DevSym{d.col0.p, d.ncol.p, d.nrow.p, d.foff.p, d.rptr.p, d.rows.p, d.chptr.p,
-               d.chlist.p, d.relptr.p, d.rel.p, d.woff.p};
+               d.chlist.p, d.relptr.p, d.rel.p, d.woff.p, d.dpiv.p};
   std::vector<FTask> ft;
   d.fops.clear();
   std::vector<int> xr;
@@ -856,7 +858,7 @@ int DeviceReducer::reduce(std::string& err) {
     // the factor schedule is fixed per symbolic structure: one graph
     HIPCHK(run_graph(st, d.g_factor,
                      {d.ftasks.p, d.xrange.p, d.ipart.p, d.pool.p, d.ipool.p, d.pool32.p, d.ipool32.p, d.ioff.p,
-                      d.piv.p, d.S.foff, d.S.rows},
+                      d.piv.p, d.S.foff, d.S.rows, d.S.dpiv},
                      factor_ops));
   }
   HIPCHK(hipGetLastError());
@@ -1116,7 +1118,7 @@ int DeviceReducer::reduce(std::string& err) {
       } else if (!opt_.use_graphs) solve_step();
       else HIPCHK(run_graph(st, d.g_lanczos,
                        {pl, pl->tasks.p, pl->part.p, d.Tmp.p, d.Yl.p, d.Wl.p, d.W.p, d.V.p, d.pool.p, d.ipool.p,
-                        d.S.foff, d.S.woff}, solve_step));
+                        d.S.foff, d.S.woff, d.S.dpiv}, solve_step));
       launch_fill(st, d.Cacc.p, (int64_t)mj * bs, 0.0);
       for (int pass = 0; pass < 2; pass++) {
         launch_xty(st, neq, mj, bs, d.P.p, mcap, d.Wl.p, bs, d.part.p, d.Cc.p);
