@@ -74,3 +74,23 @@ def test_server_matches_oracle():
     Kii = o["K"][i1][:, i1].tocsc()
     Ui = o["B"] @ ubm + spla.spsolve(Kii, F[i1])
     assert rel(r["Ui"], Ui) <= 1e-10
+
+
+def test_server_heavy_mix_same_as_alone():
+    """Sixteen slots busy at once (workgroups of one launch start late under
+    the load of the other streams): every request still gives the bits of
+    its single-slot run."""
+    rng = np.random.default_rng(0)
+    reqs = [parts.served_request(rng, median_dof=20000, cap_dof=60000) for _ in range(24)]
+    print("server: %d requests alone" % len(reqs), flush=True)
+    s1 = Server(slots=1)
+    alone = [s1.result(s1.submit(*r)) for r in reqs]
+    s1.close()
+    print("server: the same on 16 slots", flush=True)
+    s16 = Server(slots=16)
+    ids = [s16.submit(*r) for r in reqs]
+    mixed = [s16.result(i) for i in ids]
+    s16.close()
+    for a, m in zip(alone, mixed):
+        for k in ("SK", "SM", "G", "Fred", "Ui"):
+            assert np.array_equal(a[k], m[k]), k
