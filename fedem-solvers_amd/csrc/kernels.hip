@@ -1506,6 +1506,63 @@ __global__ __launch_bounds__(64) void k_chol_inv(int n, const double* __restrict
   }
 }
 
+// k_chol_inv for n <= 16 in one wave with the rows in registers (lane i <
+// 16 holds row i): the LDL^T readlane sweep of k_diag gives the unit lower
+// L1, D and X1 = L1^-1, then G = R^T R with R = D^1/2 L1^T and
+// R^-1 = X1^T D^-1/2.  Rows and columns beyond n are identity.
+__global__ __launch_bounds__(64) void k_chol16(int n, const double* __restrict__ G, double* __restrict__ R,
+                                               double* __restrict__ Ri, int* __restrict__ bad) {
+  const int lane = threadIdx.x;
+  double a[16], x[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    double v = lane == m ? 1.0 : 0.0;
+    if (lane < n && m < n) v = lane == m ? G[lane * n + m] : 0.5 * (G[lane * n + m] + G[m * n + lane]);
+    a[m] = m <= lane ? v : 0.0;
+    x[m] = lane == m ? 1.0 : 0.0;
+  }
+  bool nonpos = false;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    double dj = lane_bcast(a[j], j);
+    if (!(dj > 0.0)) {
+      nonpos = true;
+      dj = 1.0;
+    }
+    const bool below = lane > j && lane < 16;
+    const double l = below ? a[j] / dj : 0.0;
+    if (below) a[j] = l;
+    if (lane == j) a[j] = dj;
+    const double ldj = l * dj;
+#pragma unroll
+    for (int m = j + 1; m < 16; m++) {
+      const double lm = lane_bcast(l, m);  // L1[m][j]
+      if (m <= lane) a[m] -= ldj * lm;
+    }
+#pragma unroll
+    for (int m = 0; m <= j; m++) x[m] -= l * lane_bcast(x[m], j);  // X1[j][m]
+  }
+  if (nonpos && lane == 0) *bad = 1;
+  // lane i: D_i = a[i]; L1[i][m] = a[m] (m < i), X1[i][m] = x[m] (m <= i)
+  double sq[16];
+#pragma unroll
+  for (int m = 0; m < 16; m++) sq[m] = sqrt(lane_bcast(a[m], m));
+  if (lane >= n) return;
+  double si = 1.0;
+#pragma unroll
+  for (int m = 0; m < 16; m++) si = m == lane ? sq[m] : si;
+  const double ri = 1.0 / si;
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    if (m >= n) break;
+    const double sm = sq[m];
+    // R[m][i] = sqrt(D_m) L1[i][m] (m < i), sqrt(D_i) on the diagonal, 0 above
+    R[m * n + lane] = m < lane ? sm * a[m] : (m == lane ? si : 0.0);
+    // R^-1[m][i] = X1[i][m] / sqrt(D_i) (m <= i)
+    Ri[m * n + lane] = m <= lane ? x[m] * ri : 0.0;
+  }
+}
+
 // ---------------------------------------------------------------- mixed precision
 // Symmetric diagonal equilibration of the stiffness matrix for the FP32
 // factor: s[r] = 1/sqrt(K[r][r]) for every perm row r of the CSR matrix
@@ -1724,7 +1781,8 @@ void launch_sum_fixed(hipStream_t st, int n, const double* x, double* out) {
   k_sum_fixed<<<1, 256, 0, st>>>(n, x, out);
 }
 void launch_chol_inv(hipStream_t st, int n, const double* G, double* R, double* Ri, int* bad) {
-  k_chol_inv<<<1, 64, 0, st>>>(n, G, R, Ri, bad);
+  if (n <= 16) k_chol16<<<1, 64, 0, st>>>(n, G, R, Ri, bad);
+  else k_chol_inv<<<1, 64, 0, st>>>(n, G, R, Ri, bad);
 }
 void launch_diag_scale(hipStream_t st, int n, const int* rowptr, const int* colind, const double* val, double* s) {
   if (n > 0) k_diag_scale<<<nblk(n, 256), 256, 0, st>>>(n, rowptr, colind, val, s);
