@@ -1164,6 +1164,85 @@ __global__ __launch_bounds__(256) void k_spmm(int n, int nrhs, const int* __rest
   Z[(int64_t)r * ldz + j] = (s0 + s1) + (s2 + s3);
 }
 
+// k_spmm with four columns per thread (nrhs, ldx and ldz multiples of 4):
+// each entry's gather is two 16-byte loads and its index and value are
+// loaded once for the four columns; every column keeps k_spmm's four sums
+// (entry k to sum k mod 4 relative to kb, in entry order), so the results
+// are the same bits.
+template <bool SKIP>
+__global__ __launch_bounds__(256) void k_spmm4(int n, int nrhs, const int* __restrict__ rowptr,
+                                               const int* __restrict__ colind, const double* __restrict__ val,
+                                               const double* __restrict__ X, int ldx, double* __restrict__ Z, int ldz) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int wcol = nrhs <= 128 ? nrhs : 64, q4 = wcol / 4;
+  const int64_t per = (int64_t)n * q4;
+  const int jb = (int)(g / per);
+  const int64_t rem = g - (int64_t)jb * per;
+  const int r = (int)(rem / q4), j0 = jb * wcol + 4 * (int)(rem % q4);
+  if (r >= n || j0 >= nrhs) return;
+  const int kb = rowptr[r], ke = rowptr[r + 1];
+  double s[4][4];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int m = 0; m < 4; m++) s[c][m] = 0.0;
+  auto gather = [&](int c, double2& a, double2& b) {
+    const double2* p = reinterpret_cast<const double2*>(X + (int64_t)c * ldx + j0);
+    a = p[0];
+    b = p[1];
+  };
+  int k = kb;
+  for (; k + 8 <= ke; k += 8) {
+    int cc[8];
+    double v[8];
+    double2 xa[8], xb[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      cc[u] = colind[k + u];
+      v[u] = val[k + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      if (!SKIP || v[u] != 0.0) gather(cc[u], xa[u], xb[u]);
+      else xa[u] = xb[u] = double2{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      if (SKIP && v[u] == 0.0) continue;
+      s[0][u & 3] += v[u] * xa[u].x;
+      s[1][u & 3] += v[u] * xa[u].y;
+      s[2][u & 3] += v[u] * xb[u].x;
+      s[3][u & 3] += v[u] * xb[u].y;
+    }
+  }
+  for (; k + 4 <= ke; k += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double vv = val[k + u];
+      if (SKIP && vv == 0.0) continue;
+      double2 a, b;
+      gather(colind[k + u], a, b);
+      s[0][u] += vv * a.x;
+      s[1][u] += vv * a.y;
+      s[2][u] += vv * b.x;
+      s[3][u] += vv * b.y;
+    }
+  }
+  for (; k < ke; k++) {
+    const double vv = val[k];
+    if (SKIP && vv == 0.0) continue;
+    double2 a, b;
+    gather(colind[k], a, b);
+    s[0][0] += vv * a.x;
+    s[1][0] += vv * a.y;
+    s[2][0] += vv * b.x;
+    s[3][0] += vv * b.y;
+  }
+  double* z = Z + (int64_t)r * ldz + j0;
+#pragma unroll
+  for (int c = 0; c < 4; c++) z[c] = (s[c][0] + s[c][1]) + (s[c][2] + s[c][3]);
+}
+
 // Partial C = X^T Y: blockIdx.x = chunk of crows rows, blockIdx.y = 64
 // columns of X (a), blockIdx.z = 64 columns of Y (b); lane -> a, wave -> b
 // (b = 16*wave + u).  part[chunk][a][b] (na x nb row-major per chunk).
@@ -1732,8 +1811,15 @@ void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* 
   if (n > 0 && nrhs > 0) {
     const int wcol = nrhs <= 128 ? nrhs : 64;
     const int64_t threads = (int64_t)n * wcol * ((nrhs + wcol - 1) / wcol);
-    if (skip_zeros) k_spmm<true><<<nblk(threads, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
-    else k_spmm<false><<<nblk(threads, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
+    if (nrhs % 4 == 0 && ldx % 4 == 0 && ldz % 4 == 0 && (reinterpret_cast<uintptr_t>(X) & 31) == 0) {
+      const int64_t t4 = threads / 4;
+      if (skip_zeros) k_spmm4<true><<<nblk(t4, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
+      else k_spmm4<false><<<nblk(t4, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
+    } else if (skip_zeros) {
+      k_spmm<true><<<nblk(threads, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
+    } else {
+      k_spmm<false><<<nblk(threads, 256), 256, 0, st>>>(n, nrhs, rowptr, colind, val, X, ldx, Z, ldz);
+    }
   }
 }
 int xty_chunks(int n) { return (n + XTY_ROWS - 1) / XTY_ROWS; }
