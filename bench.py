@@ -260,7 +260,7 @@ def main():
     if not a.no_mixed:
         line["mixed_precision"] = mixed_plate(a.seed)
     if not a.no_serving:
-        line["batched"] = batched(128, 16, a.seed)
+        line["batched"] = batched(256, 16, a.seed)
         line["served"] = served(160, 16, a.seed)
     if not a.no_cpu_baseline:
         rate_sample = cpu_baseline(a.seed)

@@ -864,8 +864,9 @@ __global__ __launch_bounds__(1024) void k_inv_fin(const FTask* __restrict__ task
 //   mode 2 BW_UPD : W[r]     -=  sum_{k<nu}   L[nc+k][r]   X[rows[nc+k]]
 //   mode 3 BW_INV : X[c0+r]   =  sum_{k>=r}   Linv[k][r]   W[k]
 // Long K ranges are split over several tasks (slot >= 0) whose 64x64
-// partials k_mv_red then sums in slot order, so the result does not depend
-// on scheduling.
+// partials k_mv_red then sums in slot order (for the forward update rows
+// with at most 64 right-hand sides, the parent's k_fw_init does), so the
+// result does not depend on scheduling.
 __device__ __forceinline__ void mv_apply(int mode, const DevSym& S, int sn, int nc, int r, int j, int nrhs,
                                          double v, double* Wf, double* Vf, double* X) {
   switch (mode) {

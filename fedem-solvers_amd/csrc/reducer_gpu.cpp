@@ -413,7 +413,7 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
           }
           int first = sp.nslots;
           for (int k = kb; k < ke; k += KS) tasks.push_back({s, r0, rn, k, std::min(KS, ke - k), j0, sp.nslots++, 0});
-          if (mode == 1 && fuse_upd) {
+          if (mode == 1 && fuse_upd && pass < 2) {
             const int w0 = (int)(woff[s] + nc + r0), g = (int)ugroups.size() / 3;
             ugroups.insert(ugroups.end(), {first, sp.nslots - first, w0});
             for (int i = 0; i < rn; i++) ugrp[pass][w0 + i] = g;
