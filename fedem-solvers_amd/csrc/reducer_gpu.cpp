@@ -954,10 +954,16 @@ int DeviceReducer::reduce(std::string& err) {
   std::vector<double> Sres;
   if (nev > 0) {
     int rc = 0;
-    const int bs = std::max(1, std::min(std::min(opt_.lanczos_block, 64), n1));  // block Gram matrices <= 64 x 64
+    int bs = std::max(1, std::min(std::min(opt_.lanczos_block, 64), n1));  // block Gram matrices <= 64 x 64
     int mcap = opt_.lanczos_maxvec > 0 ? opt_.lanczos_maxvec : std::max(10 * nev + 40, 16 * bs);
-    mcap = std::min(mcap, n1);
-    mcap = std::max(bs, (mcap / bs) * bs);
+    if (mcap >= n1) {
+      // the basis may span the whole internal space: a block size that
+      // divides n1 lets it do so exactly (then the Ritz pairs are exact)
+      while (n1 % bs) bs--;
+      mcap = n1;
+    } else {
+      mcap = std::max(bs, (mcap / bs) * bs);
+    }
     SolvePlan* pl = nullptr;
     if (get_plan(y, d, bs, pl, err)) return -3;
     if (ensure_work(d, std::max(bs, n2), err)) return -3;

@@ -294,6 +294,26 @@ def test_lanczos_block_sizes_and_unconverged_basis():
         R.reduce()
 
 
+def test_small_part_modes_span_whole_internal_space():
+    """A beam frame of 108 internal DOFs with 6 modes (part 225 of the seeded
+    batched mix): the default basis cap exceeds the internal space, so the
+    block size divides it and the basis spans it; the modes converge and
+    match the oracle."""
+    rng = np.random.default_rng(0)
+    part, ngen = [parts.random_part(rng) for _ in range(226)][225]
+    R = gpu.GpuReducer()
+    R.set_part(part)
+    sizes = R.analyze(ngen=ngen)
+    assert sizes["ndof1"] == 108 and ngen == 6
+    R.reduce()
+    SK, SM, G = R.reduced()
+    lam, _ = R.eigen()
+    o = orc.reduce(part, ngen=ngen)
+    assert rel(SK, o["SK"]) <= TOL
+    assert rel(SM, o["SM"]) <= TOL
+    assert np.max(np.abs(lam - o["lam"]) / o["lam"]) <= 1e-9
+
+
 @pytest.mark.parametrize("panel", [64, 192, 256])
 def test_factor_panel_widths_match_oracle(panel):
     """Panels of 64 to 256 pivot columns per trailing update (sub-panel
