@@ -200,6 +200,7 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
 }
 
 // ---------------------------------------------------------------- extend-add
+#define XADD_REL 4096
 // Task: sn = parent front, a = first parent column of a 16-column block.  The
 // workgroup adds, child after child, every child update-matrix column whose
 // parent column falls in its block; the task's b/kb index the (child,
@@ -209,6 +210,9 @@ __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
 template <class T>
 __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ tasks, DevSym S,
                                                     const int* __restrict__ xrange, T* __restrict__ pool) {
+  // the child's parent positions rel[a..nu) staged in LDS (when they fit),
+  // so that the parent loads of a batch wait on one memory round, not two
+  __shared__ int srel[XADD_REL];
   const FTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int sn = t.sn;
   const int ldp = S.nrow[sn];
@@ -220,28 +224,38 @@ __global__ __launch_bounds__(256) void k_extend_add(const FTask* __restrict__ ta
     const int nc = S.ncol[c], ldc = S.nrow[c], nu = ldc - nc;
     const int* rel = S.rel + S.relptr[c];
     const T* C = pool + S.foff[c] + (int64_t)nc * ldc + nc;
-    for (int jj = a + w; jj < b; jj += 4) {
-      const int pc = rel[jj];
-      T* dst = P + (int64_t)pc * ldp;
-      const T* src = C + (int64_t)jj * ldc;
-      // the rows of one child column land in distinct parent rows, so eight
-      // iterations' loads go out before their stores
-      for (int i0 = jj + lane; i0 < nu; i0 += 64 * 8) {
-        int pr[8];
-        T v[8], o[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-          const int ii = i0 + 64 * u;
-          pr[u] = ii < nu ? rel[ii] : -1;
-          v[u] = ii < nu ? src[ii] : (T)0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) o[u] = pr[u] >= 0 ? dst[pr[u]] : (T)0;
-#pragma unroll
-        for (int u = 0; u < 8; u++)
-          if (pr[u] >= 0) dst[pr[u]] = o[u] + v[u];
-      }
+    const bool staged = nu - a <= XADD_REL;
+    if (staged) {
+      for (int i = a + (int)threadIdx.x; i < nu; i += 256) srel[i - a] = rel[i];
+      __syncthreads();
     }
+    // rl(i) = rel[i] for i >= a, from LDS or from memory
+    auto columns = [&](auto rl) {
+      for (int jj = a + w; jj < b; jj += 4) {
+        const int pc = rl(jj);
+        T* dst = P + (int64_t)pc * ldp;
+        const T* src = C + (int64_t)jj * ldc;
+        // the rows of one child column land in distinct parent rows, so
+        // eight iterations' loads go out before their stores
+        for (int i0 = jj + lane; i0 < nu; i0 += 64 * 8) {
+          int pr[8];
+          T v[8], o[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) {
+            const int ii = i0 + 64 * u;
+            pr[u] = ii < nu ? rl(ii) : -1;
+            v[u] = ii < nu ? src[ii] : (T)0;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; u++) o[u] = pr[u] >= 0 ? dst[pr[u]] : (T)0;
+#pragma unroll
+          for (int u = 0; u < 8; u++)
+            if (pr[u] >= 0) dst[pr[u]] = o[u] + v[u];
+        }
+      }
+    };
+    if (staged) columns([&](int i) { return srel[i - a]; });
+    else columns([&](int i) { return rel[i]; });
     __syncthreads();
   }
 }
