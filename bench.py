@@ -10,8 +10,8 @@ each step computes element matrices, assembles K and M, factors K with the
 Schur complement, builds B, solves the component modes by block Lanczos,
 reduces mass, stiffness and gravity and copies the reduced matrices back.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
-runs on one GPU.
+Prints one JSON line (see the keys below).  --gpus N > 1 exits: the
+reducer runs on one GPU.
 """
 import argparse
 import json
