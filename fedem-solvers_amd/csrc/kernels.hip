@@ -609,14 +609,15 @@ __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, D
 // ---------------------------------------------------------------- update
 // Task: sn, a = ti, b = tj (64x64 tiles of the trailing lower triangle),
 // k0 = panel, kb = panel width (<= 128).  F[I,J] -= (L_I D) L_J^T over the
-// panel, staged through LDS in 64-column chunks whose products accumulate
+// panel, staged through LDS in UCH-column chunks whose products accumulate
 // in the MFMA registers, so a 128-wide panel reads and writes the trailing
 // tile once for both halves.
 template <class T>
 __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks, DevSym S,
                                                 T* __restrict__ pool) {
-  __shared__ T sI[64][33];
-  __shared__ T sJ[64][33];
+  constexpr int UCH = 16, UQ = 64 * UCH / 256;  // chunk width, loads per thread per operand
+  __shared__ T sI[64][UCH + 1];
+  __shared__ T sJ[64][UCH + 1];
   const FTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn];
   const int kb = t.kb;
@@ -629,12 +630,12 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
   typename MF<T>::acc acc[2][2];
   zero_acc(acc);
-  T ri[8], rj[8], rd[8];
-  // global loads of the 32-column chunk at c0 (zero beyond the panel)
+  T ri[UQ], rj[UQ], rd[UQ];
+  // global loads of the chunk at c0 (zero beyond the panel)
   auto load = [&](int c0) {
-    const int kc = min(32, kb - c0), p0 = k0 + c0;
+    const int kc = min(UCH, kb - c0), p0 = k0 + c0;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < UQ; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
       const bool okm = m < kc;
       rd[q] = okm ? (T)S.dpiv[c0s + p0 + m] : (T)0;
@@ -643,17 +644,17 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
     }
   };
   load(0);
-  for (int c0 = 0; c0 < kb; c0 += 32) {
-    const int kc = min(32, kb - c0);
+  for (int c0 = 0; c0 < kb; c0 += UCH) {
+    const int kc = min(UCH, kb - c0);
     if (c0 > 0) __syncthreads();  // the previous chunk's products are done with the tiles
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < UQ; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
       sI[i][m] = ri[q] * rd[q];
       sJ[i][m] = rj[q];
     }
     __syncthreads();
-    if (c0 + 32 < kb) load(c0 + 32);  // the next chunk's loads are in flight during the products
+    if (c0 + UCH < kb) load(c0 + UCH);  // the next chunk's loads are in flight during the products
     mfma_tile<T>(sI, sJ, (kc + 3) & ~3, acc, lane, wr, wc);
   }
   const int lr = lane & 15, lk = lane >> 4;
