@@ -803,8 +803,9 @@ int DeviceReducer::reduce(std::string& err) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[1], st));
   // ---- assembly (CSR K, M and the K entries of the fronts)
+  // fronts start at zero (eager, outside any graph capture)
   if (d.mixed) HIPCHK(hipMemsetAsync(d.pool32.p, 0, sizeof(float) * y.fpool, st));
-  else launch_fill(st, d.pool.p, y.fpool, 0.0);
+  else HIPCHK(hipMemsetAsync(d.pool.p, 0, sizeof(double) * y.fpool, st));
   launch_assemble(st, nnz, d.cptr.p, d.cidx.p, y.ccoef.empty() ? nullptr : d.ccoef.p, d.Kp.p, d.Mp.p, d.Kv.p, d.Mv.p,
                   d.fmap.p, d.mixed ? nullptr : d.pool.p);
   if (d.mixed) {  // equilibrated stiffness into the FP32 fronts
