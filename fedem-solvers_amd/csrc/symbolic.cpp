@@ -412,32 +412,48 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
       if (coef) y.ccoef.resize(ccount[nnz]);
       fill.assign(ccount.begin(), ccount.end() - 1);
     }
+    // column offset of node q in node pn's CSR row, for the last pair asked
+    // (the DOF and term pairs of one node pair repeat it)
+    int last_p = -1, last_q = -1, last_co = 0;
+    auto col_of = [&](int pn, int qn) -> int {
+      if (pn != last_p || qn != last_q) {
+        last_p = pn;
+        last_q = qn;
+        last_co = coloff[find_nb(pn, qn)];
+      }
+      return last_co;
+    };
     for (int e = 0; e < p.nel; e++) {
       int t = p.etype[e], d = elem_ndn(t), nn = elem_nodes(t);
       const int* en = &p.enodes[p.eptr[e]];
-      // equation terms of the element's local DOFs
+      // equation terms of the element's local DOFs, each with the CSR start
+      // of its row and its rank within its node
       Term tl[8 * 6][3];
-      int nt[8 * 6];
+      int nt[8 * 6], trk[8 * 6][3];
+      int64_t trow[8 * 6][3];
       for (int a = 0; a < nn; a++)
-        for (int i = 0; i < d; i++)
-          nt[a * d + i] = i < s.nndof[en[a]] ? dof_terms(en[a], i, tl[a * d + i]) : 0;
+        for (int i = 0; i < d; i++) {
+          const int li = a * d + i;
+          nt[li] = i < s.nndof[en[a]] ? dof_terms(en[a], i, tl[li]) : 0;
+          for (int ti = 0; ti < nt[li]; ti++) {
+            const Term& T = tl[li][ti];
+            trk[li][ti] = drank[s.madof[T.node] - 1 + T.j];
+            trow[li][ti] = y.rowptr[nfirst[T.node] + trk[li][ti]];
+          }
+        }
       for (int a = 0; a < nn; a++)
         for (int b = 0; b < nn; b++) {
           int blk = a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a;
           for (int i = 0; i < d; i++)
             for (int j = 0; j < d; j++) {
               int off = a >= b ? (blk * d * d + i * d + j) : (blk * d * d + j * d + i);
-              for (int ti = 0; ti < nt[a * d + i]; ti++) {
-                const Term& P = tl[a * d + i][ti];
-                int r = nfirst[P.node] + drank[s.madof[P.node] - 1 + P.j];
-                int qn = -1, co = 0;
-                for (int tj = 0; tj < nt[b * d + j]; tj++) {
-                  const Term& Q = tl[b * d + j][tj];
-                  if (qn < 0 || nbl[qn] != Q.node) {
-                    qn = find_nb(P.node, Q.node);
-                    co = coloff[qn];
-                  }
-                  int64_t k = y.rowptr[r] + co + drank[s.madof[Q.node] - 1 + Q.j];
+              const int li = a * d + i, lj = b * d + j;
+              for (int ti = 0; ti < nt[li]; ti++) {
+                const Term& P = tl[li][ti];
+                for (int tj = 0; tj < nt[lj]; tj++) {
+                  const Term& Q = tl[lj][tj];
+                  const int co = col_of(P.node, Q.node);
+                  int64_t k = trow[li][ti] + co + trk[lj][tj];
                   if (pass == 0) { ccount[k + 1]++; continue; }
                   int64_t slot = fill[k]++;
                   y.cidx[slot] = y.elem_off[e] + off;
