@@ -467,16 +467,22 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
   // lower CSR entries -> front positions
   y.fmap.assign(nnz, -1);
   for (int r = 0; r < s.neq; r++) {
+    int last_sn = -1;  // consecutive columns of a row mostly share a front
+    int64_t rpos = 0;
     for (int64_t k = y.rowptr[r]; k < y.rowptr[r + 1]; k++) {
       int c = y.colind[k];
       if (c > r) continue;
       int sn = colsn[c];
-      const int* rb = &y.rows[y.rptr[sn]];
-      const int* re = rb + y.nrow[sn];
-      const int* it = std::lower_bound(rb, re, r);
-      if (it == re || *it != r) { err = "symbolic: matrix entry outside its front"; return -1; }
+      if (sn != last_sn) {
+        const int* rb = &y.rows[y.rptr[sn]];
+        const int* re = rb + y.nrow[sn];
+        const int* it = std::lower_bound(rb, re, r);
+        if (it == re || *it != r) { err = "symbolic: matrix entry outside its front"; return -1; }
+        last_sn = sn;
+        rpos = (int64_t)(it - rb);
+      }
       int64_t ld = y.nrow[sn];
-      y.fmap[k] = y.foff[sn] + (int64_t)(it - rb) + (int64_t)(c - y.col0[sn]) * ld;
+      y.fmap[k] = y.foff[sn] + rpos + (int64_t)(c - y.col0[sn]) * ld;
     }
   }
   (void)roots;
