@@ -133,6 +133,7 @@ int fgpu_set_option(fgpu_reducer* h, const char* name, double value) {
   else if (n == "timeUpdates") o.time_updates = value != 0.0;
   else if (n == "mixedPrecision") o.mixed = value != 0.0;
   else if (n == "solveSplitK") o.solve_split = (int)value;
+  else if (n == "factorLookahead") o.factor_lookahead = value != 0.0;
   else if (n == "factorPanel") o.factor_panel = (int)value;
   else if (n == "refineTol") o.ir_tol = value;
   else if (n == "refineMaxSteps") o.ir_maxit = (int)value;

@@ -82,6 +82,7 @@ struct HBuf {
 
 enum OpKind {
   EXTADD, DIAG, TRSM, DIAG_TRSM, UPDATE, INV_OFF, INV_PART, INV_FIN,
+  UPDATE_SIDE, FORK, JOIN,  // lookahead: trailing tiles on the side stream, fork/join with the main one
   FW_INIT, FW_INV, FW_UPD, FW_OUT, BW_INIT, BW_UPD, BW_INV,
   FW_INV_R, FW_UPD_R, BW_UPD_R, BW_INV_R  // ordered reductions of split-K partials
 };
@@ -177,6 +178,8 @@ struct DeviceReducer::Impl {
   HBuf<double> hC, hR;  // two slots each (step parity)
   HBuf<int> hbad;
   hipEvent_t lz_ev[2]{};
+  hipStream_t side = nullptr;  // lookahead stream of the factorisation
+  hipEvent_t fj[2]{};          // its fork and join events
   std::vector<int> hexl, shl, bml;  // elements by type: HEX8, ANDES4, BEAM2
   int64_t wrows = 0;
   hipEvent_t ev[12]{};
@@ -192,12 +195,19 @@ DeviceReducer::~DeviceReducer() {
     for (auto& e : d_->upd_ev) (void)hipEventDestroy(e);
     for (auto& e : d_->lz_ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : d_->fj)
+      if (e) (void)hipEventDestroy(e);
+    if (d_->side) {
+      (void)hipStreamSynchronize(d_->side);
+      (void)hipStreamDestroy(d_->side);
+    }
     (void)hipStreamDestroy(st_);
   }
 }
 
 static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, std::vector<Op>& ops,
-                                  std::vector<int>& xrange, double& flops, int panel, int& inv_slots) {
+                                  std::vector<int>& xrange, double& flops, int panel, int& inv_slots,
+                                  bool lookahead) {
   xrange.clear();
   inv_slots = 0;
   flops = 0;
@@ -273,37 +283,79 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
       }
       if ((int)tasks.size() > off) ops.push_back({TRSM, off, (int)tasks.size() - off});
     };
-    // 64x64 lower tiles of the trailing block from k0 + kb, tile columns < ntc (all when ntc < 0)
-    auto add_update = [&](int s, int k0, int kb, int ntc) {
+    // 64x64 lower tiles of the trailing block from k0 + kb, tile columns in
+    // [tlo, thi) (thi < 0: all), appended to `out`
+    auto add_update_to = [&](std::vector<FTask>& out, int s, int k0, int kb, int tlo, int thi) {
       const int rem = y.nrow[s] - k0 - kb;
       if (rem <= 0) return;
       const int T = (rem + 63) / 64;
       for (int ti = 0; ti < T; ti++)
-        for (int tj = 0; tj <= ti && (ntc < 0 || tj < ntc); tj++) tasks.push_back({s, ti, tj, k0, kb});
-      // lower-triangle entries (incl. diagonal) of the updated columns, 2 flops per FMA
-      const double w = ntc < 0 ? rem : std::min(rem, 64 * ntc);
-      flops += 2.0 * kb * (w * (w + 1) / 2 + (rem - w) * w);
+        for (int tj = tlo; tj <= ti && (thi < 0 || tj < thi); tj++) out.push_back({s, ti, tj, k0, kb});
+      // lower-triangle entries (incl. diagonal) of the updated columns c in
+      // [clo, chi), rem - c each, 2 flops per FMA
+      const double clo = std::min(rem, 64 * tlo), chi = thi < 0 ? rem : std::min(rem, 64 * thi);
+      if (chi > clo) flops += 2.0 * kb * ((chi - clo) * rem - (chi * (chi - 1) - clo * (clo - 1)) / 2);
     };
-    const int P = panel;
+    auto add_update = [&](int s, int k0, int kb, int ntc) { add_update_to(tasks, s, k0, kb, 0, ntc); };
+    const int P = panel, NA = P / 64;
+    // Lookahead on the levels of few large fronts (those that take the fused
+    // diagonal/TRSM launch): the trailing update after a panel is split into
+    // the tiles of the next panel's columns (main stream) and the rest (side
+    // stream), so the next panel's diagonal blocks, TRSM and narrow update
+    // run while the rest of the trailing block is updated.  Any update that
+    // reaches past the next panel joins the side stream first, and so does
+    // the end of the level.
+    int ntr0 = 0;
+    for (int q = 0; q < ns; q++) {
+      int s = sups[q];
+      if (y.ncol[s] > 0) ntr0 += (y.nrow[s] - std::min(64, y.ncol[s]) + 63) / 64;
+    }
+    const bool look = lookahead && ntr0 <= FUSE_TRSM;
+    bool side_pending = false;
     for (int k0 = 0; k0 < maxnc; k0 += P) {
       for (int h = 0; h < P / 64 && k0 + 64 * h < maxnc; h++) {
         const int kh = k0 + 64 * h;
         add_diag_trsm(kh);
         const int off = (int)tasks.size();
+        std::vector<FTask> side;
+        bool past = false;  // some task updates columns past the next panel
         for (int q = 0; q < ns; q++) {
           int s = sups[q];
           const int rc = y.ncol[s] - k0;
           if (rc <= 64 * h) continue;
           if (rc >= P) {
-            if (h + 1 < P / 64) add_update(s, kh, 64, P / 64 - h - 1);  // the panel's remaining columns
-            else add_update(s, k0, P, -1);                              // the trailing block, whole panel
+            if (h + 1 < P / 64) {
+              add_update(s, kh, 64, P / 64 - h - 1);  // the panel's remaining columns
+            } else if (look) {                        // the trailing block, whole panel: split
+              add_update_to(tasks, s, k0, P, 0, NA);
+              add_update_to(side, s, k0, P, NA, -1);
+              past = true;
+            } else {
+              add_update(s, k0, P, -1);
+              past = true;
+            }
           } else {
             add_update(s, kh, std::min(64, rc - 64 * h), -1);
+            past = true;
           }
         }
-        if ((int)tasks.size() > off) ops.push_back({UPDATE, off, (int)tasks.size() - off});
+        if ((int)tasks.size() > off) {
+          if (past && side_pending) {
+            ops.push_back({JOIN, 0, 0});
+            side_pending = false;
+          }
+          ops.push_back({UPDATE, off, (int)tasks.size() - off});
+        }
+        if (!side.empty()) {
+          const int soff = (int)tasks.size();
+          tasks.insert(tasks.end(), side.begin(), side.end());
+          ops.push_back({FORK, 0, 0});
+          ops.push_back({UPDATE_SIDE, soff, (int)side.size()});
+          side_pending = true;
+        }
       }
     }
+    if (side_pending) ops.push_back({JOIN, 0, 0});
   }
   // Explicit inverse of every pivot block: the diagonal tiles come from
   // k_diag, the tiles at growing distance from the diagonal follow.  Only the
@@ -496,7 +548,9 @@ int DeviceReducer::upload(std::string& err) {
   Impl& d = *d_;
   if (!st_) {
     HIPCHK(hipStreamCreate(&st_));
+    HIPCHK(hipStreamCreate(&d.side));
     for (auto& e : d.ev) HIPCHK(hipEventCreate(&e));
+    for (auto& e : d.fj) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   HIPCHK(d.xyz.upload(part_.xyz));
   HIPCHK(d.eprop.upload(part_.eprop));
@@ -553,13 +607,13 @@ int DeviceReducer::upload(std::string& err) {
   std::vector<int> xr;
   int inv_slots = 0;
   build_factor_schedule(y, ft, d.fops, xr, upd_flops_, std::min(256, std::max(64, (opt_.factor_panel / 64) * 64)),
-                        inv_slots);
+                        inv_slots, opt_.factor_lookahead);
   HIPCHK(d.ftasks.upload(ft));
   HIPCHK(d.xrange.upload(xr));
   HIPCHK(d.ipart.alloc((size_t)std::max(1, inv_slots) * 4096));
   upd_launches_ = 0;
   for (auto& op : d.fops)
-    if (op.kind == UPDATE) upd_launches_++;
+    if (op.kind == UPDATE || op.kind == UPDATE_SIDE) upd_launches_++;
   d.plans.clear();
   // a new structure: every captured launch sequence is stale
   d.g_factor.reset();
@@ -824,6 +878,10 @@ int DeviceReducer::reduce(std::string& err) {
     for (auto& e : d.upd_ev) HIPCHK(hipEventCreate(&e));
   }
   int64_t ui = 0;
+  hipError_t fork_err = hipSuccess;  // first failure of a fork or join of the side stream
+  auto note = [&](hipError_t e) {
+    if (e != hipSuccess && fork_err == hipSuccess) fork_err = e;
+  };
   auto factor_with = [&](auto* pool, auto* ipool) {
     for (const Op& op : d.fops) {
       switch (op.kind) {
@@ -832,10 +890,26 @@ int DeviceReducer::reduce(std::string& err) {
         case TRSM: launch_trsm(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool); break;
         case DIAG_TRSM: launch_diag_trsm(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool, d.piv.p); break;
         case UPDATE:
+        case UPDATE_SIDE: {
+          // with per-launch timing everything stays on the main stream
+          hipStream_t us = (op.kind == UPDATE_SIDE && !opt_.time_updates) ? d.side : st;
           if (opt_.time_updates) (void)hipEventRecord(d.upd_ev[2 * ui], st);
-          launch_update(st, d.ftasks.p + op.off, op.n, d.S, pool);
+          launch_update(us, d.ftasks.p + op.off, op.n, d.S, pool);
           if (opt_.time_updates) (void)hipEventRecord(d.upd_ev[2 * ui + 1], st);
           ui++;
+          break;
+        }
+        case FORK:
+          if (!opt_.time_updates) {
+            note(hipEventRecord(d.fj[0], st));
+            note(hipStreamWaitEvent(d.side, d.fj[0], 0));
+          }
+          break;
+        case JOIN:
+          if (!opt_.time_updates) {
+            note(hipEventRecord(d.fj[1], d.side));
+            note(hipStreamWaitEvent(st, d.fj[1], 0));
+          }
           break;
         case INV_OFF: launch_inv_off(st, d.ftasks.p + op.off, op.n, d.S, pool, d.ioff.p, ipool); break;
         case INV_PART:
@@ -862,6 +936,7 @@ int DeviceReducer::reduce(std::string& err) {
                       d.piv.p, d.S.foff, d.S.rows, d.S.dpiv},
                      factor_ops));
   }
+  HIPCHK(fork_err);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[3], st));
   // ---- B = -K_ii^{-1} K_ib: back-substitution of [0; I].  With the FP64
@@ -1537,13 +1612,13 @@ int DeviceReducer::autotune(int& panel, int& split, std::string& err) {
     std::vector<int> xr;
     d.fops.clear();
     int inv_slots = 0;
-    build_factor_schedule(sym_, ft, d.fops, xr, upd_flops_, P, inv_slots);
+    build_factor_schedule(sym_, ft, d.fops, xr, upd_flops_, P, inv_slots, opt_.factor_lookahead);
     HIPCHK(d.ipart.alloc((size_t)std::max(1, inv_slots) * 4096));
     HIPCHK(d.ftasks.upload(ft));
     HIPCHK(d.xrange.upload(xr));
     upd_launches_ = 0;
     for (auto& op : d.fops)
-      if (op.kind == UPDATE) upd_launches_++;
+      if (op.kind == UPDATE || op.kind == UPDATE_SIDE) upd_launches_++;
     d.g_factor.reset();
     opt_.factor_panel = P;
     return 0;

@@ -24,6 +24,10 @@ struct ReduceOptions {
   // K_ii refined against the FP64 matrix until the correction falls below
   // ir_tol relative (at most ir_maxit steps; no convergence is an error).
   int factor_panel = 128;   // pivot columns per trailing update (64..256, multiple of 64)
+  // top levels: trailing tiles past the next panel on a side stream (off by
+  // default: on the 48x24x24 block the side updates take the CUs that the
+  // critical-path diagonal launches need, 12.8 -> 13.1 ms)
+  bool factor_lookahead = false;
   int solve_split = 128;    // longest K range of one solve GEMM task (longer ones are split and reduced)
   bool mixed = false;
   double ir_tol = 1e-14;

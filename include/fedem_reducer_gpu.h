@@ -97,6 +97,9 @@ int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags);
  * "factorPanel" (pivot columns per trailing update: 64, 128, 192 or 256,
  * default 128), "solveSplitK" (longest contraction range of one solve task, a multiple
  * of 64, default 128; longer ranges are split and reduced in order),
+ * "factorLookahead" (1: on the levels of few large fronts, the trailing
+ * tiles past the next panel are updated on a second stream while the next
+ * panel is factored; default 0),
  * "mixedPrecision" (1: FP32 factor of the equilibrated K with FP32 MFMA
  * trailing and Schur updates and FP64 diagonal blocks; B and every K_ii
  * solve refined against the FP64 matrix), "refineTol" (relative correction

@@ -314,6 +314,24 @@ def test_small_part_modes_span_whole_internal_space():
     assert np.max(np.abs(lam - o["lam"]) / o["lam"]) <= 1e-9
 
 
+def test_factor_lookahead_same_bits():
+    """The lookahead schedule (trailing tiles past the next panel updated on a
+    second stream, forked and joined inside the captured factor graph) gives
+    the bits of the single-stream schedule."""
+    part = parts.hex_block(14, 12, 10, 0.14, 0.12, 0.10, seed=4, n_ext=6)
+    out = []
+    for la in (0, 1):
+        R = gpu.GpuReducer()
+        R.set_part(part)
+        R.set_option("factorLookahead", la)
+        R.analyze(ngen=4)
+        R.reduce()
+        R.reduce()  # the second run replays the captured graph
+        out.append(R.reduced())
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("panel", [64, 192, 256])
 def test_factor_panel_widths_match_oracle(panel):
     """Panels of 64 to 256 pivot columns per trailing update (sub-panel
