@@ -543,14 +543,25 @@ int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& e
   return 0;
 }
 
+// The lookahead side stream and its events, created only for a schedule that
+// forks (an idle extra stream per reducer would share the hardware queues
+// with the streams of other reducers).
+static hipError_t ensure_side(DeviceReducer::Impl& d) {
+  bool fork = false;
+  for (const Op& op : d.fops) fork = fork || op.kind == FORK;
+  if (!fork || d.side) return hipSuccess;
+  hipError_t e = hipStreamCreate(&d.side);
+  for (auto& ev : d.fj)
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  return e;
+}
+
 // Device upload of the part and of every structure array the kernels read.
 int DeviceReducer::upload(std::string& err) {
   Impl& d = *d_;
   if (!st_) {
     HIPCHK(hipStreamCreate(&st_));
-    HIPCHK(hipStreamCreate(&d.side));
     for (auto& e : d.ev) HIPCHK(hipEventCreate(&e));
-    for (auto& e : d.fj) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   HIPCHK(d.xyz.upload(part_.xyz));
   HIPCHK(d.eprop.upload(part_.eprop));
@@ -608,6 +619,7 @@ int DeviceReducer::upload(std::string& err) {
   int inv_slots = 0;
   build_factor_schedule(y, ft, d.fops, xr, upd_flops_, std::min(256, std::max(64, (opt_.factor_panel / 64) * 64)),
                         inv_slots, opt_.factor_lookahead);
+  HIPCHK(ensure_side(d));
   HIPCHK(d.ftasks.upload(ft));
   HIPCHK(d.xrange.upload(xr));
   HIPCHK(d.ipart.alloc((size_t)std::max(1, inv_slots) * 4096));
@@ -1613,6 +1625,7 @@ int DeviceReducer::autotune(int& panel, int& split, std::string& err) {
     d.fops.clear();
     int inv_slots = 0;
     build_factor_schedule(sym_, ft, d.fops, xr, upd_flops_, P, inv_slots, opt_.factor_lookahead);
+    HIPCHK(ensure_side(d));
     HIPCHK(d.ipart.alloc((size_t)std::max(1, inv_slots) * 4096));
     HIPCHK(d.ftasks.upload(ft));
     HIPCHK(d.xrange.upload(xr));
