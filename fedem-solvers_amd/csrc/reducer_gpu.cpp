@@ -397,7 +397,7 @@ static void build_factor_schedule(const Symbolic& y, std::vector<FTask>& tasks, 
 
 static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<SolveTask>& tasks,
                              std::vector<int>& gptr, std::vector<int>& gsrc, std::vector<int> (&ugrp)[2],
-                             std::vector<int>& ugroups, int KS) {
+                             std::vector<int>& ugroups, int KSplit) {
   const int nrhs = sp.nrhs, schur = y.nsup - 1;
   // gather lists of the forward init: work row woff[s] + i of front s adds
   // the update rows of s's children that land in its row i, child after
@@ -442,6 +442,14 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
   int pass = 0;  // forward pass being built (0: internal fronts only, 1: all)
   auto add_tiles = [&](std::vector<Op>& ops, int kind, const std::vector<int>& list, bool upd_rows) {
     const int mode = kind == FW_INV ? 0 : kind == FW_UPD ? 1 : kind == BW_UPD ? 2 : 3;
+    // an op with enough unsplit tasks to fill the device is not split
+    int64_t ntask = 0;
+    for (int s : list) {
+      const int nc = y.ncol[s], nu = y.nrow[s] - nc;
+      if (kind == BW_UPD && nu == 0) continue;
+      ntask += (int64_t)((upd_rows ? nu : nc) + 63) / 64 * ((nrhs + 63) / 64);
+    }
+    const int KS = ntask >= 1024 ? INT_MAX : KSplit;
     const int rkind = kind == FW_INV ? FW_INV_R : kind == FW_UPD ? FW_UPD_R : kind == BW_UPD ? BW_UPD_R : BW_INV_R;
     std::vector<SolveTask> red;
     int off = (int)tasks.size();
