@@ -21,6 +21,7 @@
 
 namespace fgpu {
 int write_fmx(const std::string& file, const std::string& dtype, unsigned checksum, const double* data, size_t n);
+int write_tag(FILE* f, const std::string& tag, unsigned cs);
 }
 
 extern "C" int fgpu_reducer_run(int* ierr);                             // reducer_driver.f90
@@ -333,7 +334,8 @@ int fgpu_write_fmx(const char* file, const char* dtype, int checksum, const doub
   return fgpu::write_fmx(cstr(file), cstr(dtype), (unsigned)checksum, data, (size_t)n);
 }
 
-// _SAM.fsm (saveSAM, samReducerModule.f90:586-660): the "#SAM data" tag,
+// _SAM.fsm (saveSAM, samReducerModule.f90:586-660): the "#SAM data" tag
+// (the 40-byte file tag of write_tag) and the ";1.0;\n" version field,
 // then int32 records: size(mpar), mpar(50), madof, minex, mnnn, msc, mpmnpc,
 // mmnpc, melcon, meqn, meqn1, meqn2, dofPosIn2.  MPAR entries follow
 // samModule.f90:267-280 and samReducerModule.f90:516 / inputReducerModule
@@ -368,10 +370,11 @@ int fgpu_write_sam(fgpu_reducer* h, const char* file, int checksum, int link_id,
   for (int i = 0; i < nnod; i++) mnnn[i] = p.status[i] == 1 ? 2 : 1;
   for (int e = 0; e <= nel; e++) mpmnpc[e] = p.eptr[e] + 1;
   for (size_t k = 0; k < mmnpc.size(); k++) mmnpc[k] = p.enodes[k] + 1;
-  std::string head = "#SAM data;" + std::to_string((unsigned)checksum) + ";1.0;\n";
   FILE* f = std::fopen(file, "wb");
   if (!f) return -2;
-  bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
+  static const char ver[] = ";1.0;\n";
+  bool ok = fgpu::write_tag(f, "#SAM data", (unsigned)checksum) == 0;
+  ok = ok && std::fwrite(ver, 1, sizeof ver - 1, f) == sizeof ver - 1;
   auto put = [&](const std::vector<int>& v) {
     if (ok && !v.empty()) ok = std::fwrite(v.data(), sizeof(int), v.size(), f) == v.size();
   };

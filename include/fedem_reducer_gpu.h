@@ -257,9 +257,19 @@ int fgpu_link_loads(int* lcase, int* node, int* dof, double* val);
  * ffl_addcs_*, reducer.f90:160-172): FNV-1a over the part and options. */
 int fgpu_link_checksum(int ngen, int neval, double tol_eig, double tol_fac);
 
-/* writeDoubleDB (binaryDBInterface.f90:452-500): "#FEDEM <dtype>" tag with
- * the checksum, then n column-major doubles. */
+/* writeDoubleDBfile (binaryDBInterface.f90:452-500): the 40-byte file tag
+ * "#FEDEM <dtype>" with the checksum (FFa_writeTag layout: 30 blank-padded
+ * characters, int32 endian marker 0x1234 and two zero bytes, uint32
+ * checksum), the ";1.0;\n" version field, then n column-major doubles. */
 int fgpu_write_fmx(const char* file, const char* dtype, int checksum, const double* data, int64_t n);
+/* writetagdb (binaryDB.c:643-674) as a file service: a new file holding the
+ * file tag (tag of at most 30 characters) and the given version field. */
+int fgpu_write_tag(const char* file, const char* tag, unsigned checksum, const char* version);
+/* readtagdb (binaryDB.c:676-733): tag text (trailing blanks removed),
+ * checksum (byte-swapped when the endian marker is the other order) and
+ * version line of a tagged binary file; returns the offset of the data
+ * after the version line, or < 0 (-3: not a tagged file, -4: short). */
+int64_t fgpu_read_tag(const char* file, char* tag, int tag_len, unsigned* checksum, char* version, int ver_len);
 /* saveSAM (samReducerModule.f90:586-660): the _SAM.fsm file. */
 int fgpu_write_sam(fgpu_reducer* h, const char* file, int checksum, int link_id, int neval, int ngen);
 /* Last error text of a handle into a blank-padded Fortran buffer. */

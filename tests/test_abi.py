@@ -9,7 +9,7 @@ import struct
 import numpy as np
 import pytest
 
-from fedempy_amd import LIB_PATH, parts
+from fedempy_amd import LIB_PATH, fmx, parts
 from fedempy_amd.reducer import FedemReducer
 from oracle import nastran
 from oracle import reducer as orc
@@ -151,15 +151,63 @@ def test_nastran_reader_rejects_bad_input(lib, tmp_path):
     assert b"missing property" in buf.raw
 
 
+MOTIONS = os.path.join(os.path.dirname(__file__), "golden", "motions.fnd")
+
+
+def test_file_tag_matches_reference_data_file(lib, tmp_path):
+    """The reference's motions.fnd (createTestFnd.C:38-40: FFaTag::write of
+    "#FEDEM nodal displacements" with checksum 12345678, then ";7.3;") pins
+    the tag bytes: the tag writer reproduces its first 45 bytes exactly."""
+    ref = open(MOTIONS, "rb").read()
+    f = tmp_path / "motions.fnd"
+    assert lib.fgpu_write_tag(str(f).encode(), b"#FEDEM nodal displacements", ctypes.c_uint(12345678), b";7.3;") == 0
+    assert f.read_bytes() == ref[:45]
+    # a planted change of the checksum or of the padding breaks the match
+    assert lib.fgpu_write_tag(str(f).encode(), b"#FEDEM nodal displacements", ctypes.c_uint(12345679), b";7.3;") == 0
+    assert f.read_bytes() != ref[:45]
+    assert lib.fgpu_write_tag(str(f).encode(), b"#FEDEM nodal displacements ", ctypes.c_uint(12345678), b";7.3;") == 0
+    assert f.read_bytes() == ref[:45]  # trailing blanks are padding
+    assert lib.fgpu_write_tag(str(f).encode(), b"x" * 31, ctypes.c_uint(0), b";1.0;\n") < 0
+
+
+def test_tag_reader_parses_reference_data_file(lib):
+    """readtagdb semantics on motions.fnd, then the file's own records
+    (createTestFnd.C:60-84): 5 node numbers, 3 steps of (step, time, 15
+    displacements 1..15, raised by 100 per step)."""
+    tag = ctypes.create_string_buffer(64)
+    ver = ctypes.create_string_buffer(64)
+    cs = ctypes.c_uint(0)
+    lib.fgpu_read_tag.restype = ctypes.c_int64
+    off = lib.fgpu_read_tag(MOTIONS.encode(), tag, 64, ctypes.byref(cs), ver, 64)
+    assert off == 48
+    assert tag.value == b"#FEDEM nodal displacements" and cs.value == 12345678 and ver.value == b";7.3;5;\n"
+    raw = open(MOTIONS, "rb").read()
+    t, c, v, o = fmx.read_tag(raw)
+    assert (t, c, v, o) == ("#FEDEM nodal displacements", 12345678, ";7.3;5;\n", 48)
+    nodes = np.frombuffer(raw, "<i4", 5, o)
+    assert list(nodes) == [7, 12, 38, 42, 59]
+    p = o + 20
+    for step in range(3):
+        assert struct.unpack_from("<i", raw, p)[0] == step
+        assert abs(struct.unpack_from("<d", raw, p + 4)[0] - 0.1 * step) < 1e-15
+        assert np.array_equal(np.frombuffer(raw, "<f8", 15, p + 12), np.arange(1.0, 16.0) + 100.0 * step)
+        p += 12 + 120
+    assert p == len(raw) == 464
+    with pytest.raises(ValueError):
+        fmx.read_tag(b"#FEDEM stiffness matrix;1234;1.0;\n" + bytes(16))
+
+
 def test_fmx_writer_layout(lib, tmp_path):
     a = np.arange(12.0).reshape(3, 4, order="F")
     f = tmp_path / "x_S.fmx"
     assert lib.fgpu_write_fmx(str(f).encode(), b"stiffness matrix", 1234,
                               a.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(a.size)) == 0
     raw = f.read_bytes()
-    head, body = raw.split(b"\n", 1)
-    assert head == b"#FEDEM stiffness matrix;1234;1.0;"
-    assert np.array_equal(np.frombuffer(body, "<f8"), a.ravel(order="F"))
+    want = b"#FEDEM stiffness matrix".ljust(30) + struct.pack("<IHI", 0x1234, 0, 1234) + b";1.0;\n"
+    assert raw[:46] == want
+    b, tag, cs = fmx.read_fmx(f, (3, 4))
+    assert tag == "#FEDEM stiffness matrix" and cs == 1234
+    assert np.array_equal(b, a)
 
 
 def test_fmxwriter_fortran_entry_roundtrip(lib, tmp_path):

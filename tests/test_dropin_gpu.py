@@ -10,7 +10,7 @@ import re
 import numpy as np
 import pytest
 
-from fedempy_amd import parts
+from fedempy_amd import fmx, parts
 from fedempy_amd.reducer import FedemReducer
 from oracle import reducer as orc
 
@@ -23,10 +23,8 @@ def rel(a, b):
 
 
 def read_fmx(path, shape):
-    raw = open(path, "rb").read()
-    head, body = raw.split(b"\n", 1)
-    assert head.startswith(b"#FEDEM ")
-    return np.frombuffer(body, "<f8").reshape(shape, order="F"), head.decode()
+    a, tag, _ = fmx.read_fmx(path, shape)
+    return a, tag
 
 
 def clamped_block():
@@ -59,14 +57,14 @@ def test_reducepart_writes_reference_outputs(tmp_path):
     nd = o["SK"].shape[0]
     n1, n2 = s["ndof1"], s["ndof2"]
     SK, h = read_fmx(tmp_path / "blk_S.fmx", (nd, nd))
-    assert h.startswith("#FEDEM stiffness matrix;")
+    assert h == "#FEDEM stiffness matrix"
     assert rel(SK, o["SK"]) <= TOL
     SM, _ = read_fmx(tmp_path / "blk_M.fmx", (nd, nd))
     assert rel(SM, o["SM"]) <= TOL
     G, _ = read_fmx(tmp_path / "blk_G.fmx", (nd, 3))
     assert rel(G, o["G"]) <= TOL
     B, h = read_fmx(tmp_path / "blk_B.fmx", (n1, n2))
-    assert h.startswith("#FEDEM disk matrix;")
+    assert h == "#FEDEM disk matrix"
     assert rel(B, o["B"]) <= TOL
     V, _ = read_fmx(tmp_path / "blk_V.fmx", (n1, 3))
     assert rel(V, o["vgi"]) <= TOL
@@ -77,10 +75,7 @@ def test_reducepart_writes_reference_outputs(tmp_path):
     assert rel(E * sg, o["phi"]) <= TOL
     res = (tmp_path / "blk.res").read_text()
     assert "EIGENFREQUENCIES" in res and "failed" not in res
-    sam = (tmp_path / "blk_SAM.fsm").read_bytes()
-    head, body = sam.split(b"\n", 1)
-    assert head.startswith(b"#SAM data;")
-    ints = np.frombuffer(body, "<i4")
+    _, ints = fmx.read_sam(tmp_path / "blk_SAM.fsm")
     assert ints[0] == 50 and ints[1] == len(p["xyz"]) and ints[1 + 49] == 3
     madof = ints[51:51 + len(p["xyz"]) + 1]
     assert np.array_equal(madof, s["madof"])
@@ -117,9 +112,8 @@ def test_solvepartdis_matches_direct_solve(tmp_path):
 
 
 def _fmx(path):
-    raw = open(path, "rb").read()
-    head, body = raw.split(b"\n", 1)
-    return np.frombuffer(body, "<f8"), head.decode()
+    a, tag, _ = fmx.read_fmx(path)
+    return a, tag
 
 
 @pytest.mark.parametrize("name", ["CHEXA08_", "CQUAD04_", "CBAR02_"])
