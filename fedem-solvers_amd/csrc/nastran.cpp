@@ -1,7 +1,8 @@
 // Nastran bulk-data subset reader for the part input (-linkfile).  The
 // reference parses parts with FFl (absent fedem-foundation); this reader
 // covers the cards of the element types on the device path: GRID, CHEXA,
-// CQUAD4, CBAR, MAT1, PSHELL, PSOLID, PBAR, PBARL (TUBE), ASET/ASET1, SPC1, RBE2 and the
+// CQUAD4, CBAR, MAT1, PSHELL, PSOLID, PBAR, PBARL (TUBE), ASET/ASET1, SPC1 (both with
+// "G1 THRU G2" ranges), RBE2 and the
 // FORCE/MOMENT nodal loads (basic coordinate system only), in
 // free (comma), small (8 column) and large (16 column, '*') field formats.
 // Nodes are numbered in ascending GRID id order, as SAM numbers them.
@@ -97,6 +98,47 @@ std::vector<Card> read_cards(std::istream& in) {
   return cards;
 }
 
+bool to_int(const std::string& s, int& v) {
+  if (s.empty()) return false;
+  char* end = nullptr;
+  long x = std::strtol(s.c_str(), &end, 10);
+  if (!end || *end) return false;
+  v = (int)x;
+  return true;
+}
+
+// Grid ids of the fields f[k0..] of an SPC1 / ASET1 card: blank fields are
+// skipped, "G1 THRU G2" is the range G1..G2 (its generated ids need not
+// exist: those go to `ranged`, the listed ones to `listed`).  False with a
+// message for any other field.
+bool grid_list(const std::vector<std::string>& f, size_t k0, const std::string& card, std::vector<int>& listed,
+               std::vector<int>& ranged, std::string& err) {
+  for (size_t k = k0; k < f.size(); k++) {
+    std::string t = trim(f[k]);
+    if (t.empty()) continue;
+    std::string u = t;
+    for (char& ch : u) ch = (char)std::toupper(ch);
+    int g = 0;
+    if (u == "THRU") {
+      int a = listed.empty() ? 0 : listed.back(), b = 0;
+      const bool ok = k > k0 && !listed.empty() && k + 1 < f.size() && to_int(trim(f[k + 1]), b) && b >= a;
+      if (!ok) {
+        err = card + ": THRU needs grid ids before and after it, the first not larger";
+        return false;
+      }
+      for (int q = a + 1; q < b; q++) ranged.push_back(q);
+      listed.push_back(b);
+      k++;
+    } else if (to_int(t, g) && g > 0) {
+      listed.push_back(g);
+    } else {
+      err = card + ": field '" + t + "' is not a grid id";
+      return false;
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
@@ -111,7 +153,7 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
   std::vector<Elem> elems;
   std::map<int, Mat> mats;
   std::map<int, Prop> props;
-  std::vector<int> aset;
+  std::vector<int> aset, aset_range;  // listed ASET grids; grids inside ASET1 THRU ranges
   std::map<int, int> spc;  // grid -> fixed-DOF bitmask
   std::vector<std::pair<int, std::pair<std::string, std::vector<int>>>> rbe2;
   struct Load { int sid, grid, dof0; double v[3]; };
@@ -178,14 +220,17 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
       for (size_t k = 0; k + 1 < f.size() + 1; k += 2)
         if (I(f, k)) aset.push_back(I(f, k));
     } else if (c.name == "ASET1") {
-      for (size_t k = 1; k < f.size(); k++)
-        if (I(f, k)) aset.push_back(I(f, k));
+      std::vector<int> ranged;
+      if (!grid_list(f, 1, c.name, aset, ranged, err)) return -2;
+      aset_range.insert(aset_range.end(), ranged.begin(), ranged.end());
     } else if (c.name == "SPC1") {
       int mask = 0;
       for (char ch : f.size() > 1 ? f[1] : std::string())
         if (ch >= '1' && ch <= '6') mask |= 1 << (ch - '1');
-      for (size_t k = 2; k < f.size(); k++)
-        if (I(f, k)) spc[I(f, k)] |= mask;
+      std::vector<int> listed, ranged;
+      if (!grid_list(f, 2, c.name, listed, ranged, err)) return -2;
+      for (int g : listed) spc[g] |= mask;
+      for (int g : ranged) spc[g] |= mask;
     } else if (c.name == "FORCE" || c.name == "MOMENT") {
       if (I(f, 2) != 0) { err = c.name + ": only the basic coordinate system (CID 0) is supported"; return -2; }
       double F = D(f, 3);
@@ -213,6 +258,8 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
     if (!idx.count(g)) { err = "ASET grid " + std::to_string(g) + " not defined"; return -2; }
     p.status[idx[g]] = 1;
   }
+  for (int g : aset_range)
+    if (idx.count(g)) p.status[idx[g]] = 1;
   p.eptr.push_back(0);
   for (auto& e : elems) {
     if (!props.count(e.pid)) { err = "element " + std::to_string(e.eid) + ": missing property"; return -2; }

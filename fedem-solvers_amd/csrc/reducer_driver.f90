@@ -517,6 +517,11 @@ subroutine fgpu_reducer_run (ierr) bind(C,name='fgpu_reducer_run')
   ierr = fgpu_create(h)
   if (ierr /= 0) goto 900
   rc = fgpu_set_option(h,cs('eigenshift'),eigenShift)
+  ! Ritz residual bound of the Lanczos convergence test: the eigenvalue error
+  ! of a shift-invert Ritz pair is of the order of the squared residual, so
+  ! -tolEigval is met by the default bound 1e-11 (which the modes' own
+  ! accuracy needs); a tighter -tolEigval tightens it
+  rc = fgpu_set_option(h,cs('lanczosTol'),min(1.0e-11_dp,tolEigval))
   if (fgpu_opt_int(cs('lanczosBlock')) > 0) &
        rc = fgpu_set_option(h,cs('lanczosBlock'),real(fgpu_opt_int(cs('lanczosBlock')),dp))
   if (fgpu_opt_int(cs('lanczosMaxVectors')) > 0) &

@@ -11,6 +11,7 @@
 //   dispfile              vgi = K_ii^{-1} g_i                 (cmstrs.f90:674-712)
 #include "reducer_gpu.hpp"
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -32,24 +33,41 @@ namespace fgpu {
     }                                                                                      \
   } while (0)
 
+// Device bytes held by the buffers that the calling thread allocates (the
+// server gives every slot's worker thread its own counter).
+static thread_local std::atomic<int64_t>* tl_account = nullptr;
+void set_alloc_account(std::atomic<int64_t>* a) { tl_account = a; }
+
 template <class T>
 struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+  std::atomic<int64_t>* acct = nullptr;  // counter charged with this buffer
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      if (acct) *acct -= (int64_t)(std::max<size_t>(1, n) * sizeof(T));
+    }
     p = nullptr;
     n = 0;
+    acct = nullptr;
   }
   hipError_t alloc(size_t count) {
     if (count <= n && p) return hipSuccess;
     release();
+    hipError_t e = hipMalloc((void**)&p, std::max<size_t>(1, count) * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
     n = count;
-    return hipMalloc((void**)&p, std::max<size_t>(1, count) * sizeof(T));
+    acct = tl_account;
+    if (acct) *acct += (int64_t)(std::max<size_t>(1, n) * sizeof(T));
+    return hipSuccess;
   }
   hipError_t upload(const std::vector<T>& v) {
     hipError_t e = alloc(v.size());
@@ -534,6 +552,15 @@ int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& e
   part_ = p;
   opt_ = o;
   if (build_sam(part_, sam_, err)) return -1;
+  // inputReducerModule.f90:403-409: NGEN may not exceed NDOF1; the number of
+  // eigenpairs is at most NDOF1
+  if (opt_.ngen > sam_.ndof1) {
+    err = "The number of generalized DOFs (NGEN) is larger than the number of internal nodal DOFs (NDOF1): "
+          "NGEN = " + std::to_string(opt_.ngen) + ", NDOF1 = " + std::to_string(sam_.ndof1) +
+          ". Reduce NGEN to less than NDOF1.";
+    return -1;
+  }
+  opt_.neval = std::min(opt_.neval, sam_.ndof1);
   if (build_symbolic(part_, sam_, opt_.sym, sym_, err)) return -1;
   d_->hexl.clear();
   d_->shl.clear();
@@ -1052,13 +1079,13 @@ int DeviceReducer::reduce(std::string& err) {
     int rc = 0;
     int bs = std::max(1, std::min(std::min(opt_.lanczos_block, 64), n1));  // block Gram matrices <= 64 x 64
     int mcap = opt_.lanczos_maxvec > 0 ? opt_.lanczos_maxvec : std::max(10 * nev + 40, 16 * bs);
+    // at least nev vectors (nev <= n1), in whole blocks
+    mcap = (std::max(mcap, nev) + bs - 1) / bs * bs;
     if (mcap >= n1) {
       // the basis may span the whole internal space: a block size that
       // divides n1 lets it do so exactly (then the Ritz pairs are exact)
       while (n1 % bs) bs--;
       mcap = n1;
-    } else {
-      mcap = std::max(bs, (mcap / bs) * bs);
     }
     SolvePlan* pl = nullptr;
     if (get_plan(y, d, bs, pl, err)) return -3;

@@ -1,12 +1,17 @@
 // Device-side part reduction (Guyan + fixed-interface component modes).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <map>
 #include <memory>
 #include "fgpu.hpp"
 #include "kern_common.hpp"
 
 namespace fgpu {
+
+// Device buffers that reducers allocate on the calling thread from now on
+// are counted in *a (bytes held; nullptr: not counted).
+void set_alloc_account(std::atomic<int64_t>* a);
 
 struct ReduceOptions {
   int ngen = 0;            // component modes kept in the reduced matrices

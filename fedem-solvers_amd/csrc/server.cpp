@@ -11,6 +11,8 @@
 // Every kernel of a reduction is deterministic and a reduction touches only
 // its own slot's buffers and stream, so a request's results are the same bit
 // for bit whether it runs alone or next to others.
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <condition_variable>
@@ -19,6 +21,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <stdexcept>
 #include <thread>
 #include "../../include/fedem_reducer_gpu.h"
 #include "reducer_gpu.hpp"
@@ -34,9 +37,9 @@ struct Request {
   int nlc = 0;
   std::vector<int> lptr, lnode, ldof;
   std::vector<double> lval, ub;
-  // results
+  // results (written by the worker before `done` is set)
   int status = 0;
-  bool done = false;
+  std::atomic<bool> done{false};
   std::string err;
   Results res;
   std::vector<double> fred, ui;
@@ -50,9 +53,19 @@ double ms(Clock::time_point a, Clock::time_point b) {
 
 }  // namespace
 
+// Device memory accounting of the admission control.  Every slot's reducer
+// keeps its buffers between requests (they only grow), so a slot holds
+// `bytes` of device memory whether it is running or idle; a running slot
+// holds at least its request's estimated footprint.  A request is admitted
+// when the memory of all slots, with its own slot at max(held, footprint),
+// fits the budget; if nothing else runs it is admitted after the idle slots
+// have released their buffers (a request larger than the whole budget then
+// runs alone).  A slot that holds more than its share of the budget
+// (budget / slots) after a request releases its buffers.
 class Server {
  public:
-  Server(int slots, size_t budget) : budget_(budget) {
+  Server(int slots, size_t budget) : budget_(budget), state_(slots, IDLE), fp_(slots, 0), bytes_(slots) {
+    for (auto& b : bytes_) b = 0;
     for (int k = 0; k < slots; k++) reducers_.emplace_back(new DeviceReducer());
     for (int k = 0; k < slots; k++) threads_.emplace_back(&Server::worker, this, k);
   }
@@ -79,23 +92,58 @@ class Server {
     auto it = reqs_.find(id);
     if (it == reqs_.end()) return nullptr;
     Request* r = it->second.get();
-    cv_done_.wait(g, [&] { return r->done; });
+    cv_done_.wait(g, [&] { return r->done.load(); });
     return r;
   }
-  Request* find(int id) {
+  // a finished request, or nullptr (unknown id or still running)
+  Request* find_done(int id) {
     std::lock_guard<std::mutex> g(mu_);
     auto it = reqs_.find(id);
-    return it == reqs_.end() ? nullptr : it->second.get();
+    return it == reqs_.end() || !it->second->done.load() ? nullptr : it->second.get();
   }
   void release(int id) {
     std::lock_guard<std::mutex> g(mu_);
     auto it = reqs_.find(id);
     if (it != reqs_.end() && it->second->done) reqs_.erase(it);
   }
+  // device bytes held by all slots' buffers (for tests of the budget)
+  int64_t held_bytes() {
+    std::lock_guard<std::mutex> g(mu_);
+    int64_t b = 0;
+    for (auto& x : bytes_) b += x.load();
+    return b;
+  }
 
  private:
+  enum SlotState { IDLE, WAITING, RUNNING };
+
+  // memory the slots other than `me` hold (under mu_)
+  size_t others_held(int me) const {
+    size_t b = 0;
+    for (size_t k = 0; k < bytes_.size(); k++)
+      if ((int)k != me) b += std::max<size_t>((size_t)std::max<int64_t>(0, bytes_[k].load()), state_[k] == RUNNING ? fp_[k] : 0);
+    return b;
+  }
+  bool others_running(int me) const {
+    for (size_t k = 0; k < state_.size(); k++)
+      if ((int)k != me && state_[k] == RUNNING) return true;
+    return false;
+  }
+  // drops the buffers of slot k (its reducer is not in use: idle, or the
+  // calling slot itself)
+  void trim(int k) {
+    set_alloc_account(&bytes_[k]);
+    reducers_[k].reset(new DeviceReducer());
+    set_alloc_account(&bytes_[me_of_thread()]);
+  }
+  static int& me_of_thread() {
+    static thread_local int me = -1;
+    return me;
+  }
+
   void worker(int slot) {
-    DeviceReducer& R = *reducers_[slot];
+    me_of_thread() = slot;
+    set_alloc_account(&bytes_[slot]);
     while (true) {
       Request* r = nullptr;
       {
@@ -104,22 +152,56 @@ class Server {
         if (stop_ && queue_.empty()) return;
         r = reqs_[queue_.front()].get();
         queue_.pop_front();
+        state_[slot] = WAITING;
       }
-      run(R, *r);
+      std::string err;
+      int rc;
+      try {
+        rc = run(slot, *r, err);
+      } catch (const std::exception& e) {
+        err = std::string("request failed: ") + e.what();
+        rc = -99;
+      } catch (...) {
+        err = "request failed with an unknown exception";
+        rc = -99;
+      }
+      bool shrink;
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        shrink = bytes_[slot].load() > (int64_t)(budget_ / bytes_.size());
+      }
+      if (shrink) trim(slot);  // outside the lock: the slot is still WAITING/RUNNING for the others
+      std::lock_guard<std::mutex> g(mu_);
+      state_[slot] = IDLE;
+      fp_[slot] = 0;
+      r->status = rc;
+      r->err = err;
+      r->t_end = Clock::now();
+      r->done = true;
+      cv_done_.notify_all();
+      cv_mem_.notify_all();
     }
   }
 
-  void run(DeviceReducer& R, Request& r) {
-    std::string err;
-    int rc = R.analyze(r.part, r.opt, err);  // host only
-    size_t fp = rc == 0 ? R.footprint_bytes() : 0;
+  int run(int slot, Request& r, std::string& err) {
+    int rc = reducers_[slot]->analyze(r.part, r.opt, err);  // host only
+    const size_t fp = rc == 0 ? reducers_[slot]->footprint_bytes() : 0;
     if (rc == 0) {
-      // admission: wait until the footprint fits the device budget (a request
-      // larger than the whole budget runs alone)
       std::unique_lock<std::mutex> g(mu_);
-      cv_mem_.wait(g, [&] { return stop_ || used_ == 0 || used_ + fp <= budget_; });
-      used_ += fp;
+      cv_mem_.wait(g, [&] {
+        if (stop_) return true;
+        const size_t mine = std::max<size_t>((size_t)std::max<int64_t>(0, bytes_[slot].load()), fp);
+        if (others_held(slot) + mine <= budget_) return true;
+        if (others_running(slot)) return false;
+        // nothing else runs: the idle slots give their buffers back
+        for (size_t k = 0; k < state_.size(); k++)
+          if ((int)k != slot && state_[k] == IDLE && bytes_[k].load() > 0) trim((int)k);
+        return true;
+      });
+      state_[slot] = RUNNING;
+      fp_[slot] = fp;
     }
+    DeviceReducer& R = *reducers_[slot];
     r.t_start = Clock::now();
     r.footprint = fp;
     if (rc == 0) rc = R.reduce(err);
@@ -161,23 +243,19 @@ class Server {
         }
       }
     }
-    std::lock_guard<std::mutex> g(mu_);
-    used_ -= fp;
-    r.status = rc;
-    r.err = err;
-    r.t_end = Clock::now();
-    r.done = true;
-    cv_done_.notify_all();
-    cv_mem_.notify_all();
+    return rc;
   }
 
-  size_t budget_, used_ = 0;
+  size_t budget_;
   bool stop_ = false;
   int next_id_ = 0;
   std::mutex mu_;
   std::condition_variable cv_queue_, cv_done_, cv_mem_;
   std::deque<int> queue_;
   std::map<int, std::unique_ptr<Request>> reqs_;
+  std::vector<SlotState> state_;
+  std::vector<size_t> fp_;
+  std::vector<std::atomic<int64_t>> bytes_;
   std::vector<std::unique_ptr<DeviceReducer>> reducers_;
   std::vector<std::thread> threads_;
 };
@@ -247,8 +325,8 @@ int fgpu_server_wait(fgpu_server* s, int id) {
 }
 
 int fgpu_server_sizes(fgpu_server* s, int id, int* sizes) {
-  fgpu::Request* r = s ? s->s->find(id) : nullptr;
-  if (!r || !r->done || !sizes) return -1;
+  fgpu::Request* r = s ? s->s->find_done(id) : nullptr;
+  if (!r || !sizes) return -1;
   sizes[0] = r->res.ndim;
   sizes[1] = r->res.ndof1;
   sizes[2] = r->res.ndof2;
@@ -259,8 +337,8 @@ int fgpu_server_sizes(fgpu_server* s, int id, int* sizes) {
 
 int fgpu_server_result(fgpu_server* s, int id, double* SK, double* SM, double* G, double* Fred, double* Ui,
                        double* times) {
-  fgpu::Request* r = s ? s->s->find(id) : nullptr;
-  if (!r || !r->done) return -1;
+  fgpu::Request* r = s ? s->s->find_done(id) : nullptr;
+  if (!r) return -1;
   if (r->status) return r->status;
   auto put = [](double* dst, const std::vector<double>& v) {
     if (dst && !v.empty()) std::memcpy(dst, v.data(), sizeof(double) * v.size());
@@ -280,7 +358,7 @@ int fgpu_server_result(fgpu_server* s, int id, double* SK, double* SM, double* G
 }
 
 int fgpu_server_error(fgpu_server* s, int id, char* buf, int len) {
-  fgpu::Request* r = s ? s->s->find(id) : nullptr;
+  fgpu::Request* r = s ? s->s->find_done(id) : nullptr;
   if (!r || !buf || len <= 0) return -1;
   std::snprintf(buf, (size_t)len, "%s", r->err.c_str());
   return 0;
@@ -291,5 +369,7 @@ int fgpu_server_release(fgpu_server* s, int id) {
   s->s->release(id);
   return 0;
 }
+
+int64_t fgpu_server_device_bytes(fgpu_server* s) { return s ? s->s->held_bytes() : -1; }
 
 }  // extern "C"

@@ -26,6 +26,8 @@ def _load():
         lib.fgpu_server_result.argtypes = [P, I, P, P, P, P, P, P]
         lib.fgpu_server_error.argtypes = [P, I, ctypes.c_char_p, I]
         lib.fgpu_server_release.argtypes = [P, I]
+        lib.fgpu_server_device_bytes.argtypes = [P]
+        lib.fgpu_server_device_bytes.restype = ctypes.c_int64
         _lib = lib
     return _lib
 

@@ -170,8 +170,11 @@ int fgpu_recover(fgpu_reducer* h, int nrhs, const double* Ub, const double* Fi, 
  * No counterpart in the reference (its reducer handles one part per call):
  * the batched/served form of the same reduction.  `slots` worker threads,
  * each with its own reducer and HIP stream, take requests in arrival order;
- * a request starts when its estimated device footprint fits the budget
- * next to the running ones.  After the reduction each load case gives the
+ * a request starts when its estimated device footprint, with the buffers
+ * every slot holds (slots keep them between requests), fits the budget; a
+ * slot holding more than budget/slots after a request releases them, and a
+ * request that does not fit while nothing else runs starts once the idle
+ * slots have released theirs (a request above the whole budget runs alone).  After the reduction each load case gives the
  * reduced load vector (as fgpu_reduce_loads) and the recovered internal
  * displacements (as fgpu_recover with Fi = the case's internal loads).
  * Results of a request are bitwise the same alone or among others. */
@@ -197,6 +200,8 @@ int fgpu_server_result(fgpu_server* s, int id, double* SK, double* SM, double* G
 int fgpu_server_error(fgpu_server* s, int id, char* buf, int len);
 /* Frees a finished request's results. */
 int fgpu_server_release(fgpu_server* s, int id);
+/* Device bytes the slots' buffers hold now (running and idle slots). */
+int64_t fgpu_server_device_bytes(fgpu_server* s);
 
 /* Diagnostics: times[0..10] (ms) = elements, assembly, factor, B, modes,
  * mass, gravity, total, summed duration of the trailing-update (MFMA)

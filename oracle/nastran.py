@@ -3,7 +3,8 @@
 Nastran bulk-data subset reader for the reference's FE part fixtures
 (src/vpmReducer/vpmReducerTests/FEparts/*.nas, read in the reference by FFl,
 which is absent from the reference tree).  Cards: GRID, CQUAD4, CHEXA, CBAR,
-MAT1, PSHELL, PSOLID, PBAR, PBARL (TUBE), ASET, ASET1, SPC1, RBE2, FORCE,
+MAT1, PSHELL, PSOLID, PBAR, PBARL (TUBE), ASET, ASET1, SPC1 (with "G1 THRU G2"
+ranges), RBE2, FORCE,
 MOMENT, in free (comma), small (8-column) and
 large (16-column, '*') field formats.  Returns a part dict (see
 fedempy_amd.parts) plus the RBE2 constraint list.
@@ -31,6 +32,29 @@ def _num(s):
         return float(t)
     except ValueError:
         return s  # a name field (e.g. the PBARL section type)
+
+
+def _grid_list(vals):
+    """(listed, ranged) grid ids of SPC1/ASET1 grid fields: "G1 THRU G2"
+    adds the ids strictly between G1 and G2 to `ranged` (they need not
+    exist); anything but an integer or THRU raises ValueError."""
+    listed, ranged = [], []
+    vals = [x for x in vals if x is not None]
+    k = 0
+    while k < len(vals):
+        x = vals[k]
+        if isinstance(x, str) and x.upper() == "THRU":
+            if not listed or k + 1 >= len(vals) or not isinstance(vals[k + 1], int) or vals[k + 1] < listed[-1]:
+                raise ValueError("THRU needs grid ids before and after it")
+            ranged.extend(range(listed[-1] + 1, vals[k + 1]))
+            listed.append(vals[k + 1])
+            k += 2
+            continue
+        if not isinstance(x, int) or x <= 0:
+            raise ValueError("grid field %r is not a grid id" % (x,))
+        listed.append(x)
+        k += 1
+    return listed, ranged
 
 
 def _cards(text):
@@ -73,6 +97,7 @@ def read(path):
     with open(path) as f:
         text = f.read()
     grids, elems, mats, props, aset, rbe2, spc, loads = {}, [], {}, {}, [], [], {}, []
+    aset_range = []
     known = {"GRID", "CHEXA", "CQUAD4", "CBAR", "MAT1", "PSHELL", "PSOLID", "PBAR", "PBARL", "ASET", "ASET1",
              "SPC1", "RBE2", "FORCE", "MOMENT"}
     for name, fl in _cards(text):
@@ -110,12 +135,14 @@ def read(path):
             vals = [x for x in v if x is not None]
             aset.extend(int(vals[i]) for i in range(0, len(vals), 2))
         elif name == "ASET1":
-            aset.extend(int(x) for x in v[1:] if x is not None)
+            listed, ranged = _grid_list(v[1:])
+            aset.extend(listed)
+            aset_range.extend(ranged)
         elif name == "SPC1":
             mask = sum(1 << (int(c) - 1) for c in str(fl[1]).strip() if c in "123456")
-            for x in v[2:]:
-                if x is not None:
-                    spc[int(x)] = spc.get(int(x), 0) | mask
+            listed, ranged = _grid_list(v[2:])
+            for x in listed + ranged:
+                spc[x] = spc.get(x, 0) | mask
         elif name in ("FORCE", "MOMENT"):
             if v[2]:
                 raise ValueError("only the basic coordinate system is supported")
@@ -130,9 +157,13 @@ def read(path):
     xyz = np.array([grids[g] for g in ids], float)
     status = np.zeros(len(ids), np.int32)
     for g, m in spc.items():
-        status[idx[g]] = -m
+        if g in idx:
+            status[idx[g]] = -m
     for g in aset:
         status[idx[g]] = 1
+    for g in aset_range:
+        if g in idx:
+            status[idx[g]] = 1
     melcon, conn, eprop = [], [], []
     for t, eid, pid, nodes, orient in elems:
         p = props[pid]

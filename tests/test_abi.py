@@ -142,6 +142,87 @@ def test_written_part_roundtrip_and_sam(lib, tmp_path):
         lib.fgpu_destroy(h)
 
 
+def _thru_part_text():
+    """A 3x1x1 HEX8 strip whose x=0 face is clamped by "SPC1 ... THRU" (grids
+    1-4, with the non-existent grid 3 inside the range skipped) and whose
+    external grids 13, 15, 16 come from "ASET1 ... THRU"."""
+    lines = ["BEGIN BULK", "MAT1,1,2.1e11,,0.3,7850.", "PSOLID,1,1"]
+    ids = {}
+    k = 0
+    for i in range(4):
+        for j in range(2):
+            for m in range(2):
+                k += 1
+                gid = k if k != 3 else 40  # grid 3 does not exist
+                ids[(i, j, m)] = gid
+                lines.append("GRID,%d,,%g,%g,%g" % (gid, 0.01 * i, 0.01 * j, 0.01 * m))
+    for e in range(3):
+        n = [ids[(e, 0, 0)], ids[(e + 1, 0, 0)], ids[(e + 1, 1, 0)], ids[(e, 1, 0)],
+             ids[(e, 0, 1)], ids[(e + 1, 0, 1)], ids[(e + 1, 1, 1)], ids[(e, 1, 1)]]
+        lines.append("CHEXA,%d,1,%s" % (e + 1, ",".join(map(str, n))))
+    lines += ["SPC1,1,123,1,THRU,4", "SPC1,1,123,40", "ASET1,123456,13,THRU,16", "ENDDATA"]
+    return "\n".join(lines) + "\n"
+
+
+def test_nastran_thru_ranges(lib, tmp_path):
+    """SPC1 / ASET1 "G1 THRU G2" ranges in the library's reader and in the
+    oracle's: the same node status, hence bit-exact SAM numbering."""
+    f = tmp_path / "thru.nas"
+    f.write_text(_thru_part_text())
+    o = nastran.read(str(f))
+    ids = list(o["node_ids"])
+    st = dict(zip(ids, o["status"]))
+    assert all(st[g] == -7 for g in (1, 2, 4, 40)) and st[14] == 1 == st[13] == st[15] == st[16]
+    assert sum(1 for v in st.values() if v == 0) == len(ids) - 8
+    assert lib.fgpu_link_read(str(f).encode()) == 0
+    h = ctypes.c_void_p()
+    assert lib.fgpu_create(ctypes.byref(h)) == 0
+    try:
+        assert lib.fgpu_link_attach(h) == 0
+        assert lib.fgpu_analyze(h, 0, 0, 0) == 0
+        sz = (ctypes.c_int * 8)()
+        lib.fgpu_get_sizes(h, sz)
+        s = orc.sam_numbering(o)
+        arrs = [(ctypes.c_int * k)() for k in (sz[0] + 1, sz[1], sz[1])]
+        assert lib.fgpu_get_sam(h, arrs[0], arrs[1], arrs[2], None, None, None) == 0
+        for a, key in zip(arrs, ("madof", "msc", "meqn")):
+            assert np.array_equal(np.array(a[:]), np.asarray(s[key])), key
+    finally:
+        lib.fgpu_destroy(h)
+    bad = tmp_path / "bad_thru.nas"
+    bad.write_text(_thru_part_text().replace("SPC1,1,123,1,THRU,4", "SPC1,1,123,THRU,4"))
+    assert lib.fgpu_link_read(str(bad).encode()) < 0
+    with pytest.raises(ValueError):
+        nastran.read(str(bad))
+    bad.write_text(_thru_part_text().replace("SPC1,1,123,40", "SPC1,1,123,4O"))
+    assert lib.fgpu_link_read(str(bad).encode()) < 0
+    buf = ctypes.create_string_buffer(256)
+    lib.fgpu_link_error(buf, 256)
+    assert b"not a grid id" in buf.value
+
+
+def test_ngen_larger_than_ndof1_is_rejected(lib):
+    """inputReducerModule.f90:403-409: NGEN > NDOF1 is an error (host side
+    of fgpu_analyze, no device work)."""
+    p = parts.hex_block(2, 1, 1, 0.02, 0.01, 0.01, seed=1, n_ext=4)
+    s = orc.sam_numbering(p)
+    h = ctypes.c_void_p()
+    assert lib.fgpu_create(ctypes.byref(h)) == 0
+    try:
+        arrs = [np.ascontiguousarray(p[k], dt) for k, dt in
+                (("xyz", np.float64), ("status", np.int32), ("melcon", np.int32), ("mpmnpc", np.int32),
+                 ("mmnpc", np.int32), ("eprop", np.float64))]
+        ptr = [a.ctypes.data_as(ctypes.c_void_p) for a in arrs]
+        assert lib.fgpu_set_part(h, len(p["xyz"]), ptr[0], ptr[1], len(p["melcon"]), ptr[2], ptr[3], ptr[4],
+                                 ptr[5]) == 0
+        assert lib.fgpu_analyze(h, s["ndof1"] + 1, 0, 0) == -1
+        lib.fgpu_last_error.restype = ctypes.c_char_p
+        assert b"NGEN" in lib.fgpu_last_error(h) and b"NDOF1" in lib.fgpu_last_error(h)
+        assert lib.fgpu_analyze(h, s["ndof1"], s["ndof1"] + 5, 0) == 0
+    finally:
+        lib.fgpu_destroy(h)
+
+
 def test_nastran_reader_rejects_bad_input(lib, tmp_path):
     f = tmp_path / "bad.nas"
     f.write_text("BEGIN BULK\nGRID,1,,0.,0.,0.\nCHEXA,1,1,1,2,3,4,5,6,7,8\nENDDATA\n")
