@@ -288,8 +288,8 @@ int fgpu_recover(fgpu_reducer* h, int nrhs, const double* Ub, const double* Fi, 
 int fgpu_get_times(const fgpu_reducer* h, double* t) {
   if (!h || !h->reduced || !t) return -1;
   const fgpu::PhaseTimes& p = h->red.times();
-  double v[11] = {p.elements, p.assembly, p.factor, p.bsolve, p.modes, p.mass, p.gravity, p.total,
-                  p.update_kernel_ms, (double)p.update_launches, (double)p.refine_steps};
+  double v[13] = {p.elements, p.assembly, p.factor, p.bsolve, p.modes, p.mass, p.gravity, p.total,
+                  p.update_kernel_ms, (double)p.update_launches, (double)p.refine_steps, p.analyze_host, p.upload};
   std::memcpy(t, v, sizeof v);
   return 0;
 }

@@ -75,20 +75,20 @@ struct Symbolic {
   int neq = 0, ndof1 = 0, ndof2 = 0;
   std::vector<int> eq_to_perm;        // 0-based equation -> perm index
   std::vector<int> perm_to_eq;
-  // per element, per local DOF: perm index, -1 if not an equation
-  std::vector<int> edof_ptr;          // nel+1
-  std::vector<int> edof_perm;
   std::vector<int64_t> elem_off;      // offset of each element's packed matrix
   int64_t elem_pool = 0;              // total doubles of packed element matrices
 
-  // full symmetric CSR in perm space
-  std::vector<int> rowptr, colind;
-  // pull-assembly contributions of every CSR entry: packed element entries
-  std::vector<int> cptr;              // nnz+1
-  std::vector<int64_t> cidx;          // index into the packed element pool
-  std::vector<double> ccoef;          // constraint coefficient of each contribution (empty: all 1)
-  // lower CSR entry -> offset into the front pool (-1 for upper entries)
-  std::vector<int64_t> fmap;
+  // CSR row pointers (full symmetric pattern in perm space); the column
+  // indices, the pull-assembly contribution lists and the front map are
+  // built on the device from the arrays below (structure.hip)
+  std::vector<int> rowptr;
+  std::vector<int> nfirst, neqd;      // per node: first perm equation (-1: none), equation count
+  std::vector<int> nbptr, nbl;        // per node: neighbour nodes through elements, itself included
+  std::vector<int> rnode;             // per perm row: its node
+  std::vector<int> colsn;             // per perm column: the front that eliminates it (Schur front last)
+  std::vector<int> drank;             // per DOF: rank among its node's equation DOFs (-1: none)
+  int64_t ncontrib = 0;               // element-matrix contributions to the CSR entries
+  bool has_rigid = false;             // contributions carry constraint coefficients
 
   // supernodes in postorder; the last one (index nsup-1) is the Schur front
   int nsup = 0;

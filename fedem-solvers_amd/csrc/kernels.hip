@@ -164,7 +164,7 @@ __global__ __launch_bounds__(64) void k_hex8(int nhex, const int* __restrict__ e
 // written straight into their front position.
 template <bool COEF>
 __global__ __launch_bounds__(256) void k_assemble(int64_t nnz, const int* __restrict__ cptr,
-                                                  const int64_t* __restrict__ cidx,
+                                                  const int* __restrict__ cidx,
                                                   const double* __restrict__ ccoef,
                                                   const double* __restrict__ Kp,
                                                   const double* __restrict__ Mp,
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_assemble(int64_t nnz, const int* __rest
   if (k >= nnz) return;
   double kv = 0.0, mv = 0.0;
   for (int q = cptr[k]; q < cptr[k + 1]; q++) {
-    int64_t c = cidx[q];
+    const int c = cidx[q];
     if (COEF) {  // rigid-element constraint coefficient of this contribution
       const double w = ccoef[q];
       kv += w * Kp[c];
@@ -1745,7 +1745,7 @@ void launch_hex8(hipStream_t st, int nhex, const int* elist, const int* eptr, co
                  int* ebad) {
   if (nhex > 0) k_hex8<<<nhex, 64, 0, st>>>(nhex, elist, eptr, enodes, xyz, eprop, eoff, Kp, Mp, ebad);
 }
-void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* ccoef,
+void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int* cidx, const double* ccoef,
                      const double* Kp, const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool) {
   if (nnz <= 0) return;
   if (ccoef) k_assemble<true><<<nblk(nnz, 256), 256, 0, st>>>(nnz, cptr, cidx, ccoef, Kp, Mp, Kv, Mv, fmap, pool);

@@ -15,8 +15,30 @@ void launch_andes4(hipStream_t st, int nel, const int* elist, const int* eptr, c
 void launch_beam(hipStream_t st, int nbeam, const int* elist, const int* eptr, const int* enodes, const double* xyz,
                  const double* eprop, const int64_t* eoff, double* Kp, double* Mp, int* ebad);
 // ccoef: per-contribution constraint coefficients, or nullptr (all 1)
-void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int64_t* cidx, const double* ccoef,
+void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int* cidx, const double* ccoef,
                      const double* Kp, const double* Mp, double* Kv, double* Mv, const int64_t* fmap, double* pool);
+// Device construction of the matrix structure (structure.hip): neighbour
+// lists sorted in permuted order with the DOF offset of each neighbour in
+// the node's rows; CSR columns; the element-matrix contributions of every
+// CSR entry (pass 0 counts into cnt, pass 1 appends at cptr + cnt; cidx =
+// packed element pool position, ccoef the constraint coefficient or
+// nullptr); their fixed order; the front map of the lower entries; an
+// exclusive prefix sum (out has n + 1 entries, tsum scan_tiles(n) + 1).
+// bad: bit 0 a contribution outside the CSR pattern, bit 1 an entry outside
+// its front.
+void launch_nbr_sort(hipStream_t st, int nnod, const int* nbptr, int* nbl, const int* nfirst, const int* neqd,
+                     int* nbo);
+void launch_csr_cols(hipStream_t st, int neq, const int* rnode, const int* rowptr, const int* nbptr, const int* nbl,
+                     const int* nbo, const int* nfirst, const int* neqd, int* colind);
+void launch_contrib(hipStream_t st, int pass, int nel, const int* etype, const int* eptr, const int* enodes,
+                    const int64_t* eoff, const int* madof, const int* msc, const int* nndof, const int* dep_indep,
+                    const int* drank, const int* nfirst, const double* xyz, const int* rowptr, const int* nbptr,
+                    const int* nbl, const int* nbo, int* cnt, const int* cptr, int* cidx, double* ccoef, int* bad);
+void launch_seg_sort(hipStream_t st, int nnz, const int* cptr, int* cidx, double* ccoef);
+void launch_fmap(hipStream_t st, int neq, const int* rowptr, const int* colind, const int* colsn, const DevSym& S,
+                 int64_t* fmap, int* bad);
+int scan_tiles(int n);
+void launch_exclusive_scan(hipStream_t st, int n, const int* in, int* out, int* tsum);
 void launch_fill(hipStream_t st, double* p, int64_t n, double v);
 // Pivot-block inverse tiles whose sum X_ij = -X_ii sum_k L_ik X_kj has more
 // than INV_CHUNK terms are summed in INV_CHUNK-tile chunks in parallel.

@@ -162,13 +162,18 @@ static hipError_t run_graph(hipStream_t st, GraphCache& gc, const std::vector<co
 
 struct DeviceReducer::Impl {
   GraphCache g_factor, g_lanczos;
+  int factor_runs = 0;  // factorisations since the last upload
   // part
   DBuf<double> xyz, eprop;
   DBuf<int> eptr, enodes, hexlist, shlist, bmlist, p1;
   DBuf<int64_t> eoff;
   // structure
-  DBuf<int> col0, ncol, nrow, rptr, rows, chptr, chlist, relptr, rel, rowptr, colind, cptr;
-  DBuf<int64_t> foff, woff, cidx, fmap, ioff;
+  DBuf<int> col0, ncol, nrow, rptr, rows, chptr, chlist, relptr, rel, rowptr, colind, cptr, cidx;
+  DBuf<int64_t> foff, woff, fmap, ioff;
+  // inputs of the device structure build: SAM arrays, element types, per
+  // node first equation / equation count / neighbour lists (+ DOF offsets),
+  // per row its node, per column its front
+  DBuf<int> madof, msc, nndof, dep_indep, drank, etype, nfirst, neqd, nbptr, nbl, nbo, rnode, colsn;
   DevSym S{};
   DBuf<FTask> ftasks;
   DBuf<int> xrange;  // extend-add tasks: (child, first, end) column ranges
@@ -549,6 +554,7 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
 }
 
 int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& err) {
+  const auto t0 = std::chrono::steady_clock::now();
   part_ = p;
   opt_ = o;
   if (build_sam(part_, sam_, err)) return -1;
@@ -575,6 +581,7 @@ int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& e
     }
   }
   uploaded_ = false;
+  times_.analyze_host = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
 
@@ -591,8 +598,70 @@ static hipError_t ensure_side(DeviceReducer::Impl& d) {
   return e;
 }
 
+// The assembled-matrix structure built on the device from the host's
+// ordering and fronts: CSR columns, the pull-assembly contribution lists
+// (in a fixed order) and the front map (structure.hip).  Checks the
+// contribution count against the host's and reports entries outside the
+// pattern.
+int DeviceReducer::build_structure(std::string& err) {
+  Impl& d = *d_;
+  const Symbolic& y = sym_;
+  const int nnz = y.rowptr[y.neq];
+  HIPCHK(d.madof.upload(sam_.madof));
+  HIPCHK(d.msc.upload(sam_.msc));
+  HIPCHK(d.nndof.upload(sam_.nndof));
+  HIPCHK(d.dep_indep.upload(sam_.dep_indep));
+  HIPCHK(d.drank.upload(y.drank));
+  HIPCHK(d.etype.upload(part_.etype));
+  HIPCHK(d.nfirst.upload(y.nfirst));
+  HIPCHK(d.neqd.upload(y.neqd));
+  HIPCHK(d.nbptr.upload(y.nbptr));
+  HIPCHK(d.nbl.upload(y.nbl));
+  HIPCHK(d.rnode.upload(y.rnode));
+  HIPCHK(d.colsn.upload(y.colsn));
+  HIPCHK(d.rowptr.upload(y.rowptr));
+  HIPCHK(d.nbo.alloc(std::max<size_t>(1, y.nbl.size())));
+  HIPCHK(d.colind.alloc(nnz));
+  HIPCHK(d.cptr.alloc((size_t)nnz + 1));
+  HIPCHK(d.cidx.alloc(y.ncontrib));
+  if (y.has_rigid) HIPCHK(d.ccoef.alloc(y.ncontrib));
+  else d.ccoef.release();
+  HIPCHK(d.fmap.alloc(nnz));
+  DBuf<int> cnt, tsum, bad;
+  HIPCHK(cnt.alloc((size_t)nnz));
+  HIPCHK(tsum.alloc((size_t)scan_tiles(nnz) + 1));
+  HIPCHK(bad.alloc(1));
+  hipStream_t st = st_;
+  HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), st));
+  launch_nbr_sort(st, part_.nnod, d.nbptr.p, d.nbl.p, d.nfirst.p, d.neqd.p, d.nbo.p);
+  launch_csr_cols(st, y.neq, d.rnode.p, d.rowptr.p, d.nbptr.p, d.nbl.p, d.nbo.p, d.nfirst.p, d.neqd.p, d.colind.p);
+  double* ccoef = y.has_rigid ? d.ccoef.p : nullptr;
+  for (int pass = 0; pass < 2; pass++) {
+    HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(int) * (size_t)nnz, st));
+    launch_contrib(st, pass, part_.nel, d.etype.p, d.eptr.p, d.enodes.p, d.eoff.p, d.madof.p, d.msc.p, d.nndof.p,
+                   d.dep_indep.p, d.drank.p, d.nfirst.p, d.xyz.p, d.rowptr.p, d.nbptr.p, d.nbl.p, d.nbo.p, cnt.p,
+                   d.cptr.p, d.cidx.p, ccoef, bad.p);
+    if (pass == 0) launch_exclusive_scan(st, nnz, cnt.p, d.cptr.p, tsum.p);
+  }
+  launch_seg_sort(st, nnz, d.cptr.p, d.cidx.p, ccoef);
+  launch_fmap(st, y.neq, d.rowptr.p, d.colind.p, d.colsn.p, d.S, d.fmap.p, bad.p);
+  HIPCHK(hipGetLastError());
+  int hb = 0, total = 0;
+  HIPCHK(hipMemcpyAsync(&hb, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&total, d.cptr.p + nnz, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (hb || total != y.ncontrib) {
+    err = "device structure build: " + std::string(hb & 1 ? "element contribution outside the CSR pattern" :
+                                                   hb & 2 ? "matrix entry outside its front" :
+                                                            "contribution count mismatch");
+    return -3;
+  }
+  return 0;
+}
+
 // Device upload of the part and of every structure array the kernels read.
 int DeviceReducer::upload(std::string& err) {
+  const auto t0 = std::chrono::steady_clock::now();
   Impl& d = *d_;
   if (!st_) {
     HIPCHK(hipStreamCreate(&st_));
@@ -639,15 +708,10 @@ int DeviceReducer::upload(std::string& err) {
     d.ipool32.release();
     HIPCHK(d.ipool.alloc(ioff[y.nsup]));
   }
-  HIPCHK(d.rowptr.upload(y.rowptr));
-  HIPCHK(d.colind.upload(y.colind));
-  HIPCHK(d.cptr.upload(y.cptr));
-  HIPCHK(d.cidx.upload(y.cidx));
-  if (!y.ccoef.empty()) HIPCHK(d.ccoef.upload(y.ccoef));
-  HIPCHK(d.fmap.upload(y.fmap));
   HIPCHK(d.dpiv.alloc((size_t)std::max(1, y.ndof1)));
   d.S = DevSym{d.col0.p, d.ncol.p, d.nrow.p, d.foff.p, d.rptr.p, d.rows.p, d.chptr.p,
                d.chlist.p, d.relptr.p, d.rel.p, d.woff.p, d.dpiv.p};
+  if (int rc = build_structure(err)) return rc;
   std::vector<FTask> ft;
   d.fops.clear();
   std::vector<int> xr;
@@ -664,6 +728,7 @@ int DeviceReducer::upload(std::string& err) {
   d.plans.clear();
   // a new structure: every captured launch sequence is stale
   d.g_factor.reset();
+  d.factor_runs = 0;
   d.g_lanczos.reset();
   // value buffers
   const int64_t nnz = y.rowptr[y.neq];
@@ -693,6 +758,7 @@ int DeviceReducer::upload(std::string& err) {
     }
   HIPCHK(d.grav.upload(a));
   uploaded_ = true;
+  times_.upload = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
 
@@ -907,7 +973,7 @@ int DeviceReducer::reduce(std::string& err) {
   // fronts start at zero (eager, outside any graph capture)
   if (d.mixed) HIPCHK(hipMemsetAsync(d.pool32.p, 0, sizeof(float) * y.fpool, st));
   else HIPCHK(hipMemsetAsync(d.pool.p, 0, sizeof(double) * y.fpool, st));
-  launch_assemble(st, nnz, d.cptr.p, d.cidx.p, y.ccoef.empty() ? nullptr : d.ccoef.p, d.Kp.p, d.Mp.p, d.Kv.p, d.Mv.p,
+  launch_assemble(st, nnz, d.cptr.p, d.cidx.p, y.has_rigid ? d.ccoef.p : nullptr, d.Kp.p, d.Mp.p, d.Kv.p, d.Mv.p,
                   d.fmap.p, d.mixed ? nullptr : d.pool.p);
   if (d.mixed) {  // equilibrated stiffness into the FP32 fronts
     launch_diag_scale(st, neq, d.rowptr.p, d.colind.p, d.Kv.p, d.dscale.p);
@@ -974,7 +1040,9 @@ int DeviceReducer::reduce(std::string& err) {
     if (d.mixed) factor_with(d.pool32.p, d.ipool32.p);
     else factor_with(d.pool.p, d.ipool.p);
   };
-  if (opt_.time_updates || !opt_.use_graphs) {
+  // The first factorisation after an upload runs eagerly (a part reduced
+  // once gains nothing from a capture); later ones replay one captured graph.
+  if (opt_.time_updates || !opt_.use_graphs || d.factor_runs++ == 0) {
     factor_ops();  // per-launch events need the plain stream
   } else {
     // the factor schedule is fixed per symbolic structure: one graph
@@ -1515,10 +1583,11 @@ int DeviceReducer::get_csr(std::vector<int>& rowptr, std::vector<int>& colind, s
                            std::vector<double>& M) {
   const Symbolic& y = sym_;
   rowptr = y.rowptr;
-  colind = y.colind;
   size_t nnz = y.rowptr[y.neq];
+  colind.resize(nnz);
   K.resize(nnz);
   M.resize(nnz);
+  if (hipMemcpy(colind.data(), d_->colind.p, sizeof(int) * nnz, hipMemcpyDeviceToHost) != hipSuccess) return -3;
   if (hipMemcpy(K.data(), d_->Kv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost) != hipSuccess) return -3;
   if (hipMemcpy(M.data(), d_->Mv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost) != hipSuccess) return -3;
   return 0;
@@ -1717,7 +1786,8 @@ size_t DeviceReducer::footprint_bytes() const {
   const size_t rhs = (size_t)std::max(std::max(y.ndof2, bs), 3);
   const size_t fb = opt_.mixed ? 4 : 8;  // bytes per factor entry
   size_t b = fb * ((size_t)y.fpool + ipool) + 8 * (2 * nnz + 2 * (size_t)y.elem_pool);  // fronts, inverses, K, M, elements
-  b += 12 * y.cidx.size() + 4 * y.cptr.size() + 8 * nnz + 4 * nnz;                // contributions, fmap, colind
+  b += (y.has_rigid ? 12 : 4) * (size_t)y.ncontrib + 4 * (nnz + 1) + 8 * nnz + 4 * nnz;  // contributions, fmap, colind
+  b += 8 * nnz;                                                                     // structure-build counts
   b += 8 * (size_t)y.neq * (3 * (size_t)y.ndof2 + 2 * mcap + 8 * (size_t)bs);     // B solves and Lanczos basis
   b += 8 * 2 * wrows * rhs;                                                        // solve work rows
   if (opt_.mixed) b += 8 * 3 * (size_t)y.neq * rhs;                                // refinement work

@@ -46,6 +46,8 @@ struct PhaseTimes {  // milliseconds, measured with HIP events on the work strea
   double update_kernel_ms = 0;   // sum of trailing-update (MFMA) launches, when timed
   int refine_steps = 0;          // mixed precision: refinement steps of the B solve
   int64_t update_launches = 0;
+  double analyze_host = 0;       // host analysis of the last analyze(): SAM numbering, ordering, fronts
+  double upload = 0;             // last upload: part and structure transfer, device structure build, schedules
 };
 
 struct Results {
@@ -107,6 +109,7 @@ class DeviceReducer {
   struct Impl;
 
  private:
+  int build_structure(std::string& err);
   Part part_;
   Sam sam_;
   Symbolic sym_;

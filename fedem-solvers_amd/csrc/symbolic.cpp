@@ -13,6 +13,7 @@
 // reducer sees.  The ordering does not change the reduced matrices, only the
 // rounding.
 #include <algorithm>
+#include <cstdint>
 #include <numeric>
 #include "fgpu.hpp"
 
@@ -329,161 +330,72 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
   std::vector<int> colsn(s.neq, nsi);
   for (int sidx = 0; sidx < nsi; sidx++)
     for (int j = 0; j < y.ncol[sidx]; j++) colsn[y.col0[sidx] + j] = sidx;
-  // element DOF maps and packed element matrix offsets
-  y.edof_ptr.assign(p.nel + 1, 0);
+  // packed element matrix offsets
   y.elem_off.assign(p.nel + 1, 0);
-  for (int e = 0; e < p.nel; e++) {
-    int t = p.etype[e];
-    y.edof_ptr[e + 1] = y.edof_ptr[e] + elem_nodes(t) * elem_ndn(t);
-    y.elem_off[e + 1] = y.elem_off[e] + elem_psize(t);
-  }
+  for (int e = 0; e < p.nel; e++) y.elem_off[e + 1] = y.elem_off[e] + elem_psize(p.etype[e]);
   y.elem_pool = y.elem_off[p.nel];
-  y.edof_perm.assign(y.edof_ptr[p.nel], -1);
-  for (int e = 0; e < p.nel; e++) {
-    int t = p.etype[e], d = elem_ndn(t), k = 0;
-    for (int q = p.eptr[e]; q < p.eptr[e + 1]; q++) {
-      int i = p.enodes[q];
-      for (int j = 0; j < d; j++, k++) {
-        if (j >= s.nndof[i]) continue;
-        int dof = s.madof[i] - 1 + j;
-        if (s.msc[dof] > 0) y.edof_perm[y.edof_ptr[e] + k] = y.eq_to_perm[s.meqn[dof] - 1];
-      }
-    }
+  if (y.elem_pool > INT32_MAX) {
+    err = "symbolic: the part's element matrices exceed 32-bit pool indices";
+    return -1;
   }
-  // CSR pattern: node neighbour lists (incl. self) sorted by position
-  std::vector<int> nbptr(nnod + 1, 0), nbl;
-  nbl.reserve(adj.size() + nnod);
-  std::vector<int> coloff;  // DOF offset of each neighbour within the row
-  coloff.reserve(adj.size() + nnod);
+  // Inputs of the device construction of the CSR matrix, the pull-assembly
+  // contribution lists and the front map (structure.hip): per node the
+  // first permuted equation and the equation count, its neighbour list
+  // (itself included, any order), per row its node, per column its front,
+  // per DOF its rank among the node's equation DOFs.  The CSR rows follow
+  // from the neighbour lists: a row of node u holds the equations of its
+  // neighbours in permuted order.
+  y.nfirst = nfirst;
+  y.neqd = neqd;
+  y.nbptr.assign(nnod + 1, 0);
+  y.nbl.clear();
+  y.nbl.reserve(adj.size() + nnod);
+  y.rowptr.assign(s.neq + 1, 0);
   std::vector<int> rowlen(nnod, 0);
   for (int u = 0; u < nnod; u++) {
     if (ncls[u]) {
-      size_t b = nbl.size();
-      nbl.push_back(u);
-      for (int k = adjptr[u]; k < adjptr[u + 1]; k++) nbl.push_back(adj[k]);
-      std::sort(nbl.begin() + b, nbl.end(), [&](int a, int c) { return npos[a] < npos[c]; });
-      int off = 0;
-      for (size_t k = b; k < nbl.size(); k++) { coloff.push_back(off); off += neqd[nbl[k]]; }
-      rowlen[u] = off;
+      y.nbl.push_back(u);
+      int len = neqd[u];
+      for (int k = adjptr[u]; k < adjptr[u + 1]; k++) {
+        y.nbl.push_back(adj[k]);
+        len += neqd[adj[k]];
+      }
+      rowlen[u] = len;
     }
-    nbptr[u + 1] = (int)nbl.size();
+    y.nbptr[u + 1] = (int)y.nbl.size();
   }
-  y.rowptr.assign(s.neq + 1, 0);
+  y.rnode.assign(s.neq, -1);
   for (int q = 0; q < pos; q++) {
-    int i = node_at[q];
-    for (int j = 0; j < neqd[i]; j++) y.rowptr[nfirst[i] + j + 1] = rowlen[i];
+    const int i = node_at[q];
+    for (int j = 0; j < neqd[i]; j++) {
+      y.rnode[nfirst[i] + j] = i;
+      y.rowptr[nfirst[i] + j + 1] = rowlen[i];
+    }
   }
   for (int r = 0; r < s.neq; r++) y.rowptr[r + 1] += y.rowptr[r];
-  const int64_t nnz = y.rowptr[s.neq];
-  y.colind.resize(nnz);
-  for (int u = 0; u < nnod; u++) {
-    if (!ncls[u]) continue;
-    for (int j = 0; j < neqd[u]; j++) {
-      int r = nfirst[u] + j, k = y.rowptr[r];
-      for (int q = nbptr[u]; q < nbptr[u + 1]; q++) {
-        int v = nbl[q];
-        for (int jj = 0; jj < neqd[v]; jj++) y.colind[k++] = nfirst[v] + jj;
-      }
-    }
-  }
-  // local DOF index -> equation-DOF rank within the node (-1 if no equation)
-  std::vector<int> drank(s.ndof, -1);
+  y.colsn.swap(colsn);
+  y.drank.assign(s.ndof, -1);
   for (int i = 0; i < nnod; i++) {
     int k = 0;
     for (int j = 0; j < s.nndof[i]; j++)
-      if (s.msc[s.madof[i] - 1 + j] > 0) drank[s.madof[i] - 1 + j] = k++;
+      if (s.msc[s.madof[i] - 1 + j] > 0) y.drank[s.madof[i] - 1 + j] = k++;
   }
-  // contributions: two passes over (element, node pair, dof pair, term pair);
-  // a coefficient is kept only when the part has rigid elements
-  const bool coef = !p.rgd_indep.empty();
-  std::vector<int> ccount(nnz + 1, 0);
-  auto find_nb = [&](int u, int v) -> int {
-    const int* b = &nbl[nbptr[u]];
-    const int* e = &nbl[nbptr[u + 1]];
-    const int* it = std::lower_bound(b, e, v, [&](int a, int c) { return npos[a] < npos[c]; });
-    return (int)(it - &nbl[0]);
-  };
-  for (int pass = 0; pass < 2; pass++) {
-    std::vector<int> fill;
-    if (pass == 1) {
-      for (int64_t k = 0; k < nnz; k++) ccount[k + 1] += ccount[k];
-      y.cptr.assign(ccount.begin(), ccount.end());
-      y.cidx.resize(ccount[nnz]);
-      if (coef) y.ccoef.resize(ccount[nnz]);
-      fill.assign(ccount.begin(), ccount.end() - 1);
-    }
-    // column offset of node q in node pn's CSR row, for the last pair asked
-    // (the DOF and term pairs of one node pair repeat it)
-    int last_p = -1, last_q = -1, last_co = 0;
-    auto col_of = [&](int pn, int qn) -> int {
-      if (pn != last_p || qn != last_q) {
-        last_p = pn;
-        last_q = qn;
-        last_co = coloff[find_nb(pn, qn)];
+  // contributions: every pair of equation terms of an element's DOFs
+  y.has_rigid = !p.rgd_indep.empty();
+  y.ncontrib = 0;
+  for (int e = 0; e < p.nel; e++) {
+    const int t = p.etype[e], d = elem_ndn(t);
+    int64_t nt = 0;
+    for (int q = p.eptr[e]; q < p.eptr[e + 1]; q++)
+      for (int j = 0; j < d && j < s.nndof[p.enodes[q]]; j++) {
+        Term tm[3];
+        nt += dof_terms(p.enodes[q], j, tm);
       }
-      return last_co;
-    };
-    for (int e = 0; e < p.nel; e++) {
-      int t = p.etype[e], d = elem_ndn(t), nn = elem_nodes(t);
-      const int* en = &p.enodes[p.eptr[e]];
-      // equation terms of the element's local DOFs, each with the CSR start
-      // of its row and its rank within its node
-      Term tl[8 * 6][3];
-      int nt[8 * 6], trk[8 * 6][3];
-      int64_t trow[8 * 6][3];
-      for (int a = 0; a < nn; a++)
-        for (int i = 0; i < d; i++) {
-          const int li = a * d + i;
-          nt[li] = i < s.nndof[en[a]] ? dof_terms(en[a], i, tl[li]) : 0;
-          for (int ti = 0; ti < nt[li]; ti++) {
-            const Term& T = tl[li][ti];
-            trk[li][ti] = drank[s.madof[T.node] - 1 + T.j];
-            trow[li][ti] = y.rowptr[nfirst[T.node] + trk[li][ti]];
-          }
-        }
-      for (int a = 0; a < nn; a++)
-        for (int b = 0; b < nn; b++) {
-          int blk = a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a;
-          for (int i = 0; i < d; i++)
-            for (int j = 0; j < d; j++) {
-              int off = a >= b ? (blk * d * d + i * d + j) : (blk * d * d + j * d + i);
-              const int li = a * d + i, lj = b * d + j;
-              for (int ti = 0; ti < nt[li]; ti++) {
-                const Term& P = tl[li][ti];
-                for (int tj = 0; tj < nt[lj]; tj++) {
-                  const Term& Q = tl[lj][tj];
-                  const int co = col_of(P.node, Q.node);
-                  int64_t k = trow[li][ti] + co + trk[lj][tj];
-                  if (pass == 0) { ccount[k + 1]++; continue; }
-                  int64_t slot = fill[k]++;
-                  y.cidx[slot] = y.elem_off[e] + off;
-                  if (coef) y.ccoef[slot] = P.c * Q.c;
-                }
-              }
-            }
-        }
-    }
+    y.ncontrib += nt * nt;
   }
-  // lower CSR entries -> front positions
-  y.fmap.assign(nnz, -1);
-  for (int r = 0; r < s.neq; r++) {
-    int last_sn = -1;  // consecutive columns of a row mostly share a front
-    int64_t rpos = 0;
-    for (int64_t k = y.rowptr[r]; k < y.rowptr[r + 1]; k++) {
-      int c = y.colind[k];
-      if (c > r) continue;
-      int sn = colsn[c];
-      if (sn != last_sn) {
-        const int* rb = &y.rows[y.rptr[sn]];
-        const int* re = rb + y.nrow[sn];
-        const int* it = std::lower_bound(rb, re, r);
-        if (it == re || *it != r) { err = "symbolic: matrix entry outside its front"; return -1; }
-        last_sn = sn;
-        rpos = (int64_t)(it - rb);
-      }
-      int64_t ld = y.nrow[sn];
-      y.fmap[k] = y.foff[sn] + rpos + (int64_t)(c - y.col0[sn]) * ld;
-    }
+  if (y.rowptr[s.neq] < 0 || y.ncontrib > INT32_MAX) {
+    err = "symbolic: the part's matrix is too large for 32-bit contribution indices";
+    return -1;
   }
   (void)roots;
   return 0;

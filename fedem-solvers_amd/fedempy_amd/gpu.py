@@ -208,10 +208,10 @@ class GpuReducer:
         return {"factorPanel": p.value, "solveSplitK": k.value}
 
     def times(self):
-        t = np.zeros(11)
+        t = np.zeros(13)
         self._check(self.lib.fgpu_get_times(self.h, _ptr(t)), "fgpu_get_times")
         keys = ("elements", "assembly", "factor", "bsolve", "modes", "mass", "gravity", "total",
-                "update_kernel_ms", "update_launches", "refine_steps")
+                "update_kernel_ms", "update_launches", "refine_steps", "analyze_host", "upload")
         return dict(zip(keys, t.tolist()))
 
     def info(self):

@@ -203,10 +203,14 @@ int fgpu_server_release(fgpu_server* s, int id);
 /* Device bytes the slots' buffers hold now (running and idle slots). */
 int64_t fgpu_server_device_bytes(fgpu_server* s);
 
-/* Diagnostics: times[0..10] (ms) = elements, assembly, factor, B, modes,
+/* Diagnostics: times[0..12] (ms) = elements, assembly, factor, B, modes,
  * mass, gravity, total, summed duration of the trailing-update (MFMA)
  * launches and their count (option "timeUpdates"), refinement steps of the
- * mixed-precision B solve; info[0..9] = nsup, nlevels, front pool doubles,
+ * mixed-precision B solve, host analysis of the last fgpu_analyze (SAM
+ * numbering, ordering, fronts), the last upload (part and structure
+ * transfer, device construction of the CSR pattern, contribution lists
+ * and front map, schedules; done by the first fgpu_reduce after
+ * fgpu_analyze); info[0..9] = nsup, nlevels, front pool doubles,
  * nnz(L), factor flops, max front, nnz(K), Lanczos vectors, packed element
  * pool doubles, trailing-update (MFMA) flops. */
 int fgpu_get_times(const fgpu_reducer* h, double* times);
