@@ -137,6 +137,8 @@ int fgpu_set_option(fgpu_reducer* h, const char* name, double value) {
   else if (n == "factorPanel") o.factor_panel = (int)value;
   else if (n == "refineTol") o.ir_tol = value;
   else if (n == "refineMaxSteps") o.ir_maxit = (int)value;
+  else if (n == "tolFactorize") o.tol_factorize = value;
+  else if (n == "singularityHandler") o.sing_handler = (int)value;
   else { h->err = "fgpu_set_option: unknown option " + n; return -1; }
   return 0;
 }
@@ -346,6 +348,31 @@ int fgpu_get_perm(const fgpu_reducer* h, int* perm_to_eq) {
   const std::vector<int>& pe = h->red.sym().perm_to_eq;
   std::memcpy(perm_to_eq, pe.data(), pe.size() * sizeof(int));
   return 0;
+}
+
+int fgpu_get_pivots(const fgpu_reducer* h, int* counts, int* node, int* ldof, int* eqn, int* kind, int max) {
+  if (!h || !h->analyzed) return -1;
+  const fgpu::Results& r = h->red.results();
+  if (counts) {
+    counts[0] = r.n_negative;
+    counts[1] = r.n_true_zero;
+    counts[2] = r.n_reduced_zero;
+  }
+  const fgpu::Sam& s = h->red.sam();
+  std::vector<int> dof_of(s.neq, -1);
+  for (int d = 0; d < s.ndof; d++)
+    if (s.meqn[d] > 0) dof_of[s.meqn[d] - 1] = d;
+  const int n = (int)r.singular.size();
+  for (int k = 0; k < n && k < max; k++) {
+    const int e = r.singular[k].first, d = dof_of[e];
+    int i = 0;
+    while (i + 1 < s.nnod && s.madof[i + 1] - 1 <= d) i++;
+    if (node) node[k] = i + 1;
+    if (ldof) ldof[k] = d - (s.madof[i] - 1) + 1;
+    if (eqn) eqn[k] = e + 1;
+    if (kind) kind[k] = r.singular[k].second;
+  }
+  return n;
 }
 
 int fgpu_get_lanczos(const fgpu_reducer* h, int* nvec, double* resid) {

@@ -39,6 +39,14 @@ struct DevSym {
   const int* rel;
   const int64_t* woff;  // per-front offsets of solve work buffers (rows)
   double* dpiv;         // pivots D of the LDL^T factor by pivot column (col0[front] + column)
+  // singular pivots: a pivot d_j with |d_j| <= tolf |a_jj| (a_jj the
+  // original diagonal, adiag) is replaced by |a_jj| (by amax when a_jj = 0)
+  // and its column flagged in sing: 1 a true zero (|d_j| at rounding level
+  // of a_jj, or a_jj = 0), 2 a pivot reduced to near zero
+  const double* adiag;
+  const double* amax;
+  int* sing;
+  double tolf;
 };
 
 }  // namespace fgpu

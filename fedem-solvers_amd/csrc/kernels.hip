@@ -193,6 +193,38 @@ __global__ __launch_bounds__(256) void k_assemble(int64_t nnz, const int* __rest
   }
 }
 
+// original diagonal a_jj of the first n permuted equations (times rs_j^2 when
+// rs is given: the equilibrated matrix) and the largest |a_jj| in *amax
+// (non-negative doubles order as their bit patterns; *amax starts at 0)
+__global__ void k_diag_extract(int n, const int* __restrict__ rowptr, const int* __restrict__ colind,
+                               const double* __restrict__ val, const double* __restrict__ rs,
+                               double* __restrict__ adiag, unsigned long long* __restrict__ amax) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double a = 0.0;
+  for (int k = rowptr[r]; k < rowptr[r + 1]; k++)
+    if (colind[k] == r) a = val[k];
+  if (rs) a *= rs[r] * rs[r];
+  adiag[r] = a;
+  atomicMax(amax, (unsigned long long)__double_as_longlong(fabs(a)));
+}
+
+// pivot statistics of the factor: out[0] negative pivots, out[1] true-zero
+// and out[2] reduced-to-zero pivots replaced, out[3..5] the first such
+// pivot columns (INT_MAX: none); out starts at {0, 0, 0, INT_MAX x 3}
+__global__ void k_pivot_stats(int n, const double* __restrict__ dpiv, const int* __restrict__ sing,
+                              int* __restrict__ out) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    if (sing[c]) {
+      atomicAdd(&out[sing[c]], 1);
+      atomicMin(&out[3 + sing[c]], c);
+    } else if (dpiv[c] < 0.0) {
+      atomicAdd(&out[0], 1);
+      atomicMin(&out[3], c);
+    }
+  }
+}
+
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -342,7 +374,8 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 // its diagonal, sD = D, sX = L^-1, all in FP64.
 template <class T>
 __device__ __forceinline__ void diag_block(const T* __restrict__ F, int ld, int k0, int kb, double (&sA)[64][65],
-                                           double (&sX)[64][65], double (&sD)[64], double (&sT)[4][16][17]) {
+                                           double (&sX)[64][65], double (&sD)[64], double (&sT)[4][16][17],
+                                           const DevSym& S, int colbase) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
@@ -364,7 +397,17 @@ __device__ __forceinline__ void diag_block(const T* __restrict__ F, int ld, int 
       }
 #pragma unroll
       for (int j = 0; j < 16; j++) {
-        const double dj = lane_bcast(a[j], j);
+        double dj = lane_bcast(a[j], j);
+        if (c0 + j < kb) {  // singular pivot: replaced, column flagged (same decision on every lane)
+          const int col = colbase + c0 + j;
+          const double aj = S.adiag[col];
+          if (!(fabs(dj) > S.tolf * fabs(aj))) {
+            const bool truez = aj == 0.0 || fabs(dj) <= 1.0e3 * 2.220446049250313e-16 * fabs(aj);
+            dj = aj != 0.0 ? fabs(aj) : *S.amax;
+            if (lane == j) a[j] = dj;
+            if (lane == 0) S.sing[col] = truez ? 1 : 2;
+          }
+        }
         const bool below = lane > j && lane < 16;
         const double l = below ? a[j] / dj : 0.0;
         if (below) a[j] = l;
@@ -465,13 +508,8 @@ __device__ __forceinline__ void diag_store(T* __restrict__ Xg, double* __restric
     const int i = idx & 63, m = idx >> 6;
     if (i < kb && m <= i) Xg[(int64_t)(k0 + i) + (int64_t)(k0 + m) * nc] = (T)sX[i][m];
   }
-  if (tid < kb) {
-    dpiv[col0 + k0 + tid] = sA[tid][tid];
-    if (!(sA[tid][tid] > 0.0)) {  // K_ii must be positive definite
-      atomicAdd(&piv[0], 1);
-      atomicMin(&piv[1], col0 + k0 + tid);
-    }
-  }
+  if (tid < kb) dpiv[col0 + k0 + tid] = sA[tid][tid];
+  (void)piv;
 }
 
 template <class T>
@@ -484,7 +522,7 @@ __global__ __launch_bounds__(256) void k_diag(const FTask* __restrict__ tasks, D
   const int kb = min(64, nc - k0);
   T* F = pool + S.foff[sn];
   T* Xg = ipool + ioff[sn];
-  diag_block<T>(F, ld, k0, kb, sA, sX, sD, sT);
+  diag_block<T>(F, ld, k0, kb, sA, sX, sD, sT, S, S.col0[sn] + k0);
   diag_store<T>(Xg, S.dpiv, nc, k0, kb, S.col0[sn], sA, sX, piv);
 }
 
@@ -517,7 +555,7 @@ __global__ __launch_bounds__(256) void k_diag_trsm(const FTask* __restrict__ tas
       ra[q] = (m < kb && R0 + i < ld) ? F[(int64_t)(R0 + i) + (int64_t)(k0 + m) * ld] : (T)0;
     }
   }
-  diag_block<T>(F, ld, k0, kb, sA, sX, sD, sT);
+  diag_block<T>(F, ld, k0, kb, sA, sX, sD, sT, S, S.col0[sn] + k0);
   if (t.a <= 0) diag_store<T>(Xg, S.dpiv, nc, k0, kb, S.col0[sn], sA, sX, piv);
   if (t.a < 0) return;
   __syncthreads();  // every read of sA by the store is done
@@ -1669,7 +1707,7 @@ __global__ void k_diag_scale(int n, const int* __restrict__ rowptr, const int* _
   double d = 0.0;
   for (int k = rowptr[r]; k < rowptr[r + 1]; k++)
     if (colind[k] == r) d = val[k];
-  s[r] = d > 0.0 ? 1.0 / sqrt(d) : 1.0;
+  s[r] = d != 0.0 ? 1.0 / sqrt(fabs(d)) : 1.0;
 }
 
 // The equilibrated stiffness s_r K_rc s_c of every lower CSR entry into its
@@ -1750,6 +1788,15 @@ void launch_assemble(hipStream_t st, int64_t nnz, const int* cptr, const int* ci
   if (nnz <= 0) return;
   if (ccoef) k_assemble<true><<<nblk(nnz, 256), 256, 0, st>>>(nnz, cptr, cidx, ccoef, Kp, Mp, Kv, Mv, fmap, pool);
   else k_assemble<false><<<nblk(nnz, 256), 256, 0, st>>>(nnz, cptr, cidx, ccoef, Kp, Mp, Kv, Mv, fmap, pool);
+}
+void launch_diag_extract(hipStream_t st, int n, const int* rowptr, const int* colind, const double* val,
+                         const double* rs, double* adiag, double* amax) {
+  if (n > 0)
+    k_diag_extract<<<nblk(n, 256), 256, 0, st>>>(n, rowptr, colind, val, rs, adiag,
+                                                 reinterpret_cast<unsigned long long*>(amax));
+}
+void launch_pivot_stats(hipStream_t st, int n, const double* dpiv, const int* sing, int* out) {
+  if (n > 0) k_pivot_stats<<<std::min(1024u, nblk(n, 256)), 256, 0, st>>>(n, dpiv, sing, out);
 }
 void launch_fill(hipStream_t st, double* p, int64_t n, double v) {
   if (n > 0) k_fill<<<(unsigned)std::min<int64_t>(nblk(n, 256), 16384), 256, 0, st>>>(p, n, v);

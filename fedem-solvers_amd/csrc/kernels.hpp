@@ -40,6 +40,14 @@ void launch_fmap(hipStream_t st, int neq, const int* rowptr, const int* colind, 
 int scan_tiles(int n);
 void launch_exclusive_scan(hipStream_t st, int n, const int* in, int* out, int* tsum);
 void launch_fill(hipStream_t st, double* p, int64_t n, double v);
+// singular-pivot support of the factorisation: the original (or, with rs,
+// equilibrated) diagonal of the first n permuted equations and its largest
+// magnitude (amax must start at 0); the pivot statistics
+// out[0..5] = negative, true-zero, reduced-to-zero pivots, first column of
+// each (out must start at {0, 0, 0, INT_MAX, INT_MAX, INT_MAX})
+void launch_diag_extract(hipStream_t st, int n, const int* rowptr, const int* colind, const double* val,
+                         const double* rs, double* adiag, double* amax);
+void launch_pivot_stats(hipStream_t st, int n, const double* dpiv, const int* sing, int* out);
 // Pivot-block inverse tiles whose sum X_ij = -X_ii sum_k L_ik X_kj has more
 // than INV_CHUNK terms are summed in INV_CHUNK-tile chunks in parallel.
 constexpr int INV_CHUNK = 4;

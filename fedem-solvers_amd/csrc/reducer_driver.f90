@@ -218,6 +218,13 @@ module fgpu_bind
        type(c_ptr), value :: lcase, node, dof, val
      end function fgpu_link_loads
 
+     integer(c_int) function fgpu_get_pivots (h,counts,node,ldof,eqn,kind,nmax) bind(C,name='fgpu_get_pivots')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, node, ldof, eqn, kind
+       integer(c_int), intent(out) :: counts(3)
+       integer(c_int), value :: nmax
+     end function fgpu_get_pivots
+
      integer(c_int) function fgpu_solve_internal (h,nrhs,R,X) bind(C,name='fgpu_solve_internal')
        import :: c_int, c_ptr
        type(c_ptr), value :: h
@@ -352,6 +359,48 @@ contains
     rc = fgpu_copy_error(h,msg,len(msg))
     write(lpu,"(' *** Error: ',A,' failed, ierr =',I4/5X,A)") what, ierr, trim(msg)
   end subroutine report
+
+  !> @brief Reports the singular and negative pivots of the last
+  !> factorisation (reportSingEqns, cmstrs.f90:1324-1352; CMSTRS,
+  !> cmstrs.f90:644-652; warningOnNegativePivots,
+  !> solExtensionModule.f90:1023-1031).  Node numbers are the part's ids.
+  subroutine pivot_report (lpu,h,ierr)
+    integer, intent(in)     :: lpu, ierr
+    type(c_ptr), intent(in) :: h
+    integer(c_int) :: counts(3), nsing, nnod, rc
+    integer(c_int), allocatable, target :: node(:), ldof(:), eqn(:), kind(:), ids(:)
+    integer(c_int) :: szs(4)
+    integer :: i
+    counts = 0
+    nsing = fgpu_get_pivots(h,counts,c_null_ptr,c_null_ptr,c_null_ptr,c_null_ptr,0_c_int)
+    if (nsing > 0) then
+       allocate(node(nsing),ldof(nsing),eqn(nsing),kind(nsing))
+       nsing = fgpu_get_pivots(h,counts,c_loc(node),c_loc(ldof),c_loc(eqn),c_loc(kind),nsing)
+       rc = fgpu_link_sizes(szs)
+       nnod = szs(1)
+       allocate(ids(max(nnod,1)))
+       rc = fgpu_link_node_ids(ids)
+       write(lpu,"(/' *** Error: Singular or ill-conditioned stiffness matrix detected at the following node(s):')")
+       do i = 1, nsing
+          if (node(i) >= 1 .and. node(i) <= nnod) node(i) = ids(node(i))
+          if (kind(i) == 1) then
+             write(lpu,"(5X,'Node',i9,' local DOF',i3,' ( eqn. # ',i9,' )  zero pivot')") node(i), ldof(i), eqn(i)
+          else
+             write(lpu,"(5X,'Node',i9,' local DOF',i3,' ( eqn. # ',i9,' )  reduced-to-zero pivot')") &
+                  &   node(i), ldof(i), eqn(i)
+          end if
+       end do
+       if (ierr == 0) write(lpu,"(' *** Warning: The above singularities have been circumvented by replacing',&
+            &  ' the zero (or nearly zero)'/5X,'pivots by the magnitude of the original diagonal entry.')")
+       deallocate(node,ldof,eqn,kind,ids)
+    end if
+    if (counts(1) > 0) then
+       write(lpu,"(/' *** Warning: The stiffness matrix is not positive definite.'" // &
+            &     "/5X,'There are',i6,' negative pivots.'" // &
+            &     "/5X,'This can be caused by too many ill-shaped finite elements.'" // &
+            &     "/5X,'Using this FE part may lead to an instable dynamics simulation.')") counts(1)
+    end if
+  end subroutine pivot_report
 
   !> @brief Writes one column-major FP64 array with its file tag.
   subroutine write_db (lpu,fname,dtype,chk,a,n,ierr)
@@ -527,6 +576,8 @@ subroutine fgpu_reducer_run (ierr) bind(C,name='fgpu_reducer_run')
   if (fgpu_opt_int(cs('lanczosMaxVectors')) > 0) &
        rc = fgpu_set_option(h,cs('lanczosMaxVectors'),real(fgpu_opt_int(cs('lanczosMaxVectors')),dp))
   if (fgpu_opt_bool(cs('mixedPrecision')) /= 0) rc = fgpu_set_option(h,cs('mixedPrecision'),1.0_dp)
+  rc = fgpu_set_option(h,cs('tolFactorize'),max(fgpu_opt_double(cs('tolFactorize')),1.0e-20_dp))
+  rc = fgpu_set_option(h,cs('singularityHandler'),real(fgpu_opt_int(cs('singularityHandler')),dp))
   ierr = fgpu_link_attach(h)
   if (ierr /= 0) then
      call link_failure (lpu,ierr)
@@ -567,6 +618,7 @@ subroutine fgpu_reducer_run (ierr) bind(C,name='fgpu_reducer_run')
 
   ! --- The numeric reduction on the device
   ierr = fgpu_reduce(h)
+  call pivot_report (lpu,h,ierr)
   if (ierr /= 0) then
      call report (lpu,h,'Reduction',ierr)
      goto 900
@@ -713,6 +765,8 @@ subroutine fgpu_solver_run (n,nodes,displ,ierr) bind(C,name='fgpu_solver_run')
   ierr = fgpu_create(h)
   if (ierr /= 0) goto 900
   if (fgpu_opt_bool(cs('mixedPrecision')) /= 0) rc = fgpu_set_option(h,cs('mixedPrecision'),1.0_dp)
+  rc = fgpu_set_option(h,cs('tolFactorize'),max(fgpu_opt_double(cs('tolFactorize')),1.0e-20_dp))
+  rc = fgpu_set_option(h,cs('singularityHandler'),real(fgpu_opt_int(cs('singularityHandler')),dp))
   ierr = fgpu_link_attach(h)
   if (ierr /= 0) then
      call link_failure (lpu,ierr)
@@ -739,6 +793,7 @@ subroutine fgpu_solver_run (n,nodes,displ,ierr) bind(C,name='fgpu_solver_run')
   ndof  = sizes(2)
   ndof1 = sizes(4)
   ierr = fgpu_reduce(h)
+  call pivot_report (lpu,h,ierr)
   if (ierr /= 0) then
      call report (lpu,h,'Factorization',ierr)
      goto 900

@@ -178,6 +178,8 @@ struct DeviceReducer::Impl {
   DBuf<FTask> ftasks;
   DBuf<int> xrange;  // extend-add tasks: (child, first, end) column ranges
   DBuf<double> dpiv;   // pivots D of the factor by pivot column (FP64 for both factor types)
+  DBuf<double> adiag, amax;  // original (mixed: equilibrated) diagonal of K_ii and its largest magnitude
+  DBuf<int> sing;            // per pivot column: 0, 1 true zero replaced, 2 reduced-to-zero replaced
   DBuf<double> ipart;  // partial sums of split pivot-block inverse tiles (64 x 64 each; FP32 ones use half)
   std::vector<Op> fops;
   std::map<int, std::unique_ptr<SolvePlan>> plans;
@@ -709,8 +711,12 @@ int DeviceReducer::upload(std::string& err) {
     HIPCHK(d.ipool.alloc(ioff[y.nsup]));
   }
   HIPCHK(d.dpiv.alloc((size_t)std::max(1, y.ndof1)));
+  HIPCHK(d.adiag.alloc((size_t)std::max(1, y.ndof1)));
+  HIPCHK(d.amax.alloc(1));
+  HIPCHK(d.sing.alloc((size_t)std::max(1, y.ndof1)));
   d.S = DevSym{d.col0.p, d.ncol.p, d.nrow.p, d.foff.p, d.rptr.p, d.rows.p, d.chptr.p,
-               d.chlist.p, d.relptr.p, d.rel.p, d.woff.p, d.dpiv.p};
+               d.chlist.p, d.relptr.p, d.rel.p, d.woff.p, d.dpiv.p,
+               d.adiag.p, d.amax.p, d.sing.p, std::max(opt_.tol_factorize, 1e-20)};
   if (int rc = build_structure(err)) return rc;
   std::vector<FTask> ft;
   d.fops.clear();
@@ -748,7 +754,7 @@ int DeviceReducer::upload(std::string& err) {
   }
   HIPCHK(d.ebad.alloc(part_.nel));
   HIPCHK(d.emass.alloc(part_.nel + 1));
-  HIPCHK(d.piv.alloc(2));
+  HIPCHK(d.piv.alloc(6));
   // gravity directions: unit accelerations on translational equation DOFs
   std::vector<double> a((size_t)y.neq * 3, 0.0);
   for (int i = 0; i < sam_.nnod; i++)
@@ -979,11 +985,16 @@ int DeviceReducer::reduce(std::string& err) {
     launch_diag_scale(st, neq, d.rowptr.p, d.colind.p, d.Kv.p, d.dscale.p);
     launch_front_scatter32(st, neq, d.rowptr.p, d.colind.p, d.Kv.p, d.dscale.p, d.fmap.p, d.pool32.p);
   }
+  // original diagonal of K_ii for the singular-pivot test (equilibrated for
+  // the mixed-precision factor), no pivot flagged yet
+  HIPCHK(hipMemsetAsync(d.amax.p, 0, sizeof(double), st));
+  HIPCHK(hipMemsetAsync(d.sing.p, 0, sizeof(int) * (size_t)std::max(1, n1), st));
+  launch_diag_extract(st, n1, d.rowptr.p, d.colind.p, d.Kv.p, d.mixed ? d.dscale.p : nullptr, d.adiag.p, d.amax.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[2], st));
   // ---- multifrontal LDL^T; the Schur front ends up holding K_bb - K_bi K_ii^-1 K_ib
   {
-    int init[2] = {0, INT_MAX};
+    int init[6] = {0, 0, 0, INT_MAX, INT_MAX, INT_MAX};
     HIPCHK(hipMemcpyAsync(d.piv.p, init, sizeof init, hipMemcpyHostToDevice, st));
   }
   if (opt_.time_updates && d.upd_ev.empty()) {
@@ -1052,6 +1063,7 @@ int DeviceReducer::reduce(std::string& err) {
                      factor_ops));
   }
   HIPCHK(fork_err);
+  launch_pivot_stats(st, n1, d.dpiv.p, d.sing.p, d.piv.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[3], st));
   // ---- B = -K_ii^{-1} K_ib: back-substitution of [0; I].  With the FP64
@@ -1124,7 +1136,7 @@ int DeviceReducer::reduce(std::string& err) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[4], st));
   // error checks of the numeric phase
-  int hbad[2] = {0, 0};
+  int hbad[6] = {0, 0, 0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(hbad, d.piv.p, sizeof hbad, hipMemcpyDeviceToHost, st));
   std::vector<int> eb(part_.nel);
   HIPCHK(hipMemcpyAsync(eb.data(), d.ebad.p, sizeof(int) * part_.nel, hipMemcpyDeviceToHost, st));
@@ -1136,10 +1148,24 @@ int DeviceReducer::reduce(std::string& err) {
                          : " has a non-positive Jacobian determinant");
       return -4;
     }
-  if (hbad[0] > 0) {
-    int eq = y.perm_to_eq[hbad[1]] + 1;
-    err = std::to_string(hbad[0]) + " non-positive pivot(s) in K_ii; first at equation " + std::to_string(eq);
-    return -5;
+  res_.n_negative = hbad[0];
+  res_.n_true_zero = hbad[1];
+  res_.n_reduced_zero = hbad[2];
+  if (hbad[1] + hbad[2] > 0) {
+    std::vector<int> sg(n1);
+    HIPCHK(hipMemcpy(sg.data(), d.sing.p, sizeof(int) * n1, hipMemcpyDeviceToHost));
+    for (int c = 0; c < n1; c++)
+      if (sg[c]) res_.singular.push_back({y.perm_to_eq[c], sg[c]});
+    std::sort(res_.singular.begin(), res_.singular.end());
+    const int h = opt_.sing_handler;
+    const bool abort = h == 0 || (h == 1 && hbad[2] > 0);
+    if (abort) {
+      const int c = h == 0 ? std::min(hbad[4], hbad[5]) : hbad[5];
+      err = "singular K_ii: " + std::to_string(hbad[1]) + " zero and " + std::to_string(hbad[2]) +
+            " reduced-to-zero pivot(s) (tolFactorize " + std::to_string(opt_.tol_factorize) + ", singularityHandler " +
+            std::to_string(h) + "); first at equation " + std::to_string(y.perm_to_eq[c] + 1);
+      return -5;
+    }
   }
   // ---- component modes (block Lanczos, shift-invert on the resident factor)
   std::vector<double> Sres;

@@ -35,6 +35,12 @@ struct ReduceOptions {
   bool factor_lookahead = false;
   int solve_split = 128;    // longest K range of one solve GEMM task (longer ones are split and reduced)
   bool mixed = false;
+  // singular pivots (solExtensionModule.f90:716-760; reducerInterface.C:100,108):
+  // a pivot below tol_factorize |a_jj| is replaced in any case; with
+  // sing_handler 0 any such pivot is an error, with 1 only those reduced to
+  // near zero by the elimination (true zeros are suppressed), with > 1 none
+  double tol_factorize = 1e-12;
+  int sing_handler = 1;
   double ir_tol = 1e-14;
   int ir_maxit = 40;
   uint64_t seed = 0x5eed;  // Lanczos start block
@@ -59,6 +65,11 @@ struct Results {
   int lanczos_vectors = 0;
   double lanczos_resid = 0;        // worst relative Ritz residual at the last check
   double mass_fe = 0;              // sum of element masses
+  // pivots of the K_ii factor: negative ones, replaced true zeros and
+  // replaced reduced-to-zero ones; the replaced ones as (0-based equation,
+  // kind 1 true / 2 reduced), ascending
+  int n_negative = 0, n_true_zero = 0, n_reduced_zero = 0;
+  std::vector<std::pair<int, int>> singular;
 };
 
 class DeviceReducer {

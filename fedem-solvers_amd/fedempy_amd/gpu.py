@@ -55,6 +55,7 @@ def load_library(path=None):
     lib.fgpu_get_supernodes.argtypes = [P, P, P, P, P, P]
     lib.fgpu_get_perm.argtypes = [P, P]
     lib.fgpu_get_lanczos.argtypes = [P, P, P]
+    lib.fgpu_get_pivots.argtypes = [P, P, P, P, P, P, ctypes.c_int]
     _lib = lib
     return lib
 
@@ -142,6 +143,18 @@ class GpuReducer:
         r = ctypes.c_double(0)
         self._check(self.lib.fgpu_get_lanczos(self.h, ctypes.byref(n), ctypes.byref(r)), "fgpu_get_lanczos")
         return n.value, r.value
+
+    def pivots(self):
+        """Pivots of the last factorisation (also after a failed reduce):
+        (negative, true zero, reduced-to-zero counts), and per replaced pivot
+        (1-based node, local DOF, equation, kind 1 true / 2 reduced)."""
+        c = np.zeros(3, np.int32)
+        n = self.lib.fgpu_get_pivots(self.h, _ptr(c), None, None, None, None, 0)
+        if n < 0:
+            raise ReducerError("fgpu_get_pivots returned %d" % n)
+        a = [np.zeros(max(n, 1), np.int32) for _ in range(4)]
+        self.lib.fgpu_get_pivots(self.h, _ptr(c), *[_ptr(x) for x in a], n)
+        return tuple(int(v) for v in c), [tuple(int(x[k]) for x in a) for k in range(n)]
 
     def supernodes(self):
         inf = self.info()

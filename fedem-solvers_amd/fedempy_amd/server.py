@@ -89,6 +89,10 @@ class Server:
         return dict(SK=SK, SM=SM, G=G, Fred=Fr[:, :nlc], Ui=Ui[:, :nlc], ndof2=n2,
                     queue_ms=t[0], service_ms=t[1], latency_ms=t[2], footprint=t[3])
 
+    def device_bytes(self):
+        """device memory the slots' buffers hold now"""
+        return int(self.lib.fgpu_server_device_bytes(self.h))
+
     def release(self, rid):
         self.lib.fgpu_server_release(self.h, rid)
         self._keep.pop(rid, None)

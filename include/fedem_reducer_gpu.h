@@ -223,6 +223,17 @@ int fgpu_get_element_matrices(fgpu_reducer* h, double* Kp, double* Mp, int64_t* 
 /* Symbolic structure for host-side verification: per supernode col0, ncol,
  * nrow, parent; rows (nrow each, perm indices). */
 int fgpu_get_supernodes(const fgpu_reducer* h, int* col0, int* ncol, int* nrow, int* parent, int* rows);
+/* Pivots of the last factorisation of K_ii (also after fgpu_reduce failed
+ * on them): counts[0..2] = negative pivots, true zero pivots and pivots
+ * reduced to near zero, the last two replaced by |a_jj| (the reference's
+ * SPRFC3 modification, solExtensionModule.f90:725-747; the replacement value
+ * itself is this library's choice); for the replaced ones, in equation
+ * order, the 1-based node, local DOF, equation and kind (1 true, 2
+ * reduced) of at most max of them (reportSingEqns, cmstrs.f90:1324-1352).
+ * Returns how many were replaced.  Options "tolFactorize" (default 1e-12)
+ * and "singularityHandler" (0: any replaced pivot makes fgpu_reduce fail
+ * with -5; 1, the default: only reduced-to-zero ones do; > 1: none). */
+int fgpu_get_pivots(const fgpu_reducer* h, int* counts, int* node, int* ldof, int* eqn, int* kind, int max);
 /* Lanczos statistics of the last reduction: basis vectors used and the worst
  * relative Ritz residual ||K^-1 M phi - theta phi||_M / theta at the end. */
 int fgpu_get_lanczos(const fgpu_reducer* h, int* nvec, double* resid);

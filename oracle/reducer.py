@@ -304,10 +304,16 @@ def reduce(part, ngen=0, neval=0, want_vgi=False, dense_limit=3000, loads=None):
     Mii, Mib, Mbb = M[i1][:, i1], M[i1][:, i2], M[i2][:, i2].toarray()
     n1, n2 = len(i1), len(i2)
     if n1 <= dense_limit:
-        lu = sla.cho_factor(Kii.toarray())
-        solve = lambda R: sla.cho_solve(lu, R)
+        try:
+            lu = sla.cho_factor(Kii.toarray())
+            solve = lambda R: sla.cho_solve(lu, R)
+        except np.linalg.LinAlgError:  # indefinite K_ii: LU with partial pivoting
+            lu = sla.lu_factor(Kii.toarray())
+            solve = lambda R: sla.lu_solve(lu, R)
     else:
-        f = spla.splu(Kii.tocsc())
+        # K_ii is SPD: symmetric minimum-degree ordering, no pivoting
+        f = spla.splu(Kii.tocsc(), permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0,
+                      options={"SymmetricMode": True})
         solve = lambda R: f.solve(np.asarray(R))
     B = -solve(Kib.toarray()) if n1 else np.zeros((0, n2))
     S22 = Kbb + Kib.T @ B

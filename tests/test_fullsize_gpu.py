@@ -1,0 +1,59 @@
+"""The 48x24x24 HEX8 block (91,911 DOF, 72 boundary DOF, 20 modes) and the
+200x200 ANDES4 plate (242,406 DOF, 384 boundary DOF) at full size against
+the oracle's outputs for the same seeded parts (tests/golden/fullsize_*.npz,
+written by tests/golden/make_fullsize.py): SK, SM, G and B^T r for a fixed
+r at 1e-10 (FP64 path) and 1e-8 (mixed precision), eigenvalues at 1e-9."""
+import os
+
+import numpy as np
+import pytest
+
+from fedempy_amd import gpu, parts
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def probe(n1, seed=7):
+    """the r of the fixtures' B^T r (make_fullsize.probe)"""
+    return np.random.default_rng(seed).uniform(-1.0, 1.0, n1)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def check(R, f, tol, modes):
+    SK, SM, G = R.reduced()
+    B = R.B()
+    errs = {"SK": rel(SK, f["SK"]), "SM": rel(SM, f["SM"]), "G": rel(G, f["G"]),
+            "BTr": rel(B.T @ probe(B.shape[0]), f["BTr"])}
+    if modes:
+        lam, _ = R.eigen()
+        errs["lambda"] = float(np.max(np.abs(lam - f["lam"]) / f["lam"]))
+    print({k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= (1e-9 if k == "lambda" else tol), (k, v)
+
+
+def test_hex8_48x24x24_fp64():
+    f = np.load(os.path.join(HERE, "golden", "fullsize_hex8_48x24x24.npz"))
+    print("48x24x24 HEX8 block, 20 modes, FP64", flush=True)
+    R = gpu.GpuReducer()
+    R.set_part(parts.config1(seed=0))
+    R.analyze(ngen=20)
+    R.reduce()
+    check(R, f, 1e-10, True)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_plate_200x200(mixed):
+    f = np.load(os.path.join(HERE, "golden", "fullsize_plate200.npz"))
+    print("200x200 ANDES4 plate, %s" % ("mixed precision" if mixed else "FP64"), flush=True)
+    R = gpu.GpuReducer()
+    R.set_part(parts.config2(seed=0))
+    if mixed:
+        R.set_option("mixedPrecision", 1)
+    R.analyze()
+    R.reduce()
+    check(R, f, 1e-8 if mixed else 1e-10, False)

@@ -94,3 +94,69 @@ def test_server_heavy_mix_same_as_alone():
     for a, m in zip(alone, mixed):
         for k in ("SK", "SM", "G", "Fred", "Ui"):
             assert np.array_equal(a[k], m[k]), k
+
+
+def _alone_and_mixed(reqs, slots, budget_gb=200.0):
+    s1 = Server(slots=1, budget_gb=budget_gb)
+    alone = [s1.result(s1.submit(*r)) for r in reqs]
+    s1.close()
+    sn = Server(slots=slots, budget_gb=budget_gb)
+    ids = [sn.submit(*r) for r in reqs]
+    mixed = [sn.result(i) for i in ids]
+    held = sn.device_bytes()
+    sn.close()
+    for a, m in zip(alone, mixed):
+        for k in ("SK", "SM", "G", "Fred", "Ui"):
+            assert np.array_equal(a[k], m[k]), k
+    return alone, held
+
+
+def test_served_mix_at_the_300k_cap():
+    """The served distribution as configured (DOF lognormal, median 20k,
+    capped at 300k; 1-8 load cases with recovery): 24 requests, among them
+    parts above 100k DOF, bitwise the same alone and on 16 slots."""
+    rng = np.random.default_rng(11)
+    reqs = [parts.served_request(rng) for _ in range(24)]
+    dofs = [6 * len(r[0]["xyz"]) for r in reqs]
+    assert max(dofs) > 100000, max(dofs)
+    print("served mix at the 300k cap: largest %d DOF" % max(dofs), flush=True)
+    _alone_and_mixed(reqs, 16)
+
+
+def test_batched_256_parts_same_alone_and_oracle_sample():
+    """The 256-part batched mix (plates 10-80 squared, HEX8 blocks 6-24
+    cubed, beam frames 50-500 elements, 2-16 external nodes, 0-10 modes):
+    bitwise the same alone and on 16 slots, and the eight smallest parts
+    against the oracle at 1e-10."""
+    rng = np.random.default_rng(0)
+    reqs = []
+    for _ in range(256):
+        part, ngen = parts.random_part(rng)
+        reqs.append((part, ngen, None, None))
+    print("batched mix of 256 parts", flush=True)
+    alone, _ = _alone_and_mixed(reqs, 16)
+    order = np.argsort([len(r[0]["xyz"]) for r in reqs])[:8]
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    for k in order:
+        part, ngen, _, _ = reqs[k]
+        o = orc.reduce(part, ngen=ngen)
+        n2 = alone[k]["ndof2"]
+        assert rel(alone[k]["SK"][:n2, :n2], o["SK"][:n2, :n2]) <= 1e-10
+        assert rel(alone[k]["G"][:n2], o["G"][:n2]) <= 1e-10
+        if ngen == 0:
+            assert rel(alone[k]["SM"], o["SM"]) <= 1e-10
+
+
+def test_tight_budget_releases_memory():
+    """A 1 GB budget with one large request among small ones: the large one
+    runs alone, slots holding more than their share give their buffers back,
+    results stay bitwise the same, and the slots end within the budget."""
+    rng = np.random.default_rng(5)
+    big = parts.hex_block(30, 16, 16, 0.3, 0.16, 0.16, seed=2, n_ext=8)
+    reqs = [(big, 4, parts.random_loads(rng, big, 2), None)]
+    for _ in range(8):
+        part, ngen = parts.random_part(rng, target_dof=4000)
+        reqs.append((part, ngen, parts.random_loads(rng, part, 1), None))
+    print("tight budget: one %d-DOF request among 8 small ones" % (3 * len(big["xyz"])), flush=True)
+    _, held = _alone_and_mixed(reqs, 4, budget_gb=1.0)
+    assert held <= 1.0 * 2**30, held
