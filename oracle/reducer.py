@@ -290,12 +290,33 @@ def modes(Kii, Mii, nev, dense_limit=3000, solve=None):
     return w, fix_signs(V)
 
 
-def reduce(part, ngen=0, neval=0, want_vgi=False, dense_limit=3000, loads=None):
+def refine_extended(Kii, R, X, solve, steps):
+    """steps of iterative refinement of X ~ Kii^-1 R with the residual
+    R - Kii X accumulated in extended precision (numpy longdouble, 64-bit
+    mantissa on x86): the solution's error falls to about the rounding of
+    its own entries, well below what any FP64 factorisation of an
+    ill-conditioned K_ii gives by itself."""
+    K = Kii.tocsr()
+    data = K.data.astype(np.longdouble)
+    rows = np.repeat(np.arange(K.shape[0]), np.diff(K.indptr))
+    for _ in range(steps):
+        Res = np.empty_like(X)
+        for j in range(X.shape[1]):
+            prod = data * X[K.indices, j].astype(np.longdouble)
+            acc = np.zeros(K.shape[0], np.longdouble)
+            np.add.at(acc, rows, prod)
+            Res[:, j] = (R[:, j].astype(np.longdouble) - acc).astype(np.float64)
+        X = X + solve(Res)
+    return X
+
+
+def reduce(part, ngen=0, neval=0, want_vgi=False, dense_limit=3000, loads=None, refine=0):
     """Reduced SK, SM, G (CMSTRS/GRAV layout), B, modes and vgi of a part.
 
     loads: full load vectors (neq x nrhs, equation order); out["L"] is then
     the reduced load matrix of cmstrs.f90:598-640,763-782,951-955:
-    rows dofPosIn2 of f_e + B^T f_i, then Phi^T f_i, zero padded to ndim."""
+    rows dofPosIn2 of f_e + B^T f_i, then Phi^T f_i, zero padded to ndim.
+    refine: extended-precision refinement steps of B (refine_extended)."""
     s = sam_numbering(part)
     K, M, emass = assemble(part, s)
     i1 = s["meqn1"] - 1
@@ -316,6 +337,8 @@ def reduce(part, ngen=0, neval=0, want_vgi=False, dense_limit=3000, loads=None):
                       options={"SymmetricMode": True})
         solve = lambda R: f.solve(np.asarray(R))
     B = -solve(Kib.toarray()) if n1 else np.zeros((0, n2))
+    if refine and n1:
+        B = refine_extended(Kii, -Kib.toarray(), B, solve, refine)
     S22 = Kbb + Kib.T @ B
     S22 = 0.5 * (S22 + S22.T)
     nev = max(neval, ngen) if ngen > 0 else 0

@@ -3,7 +3,10 @@
 external nodes, 20 component modes) and of the 200x200 ANDES4 plate
 (seed 0, 64 edge nodes, 384 boundary DOFs): the oracle's reduced SK, SM, G,
 the eigenvalues, and B^T r for a fixed seeded r (B itself is 53 MB and
-743 MB).  The oracle needs minutes at these sizes, so its outputs are
+743 MB).  B is refined twice with extended-precision residuals
+(oracle.reducer.refine_extended): the thin plate's K_ii is ill-conditioned
+enough (about 1e10) that two FP64 factorisations in different orders
+disagree by 3e-10 in the reduced mass, the refined B by less than 1e-14.  The oracle needs minutes at these sizes, so its outputs are
 committed as data (fullsize_<name>.npz) and the GPU tests compare with
 them.  Run from the repository root: python tests/golden/make_fullsize.py"""
 import os
@@ -27,7 +30,7 @@ def probe(n1, seed=7):
 
 def write(name, part, ngen):
     t = time.time()
-    o = orc.reduce(part, ngen=ngen, dense_limit=0)
+    o = orc.reduce(part, ngen=ngen, dense_limit=0, refine=2)
     r = probe(o["B"].shape[0])
     out = dict(SK=o["SK"], SM=o["SM"], G=o["G"], BTr=o["B"].T @ r)
     if ngen:
