@@ -280,6 +280,13 @@ int fgpu_reduce_loads(fgpu_reducer* h, int nrhs, const double* F, double* Fred) 
   END_GUARD(h)
 }
 
+int fgpu_reduce_loads_noalloc(fgpu_reducer* h, int nrhs, const double* F, double* Fred) {
+  GUARD(h)
+  if (!h->reduced || nrhs <= 0 || !F || !Fred) return -1;
+  return h->red.reduce_loads_noalloc(nrhs, F, Fred, h->err);
+  END_GUARD(h)
+}
+
 int fgpu_recover(fgpu_reducer* h, int nrhs, const double* Ub, const double* Fi, double* Ui) {
   GUARD(h)
   if (!h->reduced || nrhs <= 0 || !Ui) return -1;

@@ -1,5 +1,6 @@
 // Host-callable launchers of the gfx950 kernels (kernels.hip).
 #pragma once
+#include <string>
 #include "fgpu.hpp"
 #include "kern_common.hpp"
 
@@ -40,6 +41,13 @@ void launch_fmap(hipStream_t st, int neq, const int* rowptr, const int* colind, 
 int scan_tiles(int n);
 void launch_exclusive_scan(hipStream_t st, int n, const int* in, int* out, int* tsum);
 void launch_fill(hipStream_t st, double* p, int64_t n, double v);
+// no-allocation reduced loads (noalloc.hip): this GPU's scratch array
+// (address and size), the permuted-row gather of column-major load vectors
+// and the scatter of the reduced rows into Fred
+int noalloc_scratch(double** ptr, size_t* bytes, std::string& err);
+void launch_perm_rows(hipStream_t st, int neq, int nrhs, const int* p2e, const double* F, double* Fp);
+void launch_scatter_reduced(hipStream_t st, int nd, int n2, int ngen, int nrhs, const int* pos2, const double* r2,
+                            const double* r1, double* Fred);
 // singular-pivot support of the factorisation: the original (or, with rs,
 // equilibrated) diagonal of the first n permuted equations and its largest
 // magnitude (amax must start at 0); the pivot statistics

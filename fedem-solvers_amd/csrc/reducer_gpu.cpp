@@ -174,6 +174,7 @@ struct DeviceReducer::Impl {
   // node first equation / equation count / neighbour lists (+ DOF offsets),
   // per row its node, per column its front
   DBuf<int> madof, msc, nndof, dep_indep, drank, etype, nfirst, neqd, nbptr, nbl, nbo, rnode, colsn;
+  DBuf<int> p2e, pos2;  // perm row -> 0-based equation; dofPosIn2 (for the device-side reduced loads)
   DevSym S{};
   DBuf<FTask> ftasks;
   DBuf<int> xrange;  // extend-add tasks: (child, first, end) column ranges
@@ -763,6 +764,8 @@ int DeviceReducer::upload(std::string& err) {
       if (sam_.meqn[dof] > 0) a[(size_t)y.eq_to_perm[sam_.meqn[dof] - 1] * 3 + j] = 1.0;
     }
   HIPCHK(d.grav.upload(a));
+  HIPCHK(d.p2e.upload(y.perm_to_eq));
+  HIPCHK(d.pos2.upload(sam_.dofPosIn2.empty() ? std::vector<int>(1, 1) : sam_.dofPosIn2));
   uploaded_ = true;
   times_.upload = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
@@ -1705,6 +1708,37 @@ int DeviceReducer::reduce_loads(int nrhs, const double* F_eq, std::vector<double
     for (int a = 0; a < ngen; a++) out[(n2 + a) + (size_t)c * nd] = r1[(size_t)a * nrhs + c];
   }
   (void)err;
+  return 0;
+}
+
+// reduce_loads on device buffers, with every work buffer in the library's
+// static scratch array: no device allocation.  F (neq x nrhs) and Fred
+// (ndim x nrhs) are column-major device arrays.
+int DeviceReducer::reduce_loads_noalloc(int nrhs, const double* F, double* Fred, std::string& err) {
+  Impl& d = *d_;
+  const Symbolic& y = sym_;
+  const int neq = y.neq, n2 = y.ndof2, ngen = res_.ngen, nev = res_.nev, nd = res_.ndim;
+  double* scr = nullptr;
+  size_t cap = 0;
+  if (int rc = noalloc_scratch(&scr, &cap, err)) return rc;
+  const size_t nF = (size_t)neq * nrhs, maxa = (size_t)std::max(n2, ngen);
+  const size_t npart = (size_t)xty_chunks(neq) * maxa * nrhs;
+  const size_t need = 8 * (nF + npart + (size_t)(n2 + ngen) * nrhs);
+  if (need > cap) {
+    err = "reduced loads without allocation: " + std::to_string(nrhs) + " load cases need " + std::to_string(need) +
+          " bytes of scratch, more than the " + std::to_string(cap) + " available";
+    return -1;
+  }
+  double* Fp = scr;
+  double* part = Fp + nF;
+  double* r2 = part + npart;
+  double* r1 = r2 + (size_t)n2 * nrhs;
+  launch_perm_rows(st_, neq, nrhs, d.p2e.p, F, Fp);
+  if (n2 > 0) launch_xty(st_, neq, n2, nrhs, d.X.p, n2, Fp, nrhs, part, r2);
+  if (ngen > 0) launch_xty(st_, neq, ngen, nrhs, d.Phi.p, nev, Fp, nrhs, part, r1);
+  launch_scatter_reduced(st_, nd, n2, ngen, nrhs, d.pos2.p, r2, r1, Fred);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st_));
   return 0;
 }
 

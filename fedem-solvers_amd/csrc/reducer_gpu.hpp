@@ -97,6 +97,9 @@ class DeviceReducer {
   // reduced load vectors [f_e + B^T f_i (dofPosIn2 order); Phi^T f_i], ndim x nrhs,
   // from full load vectors F_eq (neq x nrhs, equation order)
   int reduce_loads(int nrhs, const double* F_eq, std::vector<double>& out, std::string& err);
+  // the same on device arrays (F neq x nrhs, Fred ndim x nrhs, column-major)
+  // with the work buffers in the library's static scratch array
+  int reduce_loads_noalloc(int nrhs, const double* F_dev, double* Fred_dev, std::string& err);
   // internal displacements from boundary displacements (and internal loads):
   // U_i = B U_b (+ K_ii^{-1} F_i); U_b ndof2 x nrhs (MEQN2 rows), F_i and U_i
   // ndof1 x nrhs (MEQN1 rows), F_i may be null

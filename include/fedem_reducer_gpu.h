@@ -160,6 +160,15 @@ int fgpu_get_gravity_internal(fgpu_reducer* h, double* gi);
  * content. */
 int fgpu_reduce_loads(fgpu_reducer* h, int nrhs, const double* F, double* Fred);
 
+/* fgpu_reduce_loads on device memory, allocating nothing: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  F (neq x nrhs) and Fred (ndim x
+ * nrhs) are column-major device arrays; the work buffers live in a static
+ * 32 MiB device array that each GPU holds until the process ends.  Returns
+ * -1 when nrhs needs more than that; synchronous on the handle's stream.
+ * Calls that overlap on one GPU share that array and race. */
+int fgpu_reduce_loads_noalloc(fgpu_reducer* h, int nrhs, const double* F, double* Fred);
+
 /* Internal displacements on the resident factor and B (displacement
  * recovery, displacementModule.f90:555-578): Ui = B Ub + K_ii^-1 Fi, Ub
  * ndof2 x nrhs (MEQN2 rows, may be NULL), Fi and Ui ndof1 x nrhs (MEQN1
