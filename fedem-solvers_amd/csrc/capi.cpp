@@ -104,6 +104,44 @@ int fgpu_set_rigid(fgpu_reducer* h, int nrgd, const int* indep, const int* dofma
   END_GUARD(h)
 }
 
+int fgpu_set_wavgm(fgpu_reducer* h, int nwv, const int* ref, const int* refc, const int* indc, const int* nptr,
+                   const int* nodes, const int* wptr, const double* weights) {
+  GUARD(h)
+  if (!h->have_part || nwv < 0 || (nwv > 0 && (!ref || !refc || !indc || !nptr || !nodes || !wptr))) {
+    h->err = "fgpu_set_wavgm: invalid arguments (set the part first)";
+    return -1;
+  }
+  fgpu::Part& p = h->part;
+  p.wv_ref.resize(nwv);
+  p.wv_refc.assign(refc, refc + nwv);
+  p.wv_indc.assign(indc, indc + 6 * (size_t)nwv);
+  p.wv_nptr.resize(nwv + 1);
+  p.wv_wptr.resize(nwv + 1);
+  for (int w = 0; w <= nwv; w++) {
+    p.wv_nptr[w] = nwv ? nptr[w] - 1 : 0;
+    p.wv_wptr[w] = nwv ? wptr[w] - 1 : 0;
+  }
+  for (int w = 0; w < nwv; w++) {
+    p.wv_ref[w] = ref[w] - 1;
+    if (p.wv_nptr[w + 1] < p.wv_nptr[w] || p.wv_wptr[w + 1] < p.wv_wptr[w] || p.wv_nptr[0] != 0 ||
+        p.wv_wptr[0] != 0) {
+      h->err = "fgpu_set_wavgm: bad pointers";
+      return -2;
+    }
+  }
+  p.wv_nodes.resize(nwv ? p.wv_nptr[nwv] : 0);
+  for (size_t k = 0; k < p.wv_nodes.size(); k++) p.wv_nodes[k] = nodes[k] - 1;
+  const size_t nw = nwv ? p.wv_wptr[nwv] : 0;
+  if (nw && !weights) {
+    h->err = "fgpu_set_wavgm: weights missing";
+    return -1;
+  }
+  p.wv_w.assign(weights, weights + nw);
+  h->analyzed = h->reduced = false;
+  return 0;
+  END_GUARD(h)
+}
+
 int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags) {
   GUARD(h)
   if (!h->have_part) { h->err = "fgpu_analyze: no part"; return -1; }
@@ -157,6 +195,29 @@ int fgpu_get_sam(const fgpu_reducer* h, int* madof, int* msc, int* meqn, int* me
   const fgpu::Sam& m = h->red.sam();
   auto cp = [](int* dst, const std::vector<int>& v) { if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(int)); };
   cp(madof, m.madof); cp(msc, m.msc); cp(meqn, m.meqn); cp(meqn1, m.meqn1); cp(meqn2, m.meqn2); cp(pos2, m.dofPosIn2);
+  return 0;
+}
+
+int fgpu_get_constraints(const fgpu_reducer* h, int* sizes, int* mpmceq, int* mmceq, double* ttcc) {
+  if (!h || !h->analyzed || !sizes) return -1;
+  const fgpu::Sam& m = h->red.sam();
+  int nceq = 0, nmm = 0;
+  for (int d = 0; d < m.ndof; d++)
+    if (m.msc[d] == -1) {
+      if (mpmceq) mpmceq[nceq] = nmm + 1;
+      if (mmceq) mmceq[nmm] = d + 1;
+      if (ttcc) ttcc[nmm] = 0.0;
+      nmm++;
+      for (int k = m.ceq_ptr[d]; k < m.ceq_ptr[d + 1]; k++) {
+        if (mmceq) mmceq[nmm] = m.ceq_dof[k] + 1;
+        if (ttcc) ttcc[nmm] = m.ceq_coef[k];
+        nmm++;
+      }
+      nceq++;
+    }
+  if (mpmceq) mpmceq[nceq] = nmm + 1;
+  sizes[0] = nceq;
+  sizes[1] = nmm;
   return 0;
 }
 

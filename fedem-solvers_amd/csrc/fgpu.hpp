@@ -52,6 +52,21 @@ struct Part {
   std::vector<int> rgd_mask;   // bitmask over the six local DOFs
   std::vector<int> rgd_ptr;    // nrgd+1 pointers into rgd_dep (0-based)
   std::vector<int> rgd_dep;    // 0-based dependent nodes
+  // weighted-average-motion elements (WAVGM / RBE3): element w ties the
+  // components refc (digits 1..6, 0 = all) of reference node wv_ref[w] to
+  // its independent nodes wv_nodes[wv_nptr[w] .. wv_nptr[w+1]); wv_indc[6w+c]
+  // is the 1-based start of component c's weights within the element's
+  // weights wv_w[wv_wptr[w] .. wv_wptr[w+1]) (< 0 uniform, 0 no coupling);
+  // an element without weights takes refc 123456, uniform translations
+  std::vector<int> wv_ref, wv_refc, wv_nptr, wv_nodes, wv_indc, wv_wptr;
+  std::vector<double> wv_w;
+};
+
+// One constraint equation of a WAVGM element: u(dep) = sum coef[k] u(dof[k])
+struct WavgmEquation {
+  int dep = -1;                  // 0-based dependent DOF
+  std::vector<int> node, dof;    // independent node and 0-based DOF of each term
+  std::vector<double> coef;
 };
 
 // SAM arrays (samModule.f90:27-64), 0-based storage of 1-based values where
@@ -59,16 +74,27 @@ struct Part {
 struct Sam {
   int nnod = 0, ndof = 0, neq = 0, ndof1 = 0, ndof2 = 0;
   std::vector<int> madof;      // nnod+1, 1-based accumulated DOF pointers
-  std::vector<int> msc;        // ndof, 0 fixed / 1 internal / 2 external / -1 dependent (rigid element)
+  std::vector<int> msc;        // ndof, 0 fixed / 1 internal / 2 external / -1 dependent (constraint element)
   std::vector<int> meqn;       // ndof, 0 for fixed, else 1-based equation
   std::vector<int> meqn1, meqn2, dofPosIn2;
   std::vector<int> nndof;      // per node
-  std::vector<int> dep_indep;  // ndof: independent node of a dependent DOF, else -1
+  // constraint equations of the dependent DOFs (msc -1): DOF d follows
+  // sum ceq_coef[k] u(ceq_dof[k]) over k in [ceq_ptr[d], ceq_ptr[d+1]),
+  // ceq_node[k] being the node of DOF ceq_dof[k]; every ceq_dof[k] has an
+  // equation
+  std::vector<int> ceq_ptr, ceq_node, ceq_dof;
+  std::vector<double> ceq_coef;
   int nenod = 0;               // number of external nodes
   int ndim_ext = 0;            // sum of DOFs of external nodes
 };
 
 int build_sam(const Part& p, Sam& s, std::string& err);
+
+// Equations of the part's WAVGM elements on the nodal status codes msc
+// (1 internal, 2 external, 0 fixed), in element order; msc of a dependent
+// DOF becomes -3 (or 0 when its coupled DOFs are fixed).
+int wavgm_equations(const Part& p, const std::vector<int>& madof, const std::vector<int>& nndof,
+                    std::vector<int>& msc, std::vector<WavgmEquation>& eqs, std::string& err);
 
 // ---------------------------------------------------------------- symbolic
 struct Symbolic {

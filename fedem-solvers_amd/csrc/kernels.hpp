@@ -32,9 +32,10 @@ void launch_nbr_sort(hipStream_t st, int nnod, const int* nbptr, int* nbl, const
 void launch_csr_cols(hipStream_t st, int neq, const int* rnode, const int* rowptr, const int* nbptr, const int* nbl,
                      const int* nbo, const int* nfirst, const int* neqd, int* colind);
 void launch_contrib(hipStream_t st, int pass, int nel, const int* etype, const int* eptr, const int* enodes,
-                    const int64_t* eoff, const int* madof, const int* msc, const int* nndof, const int* dep_indep,
-                    const int* drank, const int* nfirst, const double* xyz, const int* rowptr, const int* nbptr,
-                    const int* nbl, const int* nbo, int* cnt, const int* cptr, int* cidx, double* ccoef, int* bad);
+                    const int64_t* eoff, const int* madof, const int* msc, const int* nndof, const int* ceq_ptr,
+                    const int* ceq_node, const int* ceq_dof, const double* ceq_coef, const int* drank,
+                    const int* nfirst, const int* rowptr, const int* nbptr, const int* nbl, const int* nbo, int* cnt,
+                    const int* cptr, int* cidx, double* ccoef, int* bad);
 void launch_seg_sort(hipStream_t st, int nnz, const int* cptr, int* cidx, double* ccoef);
 void launch_fmap(hipStream_t st, int neq, const int* rowptr, const int* colind, const int* colsn, const DevSym& S,
                  int64_t* fmap, int* bad);

@@ -2,7 +2,7 @@
 // reference parses parts with FFl (absent fedem-foundation); this reader
 // covers the cards of the element types on the device path: GRID, CHEXA,
 // CQUAD4, CBAR, MAT1, PSHELL, PSOLID, PBAR, PBARL (TUBE), ASET/ASET1, SPC1 (both with
-// "G1 THRU G2" ranges), RBE2 and the
+// "G1 THRU G2" ranges), RBE2, RBE3 and the
 // FORCE/MOMENT nodal loads (basic coordinate system only), in
 // free (comma), small (8 column) and large (16 column, '*') field formats.
 // Nodes are numbered in ascending GRID id order, as SAM numbers them.
@@ -156,6 +156,16 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
   std::vector<int> aset, aset_range;  // listed ASET grids; grids inside ASET1 THRU ranges
   std::map<int, int> spc;  // grid -> fixed-DOF bitmask
   std::vector<std::pair<int, std::pair<std::string, std::vector<int>>>> rbe2;
+  struct R3Group {
+    double wt;
+    int comps;  // bitmask over the six components
+    std::vector<int> grids;
+  };
+  struct R3 {
+    int eid = 0, ref = 0, refc = 0;
+    std::vector<R3Group> groups;
+  };
+  std::vector<R3> rbe3;
   struct Load { int sid, grid, dof0; double v[3]; };
   std::vector<Load> loads;
   auto I = [](const std::vector<std::string>& f, size_t k) -> int {
@@ -235,6 +245,40 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
       if (I(f, 2) != 0) { err = c.name + ": only the basic coordinate system (CID 0) is supported"; return -2; }
       double F = D(f, 3);
       loads.push_back({I(f, 0), I(f, 1), c.name == "FORCE" ? 0 : 3, {F * D(f, 4), F * D(f, 5), F * D(f, 6)}});
+    } else if (c.name == "RBE3") {
+      // EID, blank, REFGRID, REFC, then groups WT, C, G...; a real field
+      // starts a group.  The UM (extra dependent DOFs) section is not
+      // supported; ALPHA (thermal expansion) does not enter the reduction.
+      R3 r;
+      r.eid = I(f, 0);
+      r.ref = I(f, 2);
+      r.refc = I(f, 3);
+      int state = 0;  // 0 expect weight, 1 expect components, 2 grids
+      for (size_t k = 4; k < f.size(); k++) {
+        std::string t = trim(f[k]);
+        if (t.empty()) continue;
+        std::string up = t;
+        for (char& ch : up) ch = (char)std::toupper(ch);
+        if (up == "UM") { err = "RBE3 " + std::to_string(r.eid) + ": UM dependent DOFs are not supported"; return -2; }
+        if (up == "ALPHA" || up == "TREF") break;
+        int iv = 0;
+        double dv = 0.0;
+        const bool is_int = to_int(t, iv);
+        if (!is_int && to_double(t, dv)) {
+          r.groups.push_back({dv, 0, {}});
+          state = 1;
+        } else if (is_int && state == 1) {
+          for (char ch : t)
+            if (ch >= '1' && ch <= '6') r.groups.back().comps |= 1 << (ch - '1');
+          state = 2;
+        } else if (is_int && state == 2) {
+          r.groups.back().grids.push_back(iv);
+        } else {
+          err = "RBE3 " + std::to_string(r.eid) + ": unexpected field '" + t + "'";
+          return -2;
+        }
+      }
+      rbe3.push_back(r);
     } else if (c.name == "RBE2") {
       std::vector<int> dep;
       for (size_t k = 3; k < f.size(); k++)
@@ -299,6 +343,45 @@ int read_nastran(const std::string& path, NastranPart& out, std::string& err) {
       c.dep.push_back(idx[g]);
     }
     np.rbe2.push_back(c);
+  }
+  for (auto& r : rbe3) {
+    const std::string tag = "RBE3 " + std::to_string(r.eid);
+    if (!idx.count(r.ref)) { err = tag + ": undefined reference grid"; return -2; }
+    Rbe3 c;
+    c.ref = idx[r.ref];
+    c.refc = r.refc;
+    // independent nodes in order of first appearance; per component the
+    // weight of each node (summed over the groups that list it with that
+    // component), components that share weights share one weight set
+    std::map<int, int> pos;
+    for (auto& g : r.groups)
+      for (int gid : g.grids) {
+        if (!idx.count(gid)) { err = tag + ": undefined independent grid"; return -2; }
+        if (!pos.count(gid)) {
+          pos[gid] = (int)c.nodes.size();
+          c.nodes.push_back(idx[gid]);
+        }
+      }
+    const int nM = (int)c.nodes.size();
+    if (nM == 0) { err = tag + ": no independent grids"; return -2; }
+    for (int comp = 0; comp < 6; comp++) {
+      std::vector<double> w(nM, 0.0);
+      bool any = false;
+      for (auto& g : r.groups)
+        if ((g.comps >> comp) & 1)
+          for (int gid : g.grids) {
+            w[pos[gid]] += g.wt;
+            any = true;
+          }
+      if (!any) continue;
+      for (int prev = 0; prev < comp && !c.indc[comp]; prev++)
+        if (c.indc[prev] > 0 && std::equal(w.begin(), w.end(), c.w.begin() + c.indc[prev] - 1)) c.indc[comp] = c.indc[prev];
+      if (!c.indc[comp]) {
+        c.indc[comp] = (int)c.w.size() + 1;
+        c.w.insert(c.w.end(), w.begin(), w.end());
+      }
+    }
+    np.rbe3.push_back(std::move(c));
   }
   for (auto& l : loads) {
     if (!idx.count(l.grid)) { err = "load on undefined grid " + std::to_string(l.grid); return -2; }

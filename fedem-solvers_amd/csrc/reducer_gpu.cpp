@@ -173,7 +173,9 @@ struct DeviceReducer::Impl {
   // inputs of the device structure build: SAM arrays, element types, per
   // node first equation / equation count / neighbour lists (+ DOF offsets),
   // per row its node, per column its front
-  DBuf<int> madof, msc, nndof, dep_indep, drank, etype, nfirst, neqd, nbptr, nbl, nbo, rnode, colsn;
+  DBuf<int> madof, msc, nndof, drank, etype, nfirst, neqd, nbptr, nbl, nbo, rnode, colsn;
+  DBuf<int> ceq_ptr, ceq_node, ceq_dof;  // constraint equations of the dependent DOFs
+  DBuf<double> ceq_coef;
   DBuf<int> p2e, pos2;  // perm row -> 0-based equation; dofPosIn2 (for the device-side reduced loads)
   DevSym S{};
   DBuf<FTask> ftasks;
@@ -613,7 +615,10 @@ int DeviceReducer::build_structure(std::string& err) {
   HIPCHK(d.madof.upload(sam_.madof));
   HIPCHK(d.msc.upload(sam_.msc));
   HIPCHK(d.nndof.upload(sam_.nndof));
-  HIPCHK(d.dep_indep.upload(sam_.dep_indep));
+  HIPCHK(d.ceq_ptr.upload(sam_.ceq_ptr));
+  HIPCHK(d.ceq_node.upload(sam_.ceq_node));
+  HIPCHK(d.ceq_dof.upload(sam_.ceq_dof));
+  HIPCHK(d.ceq_coef.upload(sam_.ceq_coef));
   HIPCHK(d.drank.upload(y.drank));
   HIPCHK(d.etype.upload(part_.etype));
   HIPCHK(d.nfirst.upload(y.nfirst));
@@ -642,8 +647,8 @@ int DeviceReducer::build_structure(std::string& err) {
   for (int pass = 0; pass < 2; pass++) {
     HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(int) * (size_t)nnz, st));
     launch_contrib(st, pass, part_.nel, d.etype.p, d.eptr.p, d.enodes.p, d.eoff.p, d.madof.p, d.msc.p, d.nndof.p,
-                   d.dep_indep.p, d.drank.p, d.nfirst.p, d.xyz.p, d.rowptr.p, d.nbptr.p, d.nbl.p, d.nbo.p, cnt.p,
-                   d.cptr.p, d.cidx.p, ccoef, bad.p);
+                   d.ceq_ptr.p, d.ceq_node.p, d.ceq_dof.p, d.ceq_coef.p, d.drank.p, d.nfirst.p, d.rowptr.p, d.nbptr.p,
+                   d.nbl.p, d.nbo.p, cnt.p, d.cptr.p, d.cidx.p, ccoef, bad.p);
     if (pass == 0) launch_exclusive_scan(st, nnz, cnt.p, d.cptr.p, tsum.p);
   }
   launch_seg_sort(st, nnz, d.cptr.p, d.cidx.p, ccoef);
@@ -1848,6 +1853,7 @@ size_t DeviceReducer::footprint_bytes() const {
   size_t b = fb * ((size_t)y.fpool + ipool) + 8 * (2 * nnz + 2 * (size_t)y.elem_pool);  // fronts, inverses, K, M, elements
   b += (y.has_rigid ? 12 : 4) * (size_t)y.ncontrib + 4 * (nnz + 1) + 8 * nnz + 4 * nnz;  // contributions, fmap, colind
   b += 8 * nnz;                                                                     // structure-build counts
+  b += 20 * sam_.ceq_dof.size() + 4 * sam_.ceq_ptr.size();                          // constraint equations
   b += 8 * (size_t)y.neq * (3 * (size_t)y.ndof2 + 2 * mcap + 8 * (size_t)bs);     // B solves and Lanczos basis
   b += 8 * 2 * wrows * rhs;                                                        // solve work rows
   if (opt_.mixed) b += 8 * 3 * (size_t)y.neq * rhs;                                // refinement work

@@ -306,6 +306,21 @@ int fgpu_link_attach(fgpu_reducer* h) {
     }
     rc = fgpu_set_rigid(h, (int)indep.size(), indep.data(), mask.data(), dptr.data(), dep.data());
   }
+  if (rc == 0 && !L.np.rbe3.empty()) {
+    std::vector<int> ref, refc, indc, nptr(1, 1), nodes, wptr(1, 1);
+    std::vector<double> w;
+    for (const auto& r : L.np.rbe3) {
+      ref.push_back(r.ref + 1);
+      refc.push_back(r.refc);
+      indc.insert(indc.end(), r.indc, r.indc + 6);
+      for (int u : r.nodes) nodes.push_back(u + 1);
+      nptr.push_back((int)nodes.size() + 1);
+      w.insert(w.end(), r.w.begin(), r.w.end());
+      wptr.push_back((int)w.size() + 1);
+    }
+    rc = fgpu_set_wavgm(h, (int)ref.size(), ref.data(), refc.data(), indc.data(), nptr.data(), nodes.data(),
+                        wptr.data(), w.data());
+  }
   if (rc) L.err = fgpu_last_error(h);
   return rc;
 }
@@ -321,6 +336,18 @@ int fgpu_link_checksum(int ngen, int neval, double tol_eig, double tol_fac) {
   f.add(p.etype.data(), p.etype.size() * sizeof(int));
   f.add(p.enodes.data(), p.enodes.size() * sizeof(int));
   f.add(p.eprop.data(), p.eprop.size() * sizeof(double));
+  for (const auto& r : L.np.rbe2) {
+    f.add(&r.indep, sizeof r.indep);
+    f.add(&r.dofs, sizeof r.dofs);
+    f.add(r.dep.data(), r.dep.size() * sizeof(int));
+  }
+  for (const auto& r : L.np.rbe3) {
+    f.add(&r.ref, sizeof r.ref);
+    f.add(&r.refc, sizeof r.refc);
+    f.add(r.indc, sizeof r.indc);
+    f.add(r.nodes.data(), r.nodes.size() * sizeof(int));
+    f.add(r.w.data(), r.w.size() * sizeof(double));
+  }
   f.add(&ngen, sizeof ngen);
   f.add(&neval, sizeof neval);
   f.add(&tol_eig, sizeof tol_eig);

@@ -1,7 +1,8 @@
-// SAM DOF numbering for a part: MADOF, MSC, MEQN, MEQN1, MEQN2, dofPosIn2.
+// SAM DOF numbering for a part: MADOF, MSC, MEQN, MEQN1, MEQN2, dofPosIn2,
+// and the constraint equations of the dependent DOFs.
 //
 // Reference: samReducerModule.f90:227-569 (initiateSAM, nodal arrays from
-// FFl), asmExtensionModule.f90:365-377 (SPRPRP turns unconnected internal
+// FFl, constraint elements), asmExtensionModule.f90:365-377 (SPRPRP turns unconnected internal
 // DOFs into fixed ones), asmExtensionModule.f90:910-980 (csCreateEqPartition)
 // and samModule.f90:892-991 (invertEqPartition).
 //
@@ -38,11 +39,20 @@ int build_sam(const Part& p, Sam& s, std::string& err) {
     s.madof[i + 1] = s.madof[i] + nd;
   }
   s.ndof = s.madof[p.nnod] - 1;
+  // nodal status codes: 0 fixed, 1 internal, 2 external
   s.msc.assign(s.ndof, 0);
-  // rigid elements: dependent DOFs carry no equation (RGD elements,
-  // samReducerModule.f90:385-459); their independent nodes keep all DOFs
-  s.dep_indep.assign(s.ndof, -1);
-  std::vector<char> indep(p.nnod, 0);
+  for (int i = 0; i < p.nnod; i++) {
+    const int st = p.status[i];
+    if (st == 1) { s.nenod++; s.ndim_ext += s.nndof[i]; }
+    for (int j = 0; j < s.nndof[i]; j++)
+      s.msc[s.madof[i] - 1 + j] = (st < 0 && ((-st) >> j) & 1) ? 0 : st == 1 ? 2 : 1;
+  }
+  // constraint elements, WAVGM (RBE3) first on the nodal status codes, then
+  // the rigid elements (samReducerModule.f90:385-459)
+  std::vector<WavgmEquation> weq;
+  if (wavgm_equations(p, s.madof, s.nndof, s.msc, weq, err)) return -1;
+  std::vector<int> rgd_of(s.ndof, -1);  // independent node of a rigid element's dependent DOF
+  std::vector<char> indep(p.nnod, 0);   // independent nodes of constraint elements
   const int nrgd = (int)p.rgd_indep.size();
   for (int r = 0; r < nrgd; r++) {
     int g = p.rgd_indep[r];
@@ -58,27 +68,71 @@ int build_sam(const Part& p, Sam& s, std::string& err) {
       for (int j = 0; j < s.nndof[u]; j++)
         if ((p.rgd_mask[r] >> j) & 1) {
           int d = s.madof[u] - 1 + j;
-          if (s.dep_indep[d] >= 0) { err = "DOF constrained by two rigid elements"; return -1; }
-          s.dep_indep[d] = g;
+          if (rgd_of[d] >= 0 || s.msc[d] < 0) { err = "DOF constrained by two constraint elements"; return -1; }
+          rgd_of[d] = g;
         }
     }
   }
-  for (int i = 0; i < p.nnod; i++) {
-    int st = p.status[i];
-    if (st == 1) { s.nenod++; s.ndim_ext += s.nndof[i]; }
+  for (int w = 0; w < (int)p.wv_ref.size(); w++)
+    for (int k = p.wv_nptr[w]; k < p.wv_nptr[w + 1]; k++) indep[p.wv_nodes[k]] = 1;
+  for (int i = 0; i < p.nnod; i++)
     for (int j = 0; j < s.nndof[i]; j++) {
-      int idof = s.madof[i] - 1 + j;
-      int m;
-      if (st < 0 && ((-st) >> j) & 1) m = 0;
-      else if (st == 1) m = 2;
-      else m = 1;
-      // unconnected internal DOF -> fixed (SPRPRP behaviour); the independent
-      // node of a rigid element is connected through the constraint
+      const int idof = s.madof[i] - 1 + j;
+      int m = s.msc[idof];
+      // unconnected internal DOF -> fixed (SPRPRP behaviour); independent
+      // nodes of constraint elements are connected through the constraint
       if (m == 1 && j >= maxdn[i] && !indep[i]) m = 0;
-      if (s.dep_indep[idof] >= 0) m = -1;
+      if (m == -3 || rgd_of[idof] >= 0) m = -1;
       s.msc[idof] = m;
     }
+  // equation terms of the dependent DOFs: a rigid element's u_g + theta_g x
+  // (x_u - x_g) (translations) or theta_g (rotations) over the equation DOFs
+  // of its independent node g, a WAVGM element's weighted terms
+  std::vector<int> weq_of(s.ndof, -1);
+  for (int q = 0; q < (int)weq.size(); q++) {
+    for (int d : weq[q].dof)
+      if (s.msc[d] <= 0) {
+        err = "constraint chains (an independent DOF that is dependent elsewhere) are not supported";
+        return -1;
+      }
+    weq_of[weq[q].dep] = q;
   }
+  s.ceq_ptr.assign(s.ndof + 1, 0);
+  for (int u = 0; u < p.nnod; u++)
+    for (int j = 0; j < s.nndof[u]; j++) {
+      const int d = s.madof[u] - 1 + j;
+      s.ceq_ptr[d + 1] = s.ceq_ptr[d];
+      if (s.msc[d] != -1) continue;
+      auto add = [&](int n, int dof, double c) {
+        s.ceq_node.push_back(n);
+        s.ceq_dof.push_back(dof);
+        s.ceq_coef.push_back(c);
+        s.ceq_ptr[d + 1]++;
+      };
+      if (weq_of[d] >= 0) {
+        const WavgmEquation& q = weq[weq_of[d]];
+        for (size_t k = 0; k < q.dof.size(); k++) add(q.node[k], q.dof[k], q.coef[k]);
+        continue;
+      }
+      const int g = rgd_of[d];
+      for (int k = 0; k < s.nndof[g]; k++)
+        if (s.msc[s.madof[g] - 1 + k] < 0) {
+          err = "constraint chains (an independent DOF that is dependent elsewhere) are not supported";
+          return -1;
+        }
+      double r[3];
+      for (int c = 0; c < 3; c++) r[c] = p.xyz[3 * u + c] - p.xyz[3 * g + c];
+      int cj[3] = {j, -1, -1};
+      double cc[3] = {1.0, 0.0, 0.0};
+      if (j == 0) { cj[1] = 4; cc[1] = r[2]; cj[2] = 5; cc[2] = -r[1]; }
+      if (j == 1) { cj[1] = 5; cc[1] = r[0]; cj[2] = 3; cc[2] = -r[2]; }
+      if (j == 2) { cj[1] = 3; cc[1] = r[1]; cj[2] = 4; cc[2] = -r[0]; }
+      for (int k = 0; k < 3; k++) {
+        if (cj[k] < 0 || cj[k] >= s.nndof[g] || cc[k] == 0.0) continue;
+        const int dg = s.madof[g] - 1 + cj[k];
+        if (s.msc[dg] > 0) add(g, dg, cc[k]);
+      }
+    }
   s.meqn.assign(s.ndof, 0);
   int ieq = 0;
   for (int d = 0; d < s.ndof; d++)

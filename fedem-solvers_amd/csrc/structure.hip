@@ -8,9 +8,10 @@
 //   k_csr_cols      CSR column indices (full symmetric pattern)
 //   k_contrib       every element-matrix entry's CSR position: counted
 //                   (pass 0), then written (pass 1) into the pull-assembly
-//                   contribution lists; with rigid elements a dependent DOF
-//                   contributes through its independent node's DOFs with
-//                   the constraint coefficients
+//                   contribution lists; a dependent DOF of a constraint
+//                   element (rigid or weighted-average) contributes through
+//                   the terms of its constraint equation with their
+//                   coefficients
 //   k_seg_sort      each CSR entry's contributions in a fixed order (element
 //                   matrix position, then coefficient), so that the assembly
 //                   sums in the same order whatever the atomics did
@@ -72,38 +73,19 @@ __global__ __launch_bounds__(64) void k_csr_cols(int neq, const int* __restrict_
 }
 
 // ---------------------------------------------------------------- contributions
+// The equation terms of a DOF: a free DOF is its own equation (one term,
+// coefficient 1), a dependent DOF of a constraint element has the terms of
+// its constraint equation (ceq_* tables), a fixed DOF none.
 struct ETerm {
   int node, rank;
   double c;
 };
 
-// equation terms of DOF j of node u: itself, or for a DOF of a rigid
-// element's dependent node the independent node g's DOFs (u_g + theta_g x
-// (x_u - x_g) for translations, theta_g for rotations); none when fixed
-__device__ inline int dof_terms(int u, int j, const int* __restrict__ madof, const int* __restrict__ msc,
-                                const int* __restrict__ nndof, const int* __restrict__ dep_indep,
-                                const int* __restrict__ drank, const double* __restrict__ xyz, ETerm* t) {
-  const int d = madof[u] - 1 + j;
-  if (msc[d] > 0) {
-    t[0] = {u, drank[d], 1.0};
-    return 1;
-  }
-  if (msc[d] != -1) return 0;
-  const int g = dep_indep[d];
-  const double r0 = xyz[3 * u] - xyz[3 * g], r1 = xyz[3 * u + 1] - xyz[3 * g + 1], r2 = xyz[3 * u + 2] - xyz[3 * g + 2];
-  int cj[3] = {j, -1, -1};
-  double cc[3] = {1.0, 0.0, 0.0};
-  if (j == 0) { cj[1] = 4; cc[1] = r2; cj[2] = 5; cc[2] = -r1; }
-  if (j == 1) { cj[1] = 5; cc[1] = r0; cj[2] = 3; cc[2] = -r2; }
-  if (j == 2) { cj[1] = 3; cc[1] = r1; cj[2] = 4; cc[2] = -r0; }
-  int n = 0;
-  for (int k = 0; k < 3; k++) {
-    if (cj[k] < 0 || cj[k] >= nndof[g] || cc[k] == 0.0) continue;
-    const int dg = madof[g] - 1 + cj[k];
-    if (msc[dg] <= 0) continue;
-    t[n++] = {g, drank[dg], cc[k]};
-  }
-  return n;
+__device__ inline ETerm dof_term(int self_node, int d, int base, int k, const int* __restrict__ drank,
+                                 const int* __restrict__ ceq_node, const int* __restrict__ ceq_dof,
+                                 const double* __restrict__ ceq_coef) {
+  if (base < 0) return {self_node, drank[d], 1.0};
+  return {ceq_node[base + k], drank[ceq_dof[base + k]], ceq_coef[base + k]};
 }
 
 // One 64-lane workgroup per element.  Pass 0 counts the contributions of
@@ -111,20 +93,30 @@ __device__ inline int dof_terms(int u, int j, const int* __restrict__ madof, con
 __global__ __launch_bounds__(64) void k_contrib(
     int pass, const int* __restrict__ etype, const int* __restrict__ eptr, const int* __restrict__ enodes,
     const int64_t* __restrict__ eoff, const int* __restrict__ madof, const int* __restrict__ msc,
-    const int* __restrict__ nndof, const int* __restrict__ dep_indep, const int* __restrict__ drank,
-    const int* __restrict__ nfirst, const double* __restrict__ xyz, const int* __restrict__ rowptr,
-    const int* __restrict__ nbptr, const int* __restrict__ nbl, const int* __restrict__ nbo, int* __restrict__ cnt,
-    const int* __restrict__ cptr, int* __restrict__ cidx, double* __restrict__ ccoef, int* __restrict__ bad) {
-  __shared__ ETerm tm[48][3];
-  __shared__ int nt[48], trow[48][3];
+    const int* __restrict__ nndof, const int* __restrict__ ceq_ptr, const int* __restrict__ ceq_node,
+    const int* __restrict__ ceq_dof, const double* __restrict__ ceq_coef, const int* __restrict__ drank,
+    const int* __restrict__ nfirst, const int* __restrict__ rowptr, const int* __restrict__ nbptr,
+    const int* __restrict__ nbl, const int* __restrict__ nbo, int* __restrict__ cnt, const int* __restrict__ cptr,
+    int* __restrict__ cidx, double* __restrict__ ccoef, int* __restrict__ bad) {
+  // per local DOF: its node, global DOF, term-table base (-1: the DOF itself)
+  // and term count
+  __shared__ int lnode[48], ldof[48], lbase[48], nt[48];
   const int e = blockIdx.x, lane = threadIdx.x;
   const int t = etype[e], d = t == 44 ? 3 : 6, nn = t == 44 ? 8 : t == 24 ? 4 : 2, nd = nn * d;
   const int* en = enodes + eptr[e];
   if (lane < nd) {
     const int a = lane / d, i = lane % d, u = en[a];
-    const int n = i < nndof[u] ? dof_terms(u, i, madof, msc, nndof, dep_indep, drank, xyz, tm[lane]) : 0;
+    int n = 0, base = -1, dof = 0;
+    if (i < nndof[u]) {
+      dof = madof[u] - 1 + i;
+      const int m = msc[dof];
+      if (m > 0) n = 1;
+      else if (m == -1) { base = ceq_ptr[dof]; n = ceq_ptr[dof + 1] - base; }
+    }
+    lnode[lane] = u;
+    ldof[lane] = dof;
+    lbase[lane] = base;
     nt[lane] = n;
-    for (int k = 0; k < n; k++) trow[lane][k] = rowptr[nfirst[tm[lane][k].node] + tm[lane][k].rank];
   }
   __syncthreads();
   const int64_t e0 = eoff[e];
@@ -132,15 +124,16 @@ __global__ __launch_bounds__(64) void k_contrib(
     const int li = idx / nd, lj = idx % nd, a = li / d, i = li % d, b = lj / d, j = lj % d;
     const int off = a >= b ? ((a * (a + 1) / 2 + b) * d + i) * d + j : ((b * (b + 1) / 2 + a) * d + j) * d + i;
     for (int ti = 0; ti < nt[li]; ti++) {
-      const ETerm P = tm[li][ti];
+      const ETerm P = dof_term(lnode[li], ldof[li], lbase[li], ti, drank, ceq_node, ceq_dof, ceq_coef);
+      const int prow = rowptr[nfirst[P.node] + P.rank];
       for (int tj = 0; tj < nt[lj]; tj++) {
-        const ETerm Q = tm[lj][tj];
+        const ETerm Q = dof_term(lnode[lj], ldof[lj], lbase[lj], tj, drank, ceq_node, ceq_dof, ceq_coef);
         const int q = find_nbr(P.node, Q.node, nbptr, nbl, nfirst);
         if (q < 0) {
           atomicOr(bad, 1);
           continue;
         }
-        const int k = trow[li][ti] + nbo[q] + Q.rank;
+        const int k = prow + nbo[q] + Q.rank;
         if (pass == 0) {
           atomicAdd(&cnt[k], 1);
         } else {
@@ -294,12 +287,13 @@ void launch_csr_cols(hipStream_t st, int neq, const int* rnode, const int* rowpt
 }
 
 void launch_contrib(hipStream_t st, int pass, int nel, const int* etype, const int* eptr, const int* enodes,
-                    const int64_t* eoff, const int* madof, const int* msc, const int* nndof, const int* dep_indep,
-                    const int* drank, const int* nfirst, const double* xyz, const int* rowptr, const int* nbptr,
-                    const int* nbl, const int* nbo, int* cnt, const int* cptr, int* cidx, double* ccoef, int* bad) {
+                    const int64_t* eoff, const int* madof, const int* msc, const int* nndof, const int* ceq_ptr,
+                    const int* ceq_node, const int* ceq_dof, const double* ceq_coef, const int* drank,
+                    const int* nfirst, const int* rowptr, const int* nbptr, const int* nbl, const int* nbo, int* cnt,
+                    const int* cptr, int* cidx, double* ccoef, int* bad) {
   if (nel > 0)
-    k_contrib<<<nel, 64, 0, st>>>(pass, etype, eptr, enodes, eoff, madof, msc, nndof, dep_indep, drank, nfirst, xyz,
-                                  rowptr, nbptr, nbl, nbo, cnt, cptr, cidx, ccoef, bad);
+    k_contrib<<<nel, 64, 0, st>>>(pass, etype, eptr, enodes, eoff, madof, msc, nndof, ceq_ptr, ceq_node, ceq_dof,
+                                  ceq_coef, drank, nfirst, rowptr, nbptr, nbl, nbo, cnt, cptr, cidx, ccoef, bad);
 }
 
 void launch_seg_sort(hipStream_t st, int nnz, const int* cptr, int* cidx, double* ccoef) {

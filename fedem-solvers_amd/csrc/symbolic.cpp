@@ -105,29 +105,15 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
       if (m > 0) { neqd[i]++; ncls[i] = m; }
     }
   }
-  // Equation terms of every DOF: a free DOF is its own equation; a dependent
-  // DOF of a rigid element is u_g + theta_g x (x_u - x_g) (translations) or
-  // theta_g (rotations) of its independent node g; a fixed DOF has none.
-  struct Term { int node, j; double c; };
-  auto dof_terms = [&](int u, int j, Term* t) -> int {
-    int d = s.madof[u] - 1 + j;
-    if (s.msc[d] > 0) { t[0] = {u, j, 1.0}; return 1; }
-    if (s.msc[d] != -1) return 0;
-    int g = s.dep_indep[d], n = 0;
-    double r[3];
-    for (int c = 0; c < 3; c++) r[c] = p.xyz[3 * u + c] - p.xyz[3 * g + c];
-    Term cand[3];
-    int nc = 1;
-    cand[0] = {g, j, 1.0};
-    if (j == 0) { cand[1] = {g, 4, r[2]}; cand[2] = {g, 5, -r[1]}; nc = 3; }
-    if (j == 1) { cand[1] = {g, 5, r[0]}; cand[2] = {g, 3, -r[2]}; nc = 3; }
-    if (j == 2) { cand[1] = {g, 3, r[1]}; cand[2] = {g, 4, -r[0]}; nc = 3; }
-    for (int k = 0; k < nc; k++) {
-      if (cand[k].j >= s.nndof[g] || cand[k].c == 0.0) continue;
-      if (s.msc[s.madof[g] - 1 + cand[k].j] <= 0) continue;
-      t[n++] = cand[k];
-    }
-    return n;
+  // Equation terms of every DOF: a free DOF is its own equation, a dependent
+  // DOF the terms of its constraint equation, a fixed DOF has none.
+  auto nterms = [&](int u, int j) -> int {
+    const int d = s.madof[u] - 1 + j, m = s.msc[d];
+    return m > 0 ? 1 : m == -1 ? s.ceq_ptr[d + 1] - s.ceq_ptr[d] : 0;
+  };
+  auto term_node = [&](int u, int j, int k) -> int {
+    const int d = s.madof[u] - 1 + j;
+    return s.msc[d] > 0 ? u : s.ceq_node[s.ceq_ptr[d] + k];
   };
   // equation nodes each element couples (its own nodes with equations and
   // the independent nodes of its dependent DOFs)
@@ -139,10 +125,11 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
       for (int q = p.eptr[e]; q < p.eptr[e + 1]; q++) {
         int u = p.enodes[q];
         for (int j = 0; j < d && j < s.nndof[u]; j++) {
-          Term t[3];
-          int nt = dof_terms(u, j, t);
-          for (int k = 0; k < nt; k++)
-            if (stamp[t[k].node] != e) { stamp[t[k].node] = e; ecn.push_back(t[k].node); }
+          const int nt = nterms(u, j);
+          for (int k = 0; k < nt; k++) {
+            const int v = term_node(u, j, k);
+            if (stamp[v] != e) { stamp[v] = e; ecn.push_back(v); }
+          }
         }
       }
       ecptr[e + 1] = (int)ecn.size();
@@ -381,16 +368,13 @@ int build_symbolic(const Part& p, const Sam& s, const SymbolicOptions& o,
       if (s.msc[s.madof[i] - 1 + j] > 0) y.drank[s.madof[i] - 1 + j] = k++;
   }
   // contributions: every pair of equation terms of an element's DOFs
-  y.has_rigid = !p.rgd_indep.empty();
+  y.has_rigid = !s.ceq_dof.empty();
   y.ncontrib = 0;
   for (int e = 0; e < p.nel; e++) {
     const int t = p.etype[e], d = elem_ndn(t);
     int64_t nt = 0;
     for (int q = p.eptr[e]; q < p.eptr[e + 1]; q++)
-      for (int j = 0; j < d && j < s.nndof[p.enodes[q]]; j++) {
-        Term tm[3];
-        nt += dof_terms(p.enodes[q], j, tm);
-      }
+      for (int j = 0; j < d && j < s.nndof[p.enodes[q]]; j++) nt += nterms(p.enodes[q], j);
     y.ncontrib += nt * nt;
   }
   if (y.rowptr[s.neq] < 0 || y.ncontrib > INT32_MAX) {

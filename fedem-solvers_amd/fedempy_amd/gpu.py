@@ -35,6 +35,8 @@ def load_library(path=None):
     lib.fgpu_last_error.restype = ctypes.c_char_p
     lib.fgpu_set_part.argtypes = [P, I, P, P, I, P, P, P, P]
     lib.fgpu_set_rigid.argtypes = [P, I, P, P, P, P]
+    lib.fgpu_set_wavgm.argtypes = [P, I, P, P, P, P, P, P, P]
+    lib.fgpu_get_constraints.argtypes = [P, P, P, P, P]
     lib.fgpu_analyze.argtypes = [P, I, I, I]
     lib.fgpu_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_double]
     for name in ("fgpu_get_sizes",):
@@ -113,6 +115,31 @@ class GpuReducer:
             self._keep_rgd = (indep, mask, dptr, dep)
             self._check(self.lib.fgpu_set_rigid(self.h, len(indep), _ptr(indep), _ptr(mask), _ptr(dptr), _ptr(dep)),
                         "fgpu_set_rigid")
+        rbe3 = part.get("rbe3") or []
+        if rbe3:  # WAVGM elements: dict(ref, nodes, [refc, indc, weights]), 0-based nodes
+            ref = np.array([el["ref"] + 1 for el in rbe3], np.int32)
+            ws = [np.asarray(el.get("weights", []), np.float64) for el in rbe3]
+            refc = np.array([int(el.get("refc", 123456)) if len(w) else 123456 for el, w in zip(rbe3, ws)], np.int32)
+            indc = np.array([list(el["indc"]) if len(w) else [-1, -1, -1, 0, 0, 0] for el, w in zip(rbe3, ws)],
+                            np.int32).ravel()
+            nodes = np.array([n + 1 for el in rbe3 for n in el["nodes"]], np.int32)
+            nptr = np.concatenate([[1], 1 + np.cumsum([len(el["nodes"]) for el in rbe3])]).astype(np.int32)
+            wptr = np.concatenate([[1], 1 + np.cumsum([len(w) for w in ws])]).astype(np.int32)
+            wv = np.concatenate(ws + [np.zeros(0)]).astype(np.float64)
+            self._keep_wv = (ref, refc, indc, nptr, nodes, wptr, wv)
+            self._check(self.lib.fgpu_set_wavgm(self.h, len(ref), _ptr(ref), _ptr(refc), _ptr(indc), _ptr(nptr),
+                                                _ptr(nodes), _ptr(wptr), _ptr(wv)), "fgpu_set_wavgm")
+
+    def constraints(self):
+        """Constraint equations in SAM's MPMCEQ/MMCEQ/TTCC layout (1-based)"""
+        sz = np.zeros(2, np.int32)
+        self._check(self.lib.fgpu_get_constraints(self.h, _ptr(sz), None, None, None), "fgpu_get_constraints")
+        mp = np.zeros(sz[0] + 1, np.int32)
+        mm = np.zeros(max(1, sz[1]), np.int32)
+        tt = np.zeros(max(1, sz[1]), np.float64)
+        self._check(self.lib.fgpu_get_constraints(self.h, _ptr(sz), _ptr(mp), _ptr(mm), _ptr(tt)),
+                    "fgpu_get_constraints")
+        return mp, mm[:sz[1]], tt[:sz[1]]
 
     def set_option(self, name, value):
         self._check(self.lib.fgpu_set_option(self.h, name.encode(), float(value)), "fgpu_set_option")

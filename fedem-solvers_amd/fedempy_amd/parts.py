@@ -188,6 +188,93 @@ def beam_frame(n_el=100, seed=0, n_ext=4, bay=0.5, E=2.1e11, nu=0.3, rho=7850.0,
     return _finish(xyz, status, np.full(nel, 11, np.int32), [np.array(c) for c in conn], props)
 
 
+def _stiff_beam_props(n, E=2.1e11, nu=0.3, rho=7850.0, ro=0.05, t=0.01, v=(1.0, 0.0, 0.0)):
+    A, I, Jt = tube_section(ro, t)
+    props = np.zeros((n, NPROP))
+    props[:, 0], props[:, 1], props[:, 2], props[:, 3] = E, nu, rho, E / (2 * (1 + nu))
+    props[:, 4], props[:, 5], props[:, 6], props[:, 7] = A, I, I, Jt
+    props[:, 8:11] = np.array(v)
+    props[:, 11] = props[:, 12] = 0.5
+    return props
+
+
+def _wavgm(ref, nodes, weights, rng):
+    """a WAVGM element dict: weights "default" (no properties: uniform
+    translations), "equal" (explicit equal translational weights), "random"
+    (random translational weights) or "rotational" (random weights on all six
+    components, translations and rotations in separate sets)"""
+    el = dict(ref=int(ref), nodes=[int(n) for n in nodes])
+    nM = len(nodes)
+    if weights == "equal":
+        el.update(refc=123456, indc=[1, 1, 1, 0, 0, 0], weights=np.ones(nM))
+    elif weights == "random":
+        el.update(refc=123456, indc=[1, 1, 1, 0, 0, 0], weights=rng.uniform(0.5, 2.0, nM))
+    elif weights == "rotational":
+        el.update(refc=123456, indc=[1, 1, 1, 1 + nM, 1 + nM, 1 + nM],
+                  weights=np.concatenate([rng.uniform(0.5, 2.0, nM), rng.uniform(0.5, 2.0, nM)]))
+    elif weights != "default":
+        raise ValueError(weights)
+    return el
+
+
+def cylinder_spiders(nL=8, nC=12, L=1.0, R=0.1, t=0.004, weights="default", seed=0, gap=0.1):
+    """Shell cylinder along z of ANDES4 elements (nC around, nL along) with a
+    weighted-average-motion spider (WAVGM / RBE3) at each end, as the flexible
+    spiders of the reference's cylinder test model (testModels.C:139-260):
+    the reference node on the axis at each end takes the weighted motion of
+    its ring; a stiff tube beam of length `gap` joins it to an external node
+    further out.  Free-free: the two external nodes carry the boundary."""
+    rng = np.random.default_rng(seed)
+    th = 2.0 * np.pi * np.arange(nC) / nC
+    rings = [np.stack([R * np.cos(th), R * np.sin(th), np.full(nC, z)], 1) for z in np.linspace(0.0, L, nL + 1)]
+    xyz = list(np.concatenate(rings))
+    ring = lambda i: np.arange(nC) + i * nC
+    conn = []
+    for i in range(nL):
+        a, b = ring(i), ring(i + 1)
+        for j in range(nC):
+            conn.append(np.array([a[j], a[(j + 1) % nC], b[(j + 1) % nC], b[j]]))
+    n0 = len(xyz)
+    xyz += [[0.0, 0.0, 0.0], [0.0, 0.0, L], [0.0, 0.0, -gap], [0.0, 0.0, L + gap]]
+    ref0, ref1, ext0, ext1 = n0, n0 + 1, n0 + 2, n0 + 3
+    nsh = len(conn)
+    conn += [np.array([ext0, ref0]), np.array([ref1, ext1])]
+    props = np.zeros((nsh + 2, NPROP))
+    props[:nsh, 0], props[:nsh, 1], props[:nsh, 2] = 2.1e11, 0.3, 7850.0
+    props[:nsh, 3] = 2.1e11 / 2.6
+    props[:nsh, 4:8] = t
+    props[nsh:] = _stiff_beam_props(2)
+    status = np.zeros(len(xyz), np.int32)
+    status[[ext0, ext1]] = 1
+    melcon = np.array([24] * nsh + [11, 11], np.int32)
+    part = _finish(np.array(xyz), status, melcon, conn, props)
+    part["rbe3"] = [_wavgm(ref0, ring(0), weights, rng), _wavgm(ref1, ring(nL), weights, rng)]
+    return part
+
+
+def block_spider(nx=6, ny=4, nz=4, lx=0.3, ly=0.2, lz=0.2, weights="random", seed=0, n_ext=4, gap=0.05):
+    """HEX8 block (hex_block, n_ext external nodes on its x=0 and x=lx faces)
+    whose x=lx face is tied by a weighted-average-motion element (WAVGM /
+    RBE3) to a reference node off the face centre; a stiff beam joins the
+    reference node to one more external node.  The independent nodes carry
+    three DOFs (solid nodes), the reference node six."""
+    rng = np.random.default_rng(seed + 1)
+    p = hex_block(nx, ny, nz, lx, ly, lz, jitter=0.05, seed=seed, n_ext=n_ext)
+    xyz = p["xyz"]
+    face = np.nonzero(np.isclose(xyz[:, 0], lx))[0]
+    nn = len(xyz)
+    ref, ext = nn, nn + 1
+    xyz = np.vstack([xyz, [[lx + gap, 0.45 * ly, 0.55 * lz], [lx + 2 * gap, 0.45 * ly, 0.55 * lz]]])
+    mp, mm = p["mpmnpc"], p["mmnpc"]
+    conn = [mm[mp[e] - 1:mp[e + 1] - 1] - 1 for e in range(len(p["melcon"]))] + [np.array([ref, ext])]
+    props = np.vstack([p["eprop"], _stiff_beam_props(1, v=(0.0, 0.0, 1.0))])
+    status = np.concatenate([p["status"], [0, 1]]).astype(np.int32)
+    melcon = np.concatenate([p["melcon"], [11]]).astype(np.int32)
+    part = _finish(xyz, status, melcon, conn, props)
+    part["rbe3"] = [_wavgm(ref, face, weights, rng)]
+    return part
+
+
 def config1(seed=0):
     """48x24x24 HEX8 steel block, 0.48x0.24x0.24 m, +-5 % jitter, 12 external
     nodes on the x=0 and x=max faces (72 boundary DOFs), 20 component modes."""
@@ -198,7 +285,8 @@ def write_nastran(part, path, loads=None):
     """Writes a part as Nastran bulk data in free-field format with every
     real at full double precision (GRID, CHEXA/CQUAD4/CBAR, MAT1,
     PSOLID/PSHELL/PBAR, ASET1 for the external nodes, SPC1 for fixed DOFs,
-    FORCE for nodal loads), so that the part read back is bit-identical.
+    RBE3 for weighted-average-motion elements with explicit weights, FORCE
+    for nodal loads), so that the part read back is bit-identical.
     Node ids are 1-based node numbers; one material and one property card
     per element.
     loads: iterable of (load set id, node number 1-based, (fx, fy, fz))."""
@@ -235,6 +323,21 @@ def write_nastran(part, path, loads=None):
         if status[i] < 0:
             comp = "".join(str(j + 1) for j in range(6) if (-int(status[i]) >> j) & 1)
             out.append("SPC1,1,%s,%d" % (comp, i + 1))
+    for k, el in enumerate(part.get("rbe3") or []):
+        # one group per independent node and weight set (WT, C, G); an element
+        # without weights is written with weight 1.0 on the translations
+        w = np.asarray(el.get("weights", []), float)
+        indc = list(el["indc"]) if len(w) else [1, 1, 1, 0, 0, 0]
+        if not len(w):
+            w = np.ones(len(el["nodes"]))
+        fields = [str(len(mel) + k + 1), "", str(int(el["ref"]) + 1), str(int(el.get("refc", 123456)) if len(el.get("weights", [])) else 123456)]
+        for ic in sorted({c for c in indc if c > 0}):
+            comp = "".join(str(c + 1) for c in range(6) if indc[c] == ic)
+            for a, n in enumerate(el["nodes"]):
+                fields += [r(w[ic - 1 + a]), comp, str(int(n) + 1)]
+        out.append("RBE3," + ",".join(fields[:8]))
+        for b in range(8, len(fields), 8):
+            out.append("+," + ",".join(fields[b:b + 8]))
     for sid, node, f in loads or []:
         out.append("FORCE,%d,%d,0,1.0,%s,%s,%s" % (sid, node, *(r(v) for v in f)))
     out.append("ENDDATA")

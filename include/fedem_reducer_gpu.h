@@ -84,6 +84,22 @@ int fgpu_set_part(fgpu_reducer* h, int nnod, const double* xyz, const int* statu
 int fgpu_set_rigid(fgpu_reducer* h, int nrgd, const int* indep, const int* dofmask, const int* dptr,
                    const int* dep);
 
+/* Weighted-average-motion elements (WAVGM / Nastran RBE3; rigidModule.f90:
+ * 89-338 MultiPointConstraints with wavgmModule.f90:58-367 wavgmConstrEqn)
+ * of the part, set last, like the rigid elements: element w has reference
+ * node ref[w] and independent nodes nodes[nptr[w]-1 .. nptr[w+1]-2];
+ * refc[w] names its dependent components (digits 1..6, 0 = all);
+ * indc[6w .. 6w+5] give, per component, the 1-based start of that
+ * component's weights within the element's own weights
+ * weights[wptr[w]-1 .. wptr[w+1]-2] (< 0 uniform weights, 0 no coupling),
+ * the refC / indC / weight triple of ffl_getwavgm (rigidModule.f90:194).  An
+ * element without weights takes refC 123456 with uniform translational
+ * weights (rigidModule.f90:195-200).  Node numbers and pointers are 1-based.
+ * Dependent DOFs get MSC -1; WAVGM elements are processed before the rigid
+ * elements, and chains of constraints are rejected at fgpu_analyze. */
+int fgpu_set_wavgm(fgpu_reducer* h, int nwavgm, const int* ref, const int* refc, const int* indc, const int* nptr,
+                   const int* nodes, const int* wptr, const double* weights);
+
 /* Options of the reduction: ngen (component modes kept), neval (eigenpairs
  * computed), flags bit0 = dispfile (vgi), bit1 = nomass.  Performs SAM
  * numbering (csAllocPointerArrays/csPreassemble, asmExtensionModule.f90:227,
@@ -106,6 +122,16 @@ int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags);
  * that ends the refinement, default 1e-14), "refineMaxSteps" (default 40;
  * reduce returns -6 when the refinement does not converge). */
 int fgpu_set_option(fgpu_reducer* h, const char* name, double value);
+
+/* Constraint equations of the dependent DOFs in SAM's layout (samModule.f90:
+ * 35-37, MPMCEQ/MMCEQ/TTCC): equation c (1-based) occupies positions
+ * mpmceq[c-1] .. mpmceq[c]-1 (1-based) of mmceq/ttcc, the first holding the
+ * dependent DOF and the constant term (0 here), the others an independent
+ * DOF and its coefficient; DOF numbers are 1-based.  sizes[0] = nceq,
+ * sizes[1] = nmmceq; call with NULL arrays to get the sizes.  Equations come
+ * in ascending dependent-DOF order (the reference stores them in the order
+ * they are made: WAVGM elements first, then rigid elements). */
+int fgpu_get_constraints(const fgpu_reducer* h, int* sizes, int* mpmceq, int* mmceq, double* ttcc);
 
 /* sizes[0..7] = nnod, ndof, neq, ndof1, ndof2, ndim, nenod, nel */
 int fgpu_get_sizes(const fgpu_reducer* h, int* sizes);

@@ -4,10 +4,15 @@ Nastran bulk-data subset reader for the reference's FE part fixtures
 (src/vpmReducer/vpmReducerTests/FEparts/*.nas, read in the reference by FFl,
 which is absent from the reference tree).  Cards: GRID, CQUAD4, CHEXA, CBAR,
 MAT1, PSHELL, PSOLID, PBAR, PBARL (TUBE), ASET, ASET1, SPC1 (with "G1 THRU G2"
-ranges), RBE2, FORCE,
+ranges), RBE2, RBE3, FORCE,
 MOMENT, in free (comma), small (8-column) and
 large (16-column, '*') field formats.  Returns a part dict (see
-fedempy_amd.parts) plus the RBE2 constraint list.
+fedempy_amd.parts) plus the RBE2 constraint list and the RBE3 elements in the
+WAVGM form of oracle.wavgm.  How FFl turns an RBE3 card into WAVGM
+properties is not in the reference tree; the reading here: the independent
+nodes in order of first appearance, per component the weight of every node
+summed over the groups that list it with that component, components with
+equal weights sharing one weight set, no group for a component: no coupling.
 """
 import re
 
@@ -97,9 +102,10 @@ def read(path):
     with open(path) as f:
         text = f.read()
     grids, elems, mats, props, aset, rbe2, spc, loads = {}, [], {}, {}, [], [], {}, []
+    rbe3 = []
     aset_range = []
     known = {"GRID", "CHEXA", "CQUAD4", "CBAR", "MAT1", "PSHELL", "PSOLID", "PBAR", "PBARL", "ASET", "ASET1",
-             "SPC1", "RBE2", "FORCE", "MOMENT"}
+             "SPC1", "RBE2", "RBE3", "FORCE", "MOMENT"}
     for name, fl in _cards(text):
         if name not in known:
             continue
@@ -149,6 +155,28 @@ def read(path):
             F = v[3] or 0.0
             loads.append((int(v[0]), int(v[1]), 0 if name == "FORCE" else 3,
                           tuple(F * (x or 0.0) for x in v[4:7])))
+        elif name == "RBE3":
+            groups, state = [], 0
+            for x in v[4:]:
+                if x is None:
+                    continue
+                if isinstance(x, str):
+                    if x.upper() == "UM":
+                        raise ValueError("RBE3 %d: UM dependent DOFs are not supported" % v[0])
+                    if x.upper() in ("ALPHA", "TREF"):
+                        break
+                    raise ValueError("RBE3 %d: unexpected field %r" % (v[0], x))
+                if isinstance(x, float):
+                    groups.append([x, 0, []])
+                    state = 1
+                elif state == 1:
+                    groups[-1][1] = sum(1 << (int(c) - 1) for c in str(x) if c in "123456")
+                    state = 2
+                elif state == 2:
+                    groups[-1][2].append(x)
+                else:
+                    raise ValueError("RBE3 %d: unexpected field %r" % (v[0], x))
+            rbe3.append((int(v[2]), int(v[3] or 0), groups))
         elif name == "RBE2":
             vals = [x for x in v if x is not None]
             rbe2.append((int(vals[1]), str(int(vals[2])), [int(x) for x in vals[3:]]))
@@ -182,11 +210,40 @@ def read(path):
         melcon.append(t)
         conn.append([idx[n] for n in nodes])
         eprop.append(pr)
+    wv = []
+    for ref, refc, groups in rbe3:
+        order = []
+        for _, _, gl in groups:
+            for g in gl:
+                if g not in order:
+                    order.append(g)
+        if not order:
+            raise ValueError("RBE3 without independent grids")
+        pos = {g: a for a, g in enumerate(order)}
+        indc, wts = [0] * 6, []
+        for c in range(6):
+            w = np.zeros(len(order))
+            hit = False
+            for wt, comps, gl in groups:
+                if (comps >> c) & 1:
+                    for g in gl:
+                        w[pos[g]] += wt
+                        hit = True
+            if not hit:
+                continue
+            for prev in range(c):
+                if indc[prev] > 0 and np.array_equal(w, np.asarray(wts[indc[prev] - 1:indc[prev] - 1 + len(order)])):
+                    indc[c] = indc[prev]
+                    break
+            if not indc[c]:
+                indc[c] = len(wts) + 1
+                wts.extend(w)
+        wv.append(dict(ref=idx[ref], refc=refc, nodes=[idx[g] for g in order], indc=indc, weights=np.array(wts)))
     sizes = [len(c) for c in conn]
     part = dict(xyz=xyz, status=status, melcon=np.array(melcon, np.int32),
                 mpmnpc=np.concatenate([[1], 1 + np.cumsum(sizes)]).astype(np.int32),
                 mmnpc=(np.concatenate([np.asarray(c) for c in conn]) + 1).astype(np.int32),
                 eprop=np.array(eprop), node_ids=np.array(ids),
-                rbe2=[(idx[gn], cm, [idx[d] for d in dep]) for gn, cm, dep in rbe2],
+                rbe2=[(idx[gn], cm, [idx[d] for d in dep]) for gn, cm, dep in rbe2], rbe3=wv,
                 loads=[(sid, idx[g], d0, f) for sid, g, d0, f in loads])
     return part

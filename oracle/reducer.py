@@ -28,6 +28,8 @@ import scipy.linalg as sla
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from . import wavgm
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _lib = None
 
@@ -124,13 +126,6 @@ def sam_numbering(part):
     madof = np.concatenate([[1], 1 + np.cumsum(nndof)])
     ndof = int(madof[-1] - 1)
     msc = np.zeros(ndof, int)
-    dep = {}
-    for gn, cm, deps in part.get("rbe2", []):
-        for dn in deps:
-            for c in cm:
-                j = int(c) - 1
-                if j < nndof[dn]:
-                    dep[(dn, j)] = gn
     for i in range(nnod):
         for j in range(nndof[i]):
             st = int(status[i])
@@ -140,11 +135,35 @@ def sam_numbering(part):
                 m = 2
             else:
                 m = 1
-            if m == 1 and j >= maxdn[i] and not any(i == g for g, _, _ in part.get("rbe2", [])):
-                m = 0
-            if (i, j) in dep:
-                m = -1  # dependent DOF of a rigid element: no equation
             msc[madof[i] - 1 + j] = m
+    # WAVGM (RBE3) elements first, on the nodal status codes, then the rigid
+    # elements (samReducerModule.f90:385-459)
+    wv = wavgm.equations(part.get("rbe3", []), part["xyz"], madof, nndof, msc)
+    dep = {}
+    for gn, cm, deps in part.get("rbe2", []):
+        for dn in deps:
+            for c in cm:
+                j = int(c) - 1
+                if j < nndof[dn]:
+                    if msc[madof[dn] - 1 + j] < 0:
+                        raise ValueError("DOF constrained by two constraint elements")
+                    dep[(dn, j)] = gn
+    indep = {g for g, _, _ in part.get("rbe2", [])} | {int(n) for el in part.get("rbe3", []) for n in el["nodes"]}
+    for i in range(nnod):
+        for j in range(nndof[i]):
+            d = madof[i] - 1 + j
+            m = msc[d]
+            if m == 1 and j >= maxdn[i] and i not in indep:
+                m = 0  # unconnected internal DOF
+            if m == -3 or (i, j) in dep:
+                m = -1  # dependent DOF of a constraint element: no equation
+            msc[d] = m
+    for terms in wv.values():
+        if any(msc[d] < 0 for d, _ in terms):
+            raise ValueError("constraint chains (an independent DOF that is dependent elsewhere) are not supported")
+    for (dn, j), gn in dep.items():
+        if any(msc[madof[gn] - 1 + k] < 0 for k in range(nndof[gn])):
+            raise ValueError("constraint chains (an independent DOF that is dependent elsewhere) are not supported")
     meqn = np.zeros(ndof, int)
     n1 = int(np.sum(msc == 1))
     meqn[msc == 1] = np.arange(1, n1 + 1)
@@ -159,14 +178,15 @@ def sam_numbering(part):
     dofpos2 = pos2[meqn[msc == 2]]
     ndim_ext = int(np.sum(nndof[status == 1]))
     return dict(madof=madof, msc=msc, meqn=meqn, meqn1=meqn1, meqn2=meqn2, dofPosIn2=dofpos2,
-                nndof=nndof, neq=neq, ndof1=len(meqn1), ndof2=len(meqn2), ndim_ext=ndim_ext, dep=dep)
+                nndof=nndof, neq=neq, ndof1=len(meqn1), ndof2=len(meqn2), ndim_ext=ndim_ext, dep=dep, wavgm=wv)
 
 
 def constraint_T(part, s):
     """T (ndof x neq): u_dof = T u_eq.  Free DOFs map to their equation,
     fixed DOFs to zero, dependent DOFs of RBE2 elements to the rigid-body
     motion of the independent node: u_d = u_i + theta_i x (x_d - x_i) for
-    translations, theta_d = theta_i for rotations."""
+    translations, theta_d = theta_i for rotations; dependent DOFs of WAVGM
+    (RBE3) elements to their equations (oracle.wavgm)."""
     rows, cols, vals = [], [], []
     madof, meqn = s["madof"], s["meqn"]
     xyz = part["xyz"]
@@ -175,7 +195,7 @@ def constraint_T(part, s):
             rows.append(d); cols.append(meqn[d] - 1); vals.append(1.0)
     for (dn, j), gn in s["dep"].items():
         d = madof[dn] - 1 + j
-        eqi = [meqn[madof[gn] - 1 + k] - 1 for k in range(6)]
+        eqi = [meqn[madof[gn] - 1 + k] - 1 if k < s["nndof"][gn] else -1 for k in range(6)]
         r = xyz[dn] - xyz[gn]
         if j < 3:
             coef = {j: 1.0}
@@ -188,6 +208,9 @@ def constraint_T(part, s):
         for k, c in coef.items():
             if eqi[k] >= 0 and c != 0.0:
                 rows.append(d); cols.append(eqi[k]); vals.append(c)
+    for d, terms in s["wavgm"].items():
+        for di, c in terms:
+            rows.append(d); cols.append(meqn[di] - 1); vals.append(c)
     return sp.csr_matrix((vals, (rows, cols)), shape=(len(s["msc"]), s["neq"]))
 
 
