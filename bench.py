@@ -60,6 +60,58 @@ print(json.dumps({"s": time.perf_counter() - t, "cores": len(os.sched_getaffinit
     return json.loads(out.stdout.strip().splitlines()[-1])
 
 
+def cpu_baselines(seed=0, budget_s=15.0):
+    """The other configurations on the host cores, in a subprocess, by the
+    same CPU port (oracle/multifrontal.py over this library's host ordering):
+    the 20x10 cantilever plate (median of 5 runs), the 200x200 plate (FP64,
+    one run), and bounded samples of the batched and served mixes (the same
+    seeded parts / requests the GPU legs draw, in their order, until about
+    `budget_s` seconds of work, at least 3): reductions, and for requests
+    their load cases and internal-displacement recovery."""
+    code = r"""
+import os, sys, time, json
+import numpy as np
+sys.path.insert(0, %r); sys.path.insert(0, %r)
+from fedempy_amd import gpu, parts
+from oracle import multifrontal as mf
+budget = %r
+seed = %d
+def one(part, ngen, loads=None, ub=None):
+    t = time.perf_counter()
+    R = gpu.GpuReducer(); R.set_part(part); neq = R.analyze(ngen=ngen)["neq"]
+    sn = R.supernodes(); R.close()
+    if loads is None:
+        mf.reduce(part, sn, ngen=ngen)
+    else:
+        mf.serve(part, sn, ngen, loads, ub)
+    return time.perf_counter() - t, neq
+out = {"cores": len(os.sched_getaffinity(0))}
+out["config0"] = float(np.median([one(parts.config0(), 0)[0] for _ in range(5)]))
+out["config2"] = one(parts.config2(seed=seed), 0)[0]
+rng = np.random.default_rng(seed)
+mix = [parts.random_part(rng) for _ in range(256)]
+t, n, dof = 0.0, 0, 0
+for p, g in mix:
+    dt, neq = one(p, g); t += dt; n += 1; dof += neq
+    if n >= 3 and t >= budget:
+        break
+out["config3"] = {"parts": n, "s": t, "mean_neq": dof / n}
+rng = np.random.default_rng(seed)
+reqs = [parts.served_request(rng) for _ in range(160)]
+t, n, dof = 0.0, 0, 0
+for p, g, loads, ub in reqs:
+    dt, neq = one(p, g, loads, ub); t += dt; n += 1; dof += neq
+    if n >= 3 and t >= budget:
+        break
+out["config4"] = {"requests": n, "s": t, "mean_neq": dof / n}
+print(json.dumps(out))
+""" % (ROOT, os.path.join(ROOT, "fedem-solvers_amd"), budget_s, seed)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=1200)
+    if out.returncode != 0:
+        raise RuntimeError(out.stderr[-2000:])
+    return json.loads(out.stdout.strip().splitlines()[-1])
+
+
 def measured_traffic():
     """HBM bytes per k_update launch (FETCH_SIZE + WRITE_SIZE, copy-kernel
     calibrated) from the committed PMC summary, or None without one."""
@@ -195,6 +247,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="tune the factor panel width and solve split on the part before timing (default: the "
                          "library defaults, which the committed profiles were measured with)")
+    ap.add_argument("--only", choices=("mixed", "serving", "cpu"), default=None,
+                    help="run one side measurement alone and print it as a JSON line (for profiling)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain device copy (torch.neg over N MiB of FP64) whose HBM "
                          "counters calibrate FETCH_SIZE/WRITE_SIZE of the profiled kernels")
@@ -204,6 +258,15 @@ def main():
         sys.exit(2)
     from fedempy_amd import gpu, parts
 
+    if a.only == "mixed":
+        print(json.dumps({"mixed_precision": mixed_plate(a.seed)}))
+        return
+    if a.only == "serving":
+        print(json.dumps({"batched": batched(256, 16, a.seed), "served": served(160, 16, a.seed)}))
+        return
+    if a.only == "cpu":
+        print(json.dumps({"cpu_baselines": cpu_baselines(a.seed)}))
+        return
     if a.calib_copy_mib > 0:
         import torch
         src = torch.ones(a.calib_copy_mib * 2**20 // 8, dtype=torch.float64, device="cuda")
@@ -325,6 +388,28 @@ def main():
                       "element matrices and assembly, multifrontal Cholesky with the Schur front, B, "
                       "20 ARPACK shift-invert modes, mass and gravity reduction) by oracle/multifrontal.py "
                       "on numpy/OpenBLAS over %d host cores: %.1f s" % (cb["cores"], cb["s"])}
+        cbs = cpu_baselines(a.seed)
+        c = cbs["cores"]
+        other = {
+            "config0": {"value": 1.0 / cbs["config0"], "unit": "reductions/s", "cores": c, "kind": "port",
+                        "sample": "20x10 ANDES4 cantilever plate, 12 boundary DOF: median of 5 reductions, "
+                                  "%.3f s" % cbs["config0"]},
+            "config2": {"value": 1.0 / cbs["config2"], "unit": "reductions/s", "cores": c, "kind": "port",
+                        "sample": "200x200 ANDES4 plate, 384 boundary DOF, FP64: one reduction, %.1f s"
+                                  % cbs["config2"]},
+            "config3": {"value": cbs["config3"]["parts"] / cbs["config3"]["s"], "unit": "parts/s", "cores": c,
+                        "kind": "port",
+                        "sample": "the first %d of the 256 batched parts (mean %.0f equations), %.1f s"
+                                  % (cbs["config3"]["parts"], cbs["config3"]["mean_neq"],
+                                     cbs["config3"]["s"])},
+            "config4": {"value": cbs["config4"]["requests"] / cbs["config4"]["s"], "unit": "requests/s",
+                        "cores": c, "kind": "port",
+                        "sample": "the first %d of the 160 served requests (mean %.0f equations) with their load "
+                                  "cases and recovery, one after the other, %.1f s"
+                                  % (cbs["config4"]["requests"], cbs["config4"]["mean_neq"],
+                                     cbs["config4"]["s"])},
+        }
+        line["cpu_baselines_other_configs"] = other
     print(json.dumps(line))
 
 

@@ -83,9 +83,10 @@ class Fronts:
         return Z.ravel() if vec else Z
 
 
-def reduce(part, sn, ngen=0):
+def reduce(part, sn, ngen=0, keep=False):
     """SK, SM, G of the part (as oracle.reducer.reduce) through the fronts
-    of `sn`; eigenvalues when ngen > 0."""
+    of `sn`; eigenvalues when ngen > 0.  keep: also the fronts, B, the modes
+    and the numbering, for load cases on the same factor (serve)."""
     s = orc.sam_numbering(part)
     K, M, _ = orc.assemble(part, s)
     p2e = np.asarray(sn["perm_to_eq"])
@@ -122,4 +123,34 @@ def reduce(part, sn, ngen=0):
         SM[n2:, :n2] = M12[:, pos]
         SM[:n2, n2:] = M12[:, pos].T
         G[n2:] = phi.T @ g[:n1]
-    return dict(SK=SK, SM=SM, G=G, lam=lam)
+    out = dict(SK=SK, SM=SM, G=G, lam=lam)
+    if keep:
+        out.update(fronts=fr, B=B, phi=phi, s=s, p2e=p2e, pos=pos)
+    return out
+
+
+def serve(part, sn, ngen, loads, ub):
+    """One request of the served mix on the CPU: the reduction, then for
+    every load case the reduced load vector [f_b + B^T f_i (dofPosIn2 order);
+    Phi^T f_i] and the internal displacements B u_b + K_ii^-1 f_i from the
+    boundary displacements ub (reduced order), all on the same fronts.
+    loads: per case a list of (node 1-based, dof 1-based, value)."""
+    r = reduce(part, sn, ngen, keep=True)
+    s, p2e, pos, B, fr, phi = r["s"], r["p2e"], r["pos"], r["B"], r["fronts"], r["phi"]
+    n1, n2, nlc = s["ndof1"], s["ndof2"], len(loads)
+    F = np.zeros((s["neq"], nlc))
+    for c, case in enumerate(loads):
+        for node, dof, val in case:
+            d = s["madof"][node - 1] - 1 + dof - 1
+            if dof <= s["nndof"][node - 1] and s["meqn"][d] > 0:
+                F[s["meqn"][d] - 1, c] += val
+    Fp = F[p2e]
+    fred = Fp[n1:] + B.T @ Fp[:n1]
+    red = np.zeros((n2 + ngen, nlc))
+    red[:n2] = fred[pos]
+    if ngen:
+        red[n2:] = phi.T @ Fp[:n1]
+    ubp = np.zeros((n2, nlc))
+    ubp[pos] = np.asarray(ub)[:n2]
+    Ui = B @ ubp + fr.solve(Fp[:n1])
+    return r, red, Ui
