@@ -151,7 +151,7 @@ int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags) {
   o.neval = neval < 0 ? 0 : neval;
   o.want_vgi = flags & 1;
   o.no_mass = (flags & 2) != 0;
-  int rc = h->red.analyze(h->part, o, h->err);
+  int rc = h->red.analyze(h->part, o, h->err, true);
   h->analyzed = rc == 0;
   h->reduced = false;
   return rc;

@@ -1450,29 +1450,60 @@ __global__ __launch_bounds__(256) void k_xty_sum(int nchunk, int nab, const doub
 // Sign convention of the component modes: for column i of Phi (rows in
 // permuted order, ld columns), the component of largest magnitude over the
 // internal equations taken in equation order (q -> row p1[q]; the first
-// maximum wins) is made positive.  One workgroup per column.
-__global__ __launch_bounds__(256) void k_mode_signs(int n1, const int* __restrict__ p1, const double* __restrict__ Phi,
-                                                    int ld, double* __restrict__ sign) {
-  __shared__ double sv[256], sa[256];
-  __shared__ int sq[256];
-  const int i = blockIdx.x, tid = threadIdx.x;
-  double bv = 0.0, ba = -1.0;
-  int bq = INT_MAX;
-  for (int q = tid; q < n1; q += 256) {
-    const double v = Phi[(int64_t)p1[q] * ld + i], a = fabs(v);
-    if (a > ba) { ba = a; bv = v; bq = q; }  // q ascends within a thread
-  }
-  sv[tid] = bv; sa[tid] = ba; sq[tid] = bq;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (tid < o) {
-      const double a2 = sa[tid + o];
-      const int q2 = sq[tid + o];
-      if (a2 > sa[tid] || (a2 == sa[tid] && q2 < sq[tid])) { sa[tid] = a2; sv[tid] = sv[tid + o]; sq[tid] = q2; }
+// maximum wins) is made positive.  Pass 1: workgroup g scans equations
+// g*span .. with lane -> column (64 columns at a time) and wave -> every
+// fourth equation, and leaves per column its (|v|, v, q) best; pass 2: one
+// wave per column combines the workgroups' bests (larger |v|, then smaller
+// q), so the choice does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_mode_signs_part(int n1, int span, const int* __restrict__ p1,
+                                                         const double* __restrict__ Phi, int ld, int nev,
+                                                         double* __restrict__ best) {
+  __shared__ double sa[4][64], sv[4][64];
+  __shared__ int sq[4][64];
+  const int g = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int q0 = g * span, q1 = min(n1, q0 + span);
+  for (int j0 = 0; j0 < nev; j0 += 64) {
+    const int i = j0 + lane;
+    double ba = -1.0, bv = 0.0;
+    int bq = INT_MAX;
+    if (i < nev)
+      for (int q = q0 + wv; q < q1; q += 4) {  // q ascends within a lane
+        const double v = Phi[(int64_t)p1[q] * ld + i], a = fabs(v);
+        if (a > ba) { ba = a; bv = v; bq = q; }
+      }
+    sa[wv][lane] = ba;
+    sv[wv][lane] = bv;
+    sq[wv][lane] = bq;
+    __syncthreads();
+    if (wv == 0 && i < nev) {
+      for (int w = 1; w < 4; w++) {
+        const double a2 = sa[w][lane];
+        const int q2 = sq[w][lane];
+        if (a2 > ba || (a2 == ba && q2 < bq)) { ba = a2; bv = sv[w][lane]; bq = q2; }
+      }
+      double* o = best + ((int64_t)g * nev + i) * 3;
+      o[0] = ba;
+      o[1] = bv;
+      o[2] = (double)bq;
     }
     __syncthreads();
   }
-  if (tid == 0) sign[i] = sv[0] < 0.0 ? -1.0 : 1.0;
+}
+
+__global__ __launch_bounds__(64) void k_mode_signs_fin(int ngroups, int nev, const double* __restrict__ best,
+                                                       double* __restrict__ sign) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  double ba = -1.0, bv = 0.0, bq = 1e300;
+  for (int g = lane; g < ngroups; g += 64) {
+    const double* o = best + ((int64_t)g * nev + i) * 3;
+    if (o[0] > ba || (o[0] == ba && o[2] < bq)) { ba = o[0]; bv = o[1]; bq = o[2]; }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double a2 = __shfl_xor(ba, off), v2 = __shfl_xor(bv, off), q2 = __shfl_xor(bq, off);
+    if (a2 > ba || (a2 == ba && q2 < bq)) { ba = a2; bv = v2; bq = q2; }
+  }
+  if (lane == 0) sign[i] = bv < 0.0 ? -1.0 : 1.0;
 }
 
 __global__ void k_scale_cols(int64_t n, int nc, const double* __restrict__ sign, double* __restrict__ X) {
@@ -1919,7 +1950,12 @@ void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, i
 }
 void launch_mode_signs(hipStream_t st, int n1, const int* p1, double* Phi, int64_t nrows, int nev, double* sign) {
   if (nev <= 0) return;
-  k_mode_signs<<<nev, 256, 0, st>>>(n1, p1, Phi, nev, sign);
+  // sign[0 .. nev) is the result; the groups' bests follow it
+  double* best = sign + nev;
+  const int span = std::max(1, (n1 + MODE_SIGN_GROUPS - 1) / MODE_SIGN_GROUPS);
+  const int ng = std::max(1, (n1 + span - 1) / span);
+  k_mode_signs_part<<<ng, 256, 0, st>>>(n1, span, p1, Phi, nev, nev, best);
+  k_mode_signs_fin<<<nev, 64, 0, st>>>(ng, nev, best, sign);
   if (nrows > 0) k_scale_cols<<<nblk(nrows * nev, 256), 256, 0, st>>>(nrows, nev, sign, Phi);
 }
 void launch_emass(hipStream_t st, int n, const int* elist, int nen, int d, const int64_t* eoff, const double* Mp,

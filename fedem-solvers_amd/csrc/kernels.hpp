@@ -109,7 +109,8 @@ void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, i
                        double* Y, int ldy, double alpha, double beta);
 // sign convention of the modes Phi (nrows x nev row-major): largest-magnitude
 // internal component (rows p1[0..n1-1], equation order) made positive;
-// sign: nev doubles of scratch
+// sign: nev * (1 + 3 * MODE_SIGN_GROUPS) doubles of scratch
+constexpr int MODE_SIGN_GROUPS = 256;
 void launch_mode_signs(hipStream_t st, int n1, const int* p1, double* Phi, int64_t nrows, int nev, double* sign);
 void launch_identity_rows(hipStream_t st, double* X, int64_t nrows, int nrhs, int ndof1);
 // Lanczos: G (n x n, n <= 64) = R^T R, R and R^-1 row-major; *bad = 1 if

@@ -37,6 +37,15 @@ namespace fgpu {
 // server gives every slot's worker thread its own counter).
 static thread_local std::atomic<int64_t>* tl_account = nullptr;
 void set_alloc_account(std::atomic<int64_t>* a) { tl_account = a; }
+// Stream the uploads of this thread go through (the handle's own stream while
+// one of its methods runs), so that they never wait for other handles' work
+// the way copies on the null stream would.
+static thread_local hipStream_t tl_stream = nullptr;
+struct StreamScope {
+  hipStream_t prev;
+  explicit StreamScope(hipStream_t s) : prev(tl_stream) { tl_stream = s; }
+  ~StreamScope() { tl_stream = prev; }
+};
 
 template <class T>
 struct DBuf {
@@ -72,7 +81,9 @@ struct DBuf {
   hipError_t upload(const std::vector<T>& v) {
     hipError_t e = alloc(v.size());
     if (e != hipSuccess || v.empty()) return e;
-    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (!tl_stream) return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    e = hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, tl_stream);
+    return e == hipSuccess ? hipStreamSynchronize(tl_stream) : e;  // v may go out of scope after the call
   }
 };
 
@@ -109,6 +120,7 @@ struct Op {
 };
 
 struct SolvePlan {
+  bool valid = false;  // built for the current structure
   int nrhs = 0;
   int nslots = 0;  // 64x64 partial buffers of the split-K solve tasks
   std::vector<Op> fw_int, fw_all, bw;
@@ -120,6 +132,12 @@ struct SolvePlan {
   // work row of the tile) of the internal-only and the full forward pass.
   DBuf<int> ugrp_int, ugrp_all, ugroups;
   DBuf<double> part;
+};
+
+// Host lists of a solve plan before their upload.
+struct PlanHost {
+  std::vector<SolveTask> tasks;
+  std::vector<int> gptr, gsrc, ugrp[2], ugroups;
 };
 
 // A captured launch sequence (HIP graph), replayed while the buffers it
@@ -174,6 +192,7 @@ struct DeviceReducer::Impl {
   // node first equation / equation count / neighbour lists (+ DOF offsets),
   // per row its node, per column its front
   DBuf<int> madof, msc, nndof, drank, etype, nfirst, neqd, nbptr, nbl, nbo, rnode, colsn;
+  DBuf<double> signs;                    // mode signs and the per-group bests of their search
   DBuf<int> ceq_ptr, ceq_node, ceq_dof;  // constraint equations of the dependent DOFs
   DBuf<double> ceq_coef;
   DBuf<int> p2e, pos2;  // perm row -> 0-based equation; dofPosIn2 (for the device-side reduced loads)
@@ -558,7 +577,7 @@ static void build_solve_plan(const Symbolic& y, SolvePlan& sp, std::vector<Solve
   }
 }
 
-int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& err) {
+int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& err, bool upload_now) {
   const auto t0 = std::chrono::steady_clock::now();
   part_ = p;
   opt_ = o;
@@ -587,6 +606,14 @@ int DeviceReducer::analyze(const Part& p, const ReduceOptions& o, std::string& e
   }
   uploaded_ = false;
   times_.analyze_host = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  // upload_now with a device: the upload (and the device structure build and
+  // the solve plans) happens here, on the caller's thread, so that a caller
+  // that analyses the next part while another handle reduces overlaps all of
+  // it; otherwise it waits for the first reduce()
+  if (!upload_now) return 0;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) return upload(err);
+  (void)hipGetLastError();
   return 0;
 }
 
@@ -597,7 +624,7 @@ static hipError_t ensure_side(DeviceReducer::Impl& d) {
   bool fork = false;
   for (const Op& op : d.fops) fork = fork || op.kind == FORK;
   if (!fork || d.side) return hipSuccess;
-  hipError_t e = hipStreamCreate(&d.side);
+  hipError_t e = hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking);
   for (auto& ev : d.fj)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
   return e;
@@ -668,13 +695,28 @@ int DeviceReducer::build_structure(std::string& err) {
 }
 
 // Device upload of the part and of every structure array the kernels read.
+static void lanczos_sizes(int n1, int nev, const ReduceOptions& o, int& bs, int& mcap);
+static SolvePlan* plan_slot(DeviceReducer::Impl& d, int nrhs);
+static void plan_host(const Symbolic& y, SolvePlan& sp, PlanHost& h, int split);
+static int plan_upload(SolvePlan& sp, const PlanHost& h, std::string& err);
+
 int DeviceReducer::upload(std::string& err) {
   const auto t0 = std::chrono::steady_clock::now();
+  // FGPU_TRACE_REDUCE=1: host-side timeline of the upload on stderr
+  const bool trace = std::getenv("FGPU_TRACE_REDUCE") != nullptr;
+  auto mark = [&](const char* what) {
+    if (trace)
+      std::fprintf(stderr, "upload +%.2f ms: %s\n",
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(), what);
+  };
   Impl& d = *d_;
   if (!st_) {
-    HIPCHK(hipStreamCreate(&st_));
+    // non-blocking: no implicit ordering with the null stream (and so with
+    // other threads' null-stream copies); default priority
+    HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
     for (auto& e : d.ev) HIPCHK(hipEventCreate(&e));
   }
+  StreamScope scope(st_);
   HIPCHK(d.xyz.upload(part_.xyz));
   HIPCHK(d.eprop.upload(part_.eprop));
   HIPCHK(d.eptr.upload(part_.eptr));
@@ -688,6 +730,7 @@ int DeviceReducer::upload(std::string& err) {
   HIPCHK(d.shlist.upload(d.shl));
   HIPCHK(d.bmlist.upload(d.bml));
   HIPCHK(d.eoff.upload(sym_.elem_off));
+  mark("part arrays");
   const Symbolic& y = sym_;
   HIPCHK(d.col0.upload(y.col0));
   HIPCHK(d.ncol.upload(y.ncol));
@@ -723,21 +766,51 @@ int DeviceReducer::upload(std::string& err) {
   d.S = DevSym{d.col0.p, d.ncol.p, d.nrow.p, d.foff.p, d.rptr.p, d.rows.p, d.chptr.p,
                d.chlist.p, d.relptr.p, d.rel.p, d.woff.p, d.dpiv.p,
                d.adiag.p, d.amax.p, d.sing.p, std::max(opt_.tol_factorize, 1e-20)};
-  if (int rc = build_structure(err)) return rc;
+  // The host-side schedules (the factorisation's and the solve plans the
+  // reduction uses) are built on host threads while the device builds the
+  // structure.  The solve plans keep their device buffers: freeing them
+  // would wait for the whole device.
+  for (auto& kv : d.plans) kv.second->valid = false;
+  std::vector<int> want;
+  {
+    const int n1 = y.ndof1, n2 = y.ndof2;
+    const int nev = opt_.ngen > 0 ? std::max(opt_.neval, opt_.ngen) : 0;
+    if (n1 > 0 && n2 > 0) want.push_back(n2);
+    if (n1 > 0 && nev > 0) {
+      int bs = 0, mcap = 0;
+      lanczos_sizes(n1, nev, opt_, bs, mcap);
+      if (std::find(want.begin(), want.end(), bs) == want.end()) want.push_back(bs);
+    }
+    if (n1 > 0 && opt_.want_vgi && std::find(want.begin(), want.end(), 3) == want.end()) want.push_back(3);
+  }
+  std::vector<SolvePlan*> sps;
+  for (int nr : want) sps.push_back(plan_slot(d, nr));
+  std::vector<PlanHost> phs(sps.size());
   std::vector<FTask> ft;
   d.fops.clear();
   std::vector<int> xr;
   int inv_slots = 0;
-  build_factor_schedule(y, ft, d.fops, xr, upd_flops_, std::min(256, std::max(64, (opt_.factor_panel / 64) * 64)),
-                        inv_slots, opt_.factor_lookahead);
+  const int panel = std::min(256, std::max(64, (opt_.factor_panel / 64) * 64));
+  std::vector<std::future<void>> host_jobs;
+  host_jobs.push_back(std::async(std::launch::async, [&]() {
+    build_factor_schedule(y, ft, d.fops, xr, upd_flops_, panel, inv_slots, opt_.factor_lookahead);
+  }));
+  for (size_t i = 0; i < sps.size(); i++)
+    host_jobs.push_back(std::async(std::launch::async, [&, i]() { plan_host(y, *sps[i], phs[i], d.solve_split); }));
+  const int src = build_structure(err);
+  for (auto& f : host_jobs) f.get();
+  if (src) return src;
+  mark("structure built");
+  for (size_t i = 0; i < sps.size(); i++)
+    if (int rc = plan_upload(*sps[i], phs[i], err)) return rc;
   HIPCHK(ensure_side(d));
   HIPCHK(d.ftasks.upload(ft));
+  mark("factor schedule built");
   HIPCHK(d.xrange.upload(xr));
   HIPCHK(d.ipart.alloc((size_t)std::max(1, inv_slots) * 4096));
   upd_launches_ = 0;
   for (auto& op : d.fops)
     if (op.kind == UPDATE || op.kind == UPDATE_SIDE) upd_launches_++;
-  d.plans.clear();
   // a new structure: every captured launch sequence is stale
   d.g_factor.reset();
   d.factor_runs = 0;
@@ -761,6 +834,7 @@ int DeviceReducer::upload(std::string& err) {
   HIPCHK(d.ebad.alloc(part_.nel));
   HIPCHK(d.emass.alloc(part_.nel + 1));
   HIPCHK(d.piv.alloc(6));
+  mark("value buffers");
   // gravity directions: unit accelerations on translational equation DOFs
   std::vector<double> a((size_t)y.neq * 3, 0.0);
   for (int i = 0; i < sam_.nnod; i++)
@@ -771,34 +845,119 @@ int DeviceReducer::upload(std::string& err) {
   HIPCHK(d.grav.upload(a));
   HIPCHK(d.p2e.upload(y.perm_to_eq));
   HIPCHK(d.pos2.upload(sam_.dofPosIn2.empty() ? std::vector<int>(1, 1) : sam_.dofPosIn2));
+  mark("gravity and maps");
+  // the solve plans and work buffers the reduction will use, made here (on
+  // the caller's analysis thread) instead of on the reduction's path
+  if (int rc = prepare_solves(err)) return rc;
+  mark("solve plans");
   uploaded_ = true;
   times_.upload = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
 
+// The slot of the plan for nrhs right-hand sides: the existing plan (its
+// device buffers reused, its host lists cleared) or a new one in the map.
+static SolvePlan* plan_slot(DeviceReducer::Impl& d, int nrhs) {
+  auto it = d.plans.find(nrhs);
+  SolvePlan* sp = nullptr;
+  if (it != d.plans.end()) {
+    sp = it->second.get();
+    sp->nslots = 0;
+    sp->fw_int.clear();
+    sp->fw_all.clear();
+    sp->bw.clear();
+  } else {
+    auto fresh = std::make_unique<SolvePlan>();
+    sp = fresh.get();
+    d.plans[nrhs] = std::move(fresh);
+  }
+  sp->valid = false;
+  sp->nrhs = nrhs;
+  return sp;
+}
+
+// Host part of a plan (pure host work, safe on another thread).
+static void plan_host(const Symbolic& y, SolvePlan& sp, PlanHost& h, int split) {
+  build_solve_plan(y, sp, h.tasks, h.gptr, h.gsrc, h.ugrp, h.ugroups, split);
+}
+
+static int plan_upload(SolvePlan& sp, const PlanHost& h, std::string& err) {
+  HIPCHK(sp.tasks.upload(h.tasks));
+  HIPCHK(sp.gptr.upload(h.gptr));
+  HIPCHK(sp.gsrc.upload(h.gsrc.empty() ? std::vector<int>(1, 0) : h.gsrc));
+  HIPCHK(sp.ugrp_int.upload(h.ugrp[0]));
+  HIPCHK(sp.ugrp_all.upload(h.ugrp[1]));
+  HIPCHK(sp.ugroups.upload(h.ugroups.empty() ? std::vector<int>(3, 0) : h.ugroups));
+  HIPCHK(sp.part.alloc((size_t)std::max(1, sp.nslots) * 4096));
+  sp.valid = true;
+  return 0;
+}
+
 static int get_plan(const Symbolic& y, DeviceReducer::Impl& d, int nrhs, SolvePlan*& out, std::string& err) {
   auto it = d.plans.find(nrhs);
-  if (it != d.plans.end()) { out = it->second.get(); return 0; }
-  auto sp = std::make_unique<SolvePlan>();
-  sp->nrhs = nrhs;
-  std::vector<SolveTask> tasks;
-  std::vector<int> gptr, gsrc, ugrp[2], ugroups;
-  build_solve_plan(y, *sp, tasks, gptr, gsrc, ugrp, ugroups, d.solve_split);
-  HIPCHK(sp->tasks.upload(tasks));
-  HIPCHK(sp->gptr.upload(gptr));
-  HIPCHK(sp->gsrc.upload(gsrc.empty() ? std::vector<int>(1, 0) : gsrc));
-  HIPCHK(sp->ugrp_int.upload(ugrp[0]));
-  HIPCHK(sp->ugrp_all.upload(ugrp[1]));
-  HIPCHK(sp->ugroups.upload(ugroups.empty() ? std::vector<int>(3, 0) : ugroups));
-  HIPCHK(sp->part.alloc((size_t)std::max(1, sp->nslots) * 4096));
-  out = sp.get();
-  d.plans[nrhs] = std::move(sp);
+  if (it != d.plans.end() && it->second->valid) { out = it->second.get(); return 0; }
+  SolvePlan* sp = plan_slot(d, nrhs);
+  PlanHost h;
+  plan_host(y, *sp, h, d.solve_split);
+  if (int rc = plan_upload(*sp, h, err)) return rc;
+  out = sp;
   return 0;
 }
 
 static int ensure_work(DeviceReducer::Impl& d, int nrhs, std::string& err) {
   HIPCHK(d.W.alloc((size_t)d.wrows * nrhs));
   HIPCHK(d.V.alloc((size_t)d.wrows * nrhs));
+  return 0;
+}
+
+// Lanczos block size and basis capacity for nev modes of n1 internal
+// equations: blocks of lanczos_block vectors (<= 64, the Gram matrices' limit),
+// lanczosMaxVectors or max(10 nev + 40, 16 blocks) vectors, at least nev, in
+// whole blocks; a basis that would span the whole internal space takes a
+// block size dividing n1, so that it does so exactly.
+static void lanczos_sizes(int n1, int nev, const ReduceOptions& o, int& bs, int& mcap) {
+  bs = std::max(1, std::min(std::min(o.lanczos_block, 64), n1));
+  mcap = o.lanczos_maxvec > 0 ? o.lanczos_maxvec : std::max(10 * nev + 40, 16 * bs);
+  mcap = (std::max(mcap, nev) + bs - 1) / bs * bs;
+  if (mcap >= n1) {
+    while (n1 % bs) bs--;
+    mcap = n1;
+  }
+}
+
+int DeviceReducer::prepare_solves(std::string& err) {
+  Impl& d = *d_;
+  const Symbolic& y = sym_;
+  const int n1 = y.ndof1, n2 = y.ndof2, neq = y.neq;
+  const int nev = opt_.ngen > 0 ? std::max(opt_.neval, opt_.ngen) : 0;
+  if (n1 <= 0) return 0;
+  SolvePlan* p = nullptr;
+  if (n2 > 0) {
+    if (get_plan(y, d, n2, p, err)) return -3;
+    if (ensure_work(d, n2, err)) return -3;
+    HIPCHK(d.X.alloc((size_t)neq * n2));
+    HIPCHK(d.Z.alloc((size_t)neq * n2));
+    HIPCHK(d.Y.alloc((size_t)neq * n2));
+  }
+  if (nev > 0) {
+    int bs = 0, mcap = 0;
+    lanczos_sizes(n1, nev, opt_, bs, mcap);
+    if (get_plan(y, d, bs, p, err)) return -3;
+    if (ensure_work(d, std::max(bs, n2), err)) return -3;
+    HIPCHK(d.Q.alloc((size_t)neq * mcap));
+    HIPCHK(d.P.alloc((size_t)neq * mcap));
+    HIPCHK(d.Wl.alloc((size_t)neq * bs));
+    HIPCHK(d.Zl.alloc((size_t)neq * bs));
+    HIPCHK(d.Yl.alloc((size_t)neq * bs));
+    HIPCHK(d.Tmp.alloc((size_t)neq * bs));
+    HIPCHK(d.Cc.alloc((size_t)mcap * std::max(bs, nev) + 64));
+    HIPCHK(d.part.alloc((size_t)xty_chunks(neq) * mcap * std::max(bs, nev)));
+    HIPCHK(d.small.alloc((size_t)mcap * mcap + 64));
+    HIPCHK(d.Cacc.alloc((size_t)mcap * bs));
+    HIPCHK(d.Rdev.alloc((size_t)2 * bs * bs));
+    HIPCHK(d.signs.alloc((size_t)nev * (1 + 3 * MODE_SIGN_GROUPS)));
+  }
+  if (opt_.want_vgi && get_plan(y, d, 3, p, err)) return -3;
   return 0;
 }
 
@@ -952,6 +1111,15 @@ int DeviceReducer::reduce(std::string& err) {
     int rc = upload(err);
     if (rc) return rc;
   }
+  StreamScope scope(st_);
+  // FGPU_TRACE_REDUCE=1: host-side timeline of the phases on stderr
+  const bool trace = std::getenv("FGPU_TRACE_REDUCE") != nullptr;
+  const auto tr0 = std::chrono::steady_clock::now();
+  auto mark = [&](const char* what) {
+    if (trace)
+      std::fprintf(stderr, "reduce +%.2f ms: %s\n",
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count(), what);
+  };
   Impl& d = *d_;
   const Symbolic& y = sym_;
   const int neq = y.neq, n1 = y.ndof1, n2 = y.ndof2, schur = y.nsup - 1;
@@ -967,6 +1135,7 @@ int DeviceReducer::reduce(std::string& err) {
   res_.nev = nev;
   res_.ndim = sam_.ndim_ext + ngen;
   HIPCHK(hipEventRecord(d.ev[0], st));
+  mark("start issued");
   // ---- element matrices
   HIPCHK(hipMemsetAsync(d.ebad.p, 0, sizeof(int) * part_.nel, st));
   launch_hex8(st, (int)d.hexl.size(), d.hexlist.p, d.eptr.p, d.enodes.p, d.xyz.p, d.eprop.p, d.eoff.p, d.Kp.p,
@@ -1000,6 +1169,7 @@ int DeviceReducer::reduce(std::string& err) {
   launch_diag_extract(st, n1, d.rowptr.p, d.colind.p, d.Kv.p, d.mixed ? d.dscale.p : nullptr, d.adiag.p, d.amax.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[2], st));
+  mark("elements and assembly issued");
   // ---- multifrontal LDL^T; the Schur front ends up holding K_bb - K_bi K_ii^-1 K_ib
   {
     int init[6] = {0, 0, 0, INT_MAX, INT_MAX, INT_MAX};
@@ -1074,6 +1244,7 @@ int DeviceReducer::reduce(std::string& err) {
   launch_pivot_stats(st, n1, d.dpiv.p, d.sing.p, d.piv.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[3], st));
+  mark("factor issued");
   // ---- B = -K_ii^{-1} K_ib: back-substitution of [0; I].  With the FP64
   // factor and component modes it is queued after the Lanczos steps instead,
   // where it runs on the device while the host solves the final projected
@@ -1143,6 +1314,7 @@ int DeviceReducer::reduce(std::string& err) {
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(d.ev[4], st));
+  mark("B issued");
   // error checks of the numeric phase
   int hbad[6] = {0, 0, 0, 0, 0, 0};
   HIPCHK(hipMemcpyAsync(hbad, d.piv.p, sizeof hbad, hipMemcpyDeviceToHost, st));
@@ -1161,7 +1333,8 @@ int DeviceReducer::reduce(std::string& err) {
   res_.n_reduced_zero = hbad[2];
   if (hbad[1] + hbad[2] > 0) {
     std::vector<int> sg(n1);
-    HIPCHK(hipMemcpy(sg.data(), d.sing.p, sizeof(int) * n1, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(sg.data(), d.sing.p, sizeof(int) * n1, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     for (int c = 0; c < n1; c++)
       if (sg[c]) res_.singular.push_back({y.perm_to_eq[c], sg[c]});
     std::sort(res_.singular.begin(), res_.singular.end());
@@ -1179,16 +1352,8 @@ int DeviceReducer::reduce(std::string& err) {
   std::vector<double> Sres;
   if (nev > 0) {
     int rc = 0;
-    int bs = std::max(1, std::min(std::min(opt_.lanczos_block, 64), n1));  // block Gram matrices <= 64 x 64
-    int mcap = opt_.lanczos_maxvec > 0 ? opt_.lanczos_maxvec : std::max(10 * nev + 40, 16 * bs);
-    // at least nev vectors (nev <= n1), in whole blocks
-    mcap = (std::max(mcap, nev) + bs - 1) / bs * bs;
-    if (mcap >= n1) {
-      // the basis may span the whole internal space: a block size that
-      // divides n1 lets it do so exactly (then the Ritz pairs are exact)
-      while (n1 % bs) bs--;
-      mcap = n1;
-    }
+    int bs = 0, mcap = 0;
+    lanczos_sizes(n1, nev, opt_, bs, mcap);
     SolvePlan* pl = nullptr;
     if (get_plan(y, d, bs, pl, err)) return -3;
     if (ensure_work(d, std::max(bs, n2), err)) return -3;
@@ -1454,9 +1619,11 @@ int DeviceReducer::reduce(std::string& err) {
     HIPCHK(d.Phi.alloc((size_t)neq * nev));
     launch_gemm_small(st, neq, m, nev, d.Q.p, mcap, d.small.p, d.Phi.p, nev, 0.0, 1.0);
     // sign convention: the largest-magnitude component of every mode is positive
-    launch_mode_signs(st, n1, d.p1.p, d.Phi.p, neq, nev, d.Cc.p);
+    HIPCHK(d.signs.alloc((size_t)nev * (1 + 3 * MODE_SIGN_GROUPS)));
+    launch_mode_signs(st, n1, d.p1.p, d.Phi.p, neq, nev, d.signs.p);
   }
   HIPCHK(hipEventRecord(d.ev[5], st));
+  mark("modes done on host");
   // ---- reduced mass: Z = M [B; I], M22 = [B; I]^T Z ; modes: K11, M11, M12
   std::vector<double> M22, K11, M11, M12, Ge, Gg, S22((size_t)n2 * n2);
   const int maxab = std::max(std::max(n2, 3), std::max(nev, 1));
@@ -1510,6 +1677,7 @@ int DeviceReducer::reduce(std::string& err) {
     }
   }
   HIPCHK(hipEventRecord(d.ev[7], st));
+  mark("mass and gravity issued");
   // ---- Schur complement out of the root front
   if (n2 > 0 && d.mixed) S22 = d.s22;
   else if (n2 > 0) {
@@ -1518,6 +1686,7 @@ int DeviceReducer::reduce(std::string& err) {
   if (!opt_.no_mass)
     HIPCHK(hipMemcpyAsync(&res_.mass_fe, d.emass.p + part_.nel, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  mark("device done");
   times_.elements = ev_ms(d.ev[0], d.ev[1]);
   times_.assembly = ev_ms(d.ev[1], d.ev[2]);
   times_.factor = ev_ms(d.ev[2], d.ev[3]);
@@ -1568,12 +1737,18 @@ int DeviceReducer::reduce(std::string& err) {
   return 0;
 }
 
+hipError_t DeviceReducer::d2h(void* dst, const void* src, size_t bytes) const {
+  if (!st_) return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st_);
+  return e == hipSuccess ? hipStreamSynchronize(st_) : e;
+}
+
 int DeviceReducer::get_B(std::vector<double>& B) {
   const Symbolic& y = sym_;
   const int n1 = y.ndof1, n2 = y.ndof2;
   std::vector<double> x((size_t)y.neq * n2);
   if (n2 == 0) { B.clear(); return 0; }
-  if (hipMemcpy(x.data(), d_->X.p, sizeof(double) * x.size(), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (d2h(x.data(), d_->X.p, sizeof(double) * x.size()) != hipSuccess) return -3;
   B.assign((size_t)n1 * n2, 0.0);
   for (int q = 0; q < n1; q++) {
     int r = y.eq_to_perm[sam_.meqn1[q] - 1];
@@ -1588,7 +1763,7 @@ int DeviceReducer::get_phi(std::vector<double>& P) {
   P.assign((size_t)n1 * nev, 0.0);
   if (nev == 0) return 0;
   std::vector<double> x((size_t)y.neq * nev);
-  if (hipMemcpy(x.data(), d_->Phi.p, sizeof(double) * x.size(), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (d2h(x.data(), d_->Phi.p, sizeof(double) * x.size()) != hipSuccess) return -3;
   for (int q = 0; q < n1; q++) {
     int r = y.eq_to_perm[sam_.meqn1[q] - 1];
     for (int c = 0; c < nev; c++) P[q + (size_t)c * n1] = x[(size_t)r * nev + c];
@@ -1606,7 +1781,7 @@ int DeviceReducer::get_schur(std::vector<double>& S) {
     S = d_->s22;
     return 0;
   }
-  if (hipMemcpy(f.data(), d_->pool.p + y.foff[y.nsup - 1], sizeof(double) * f.size(), hipMemcpyDeviceToHost) != hipSuccess)
+  if (d2h(f.data(), d_->pool.p + y.foff[y.nsup - 1], sizeof(double) * f.size()) != hipSuccess)
     return -3;
   for (int j = 0; j < n2; j++)
     for (int i = 0; i < n2; i++) S[i + (size_t)j * n2] = i >= j ? f[i + (size_t)j * n2] : f[j + (size_t)i * n2];
@@ -1621,21 +1796,22 @@ int DeviceReducer::get_csr(std::vector<int>& rowptr, std::vector<int>& colind, s
   colind.resize(nnz);
   K.resize(nnz);
   M.resize(nnz);
-  if (hipMemcpy(colind.data(), d_->colind.p, sizeof(int) * nnz, hipMemcpyDeviceToHost) != hipSuccess) return -3;
-  if (hipMemcpy(K.data(), d_->Kv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost) != hipSuccess) return -3;
-  if (hipMemcpy(M.data(), d_->Mv.p, sizeof(double) * nnz, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (d2h(colind.data(), d_->colind.p, sizeof(int) * nnz) != hipSuccess) return -3;
+  if (d2h(K.data(), d_->Kv.p, sizeof(double) * nnz) != hipSuccess) return -3;
+  if (d2h(M.data(), d_->Mv.p, sizeof(double) * nnz) != hipSuccess) return -3;
   return 0;
 }
 
 int DeviceReducer::get_element_matrices(std::vector<double>& Kp, std::vector<double>& Mp) {
   Kp.resize(sym_.elem_pool);
   Mp.resize(sym_.elem_pool);
-  if (hipMemcpy(Kp.data(), d_->Kp.p, sizeof(double) * Kp.size(), hipMemcpyDeviceToHost) != hipSuccess) return -3;
-  if (hipMemcpy(Mp.data(), d_->Mp.p, sizeof(double) * Mp.size(), hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (d2h(Kp.data(), d_->Kp.p, sizeof(double) * Kp.size()) != hipSuccess) return -3;
+  if (d2h(Mp.data(), d_->Mp.p, sizeof(double) * Mp.size()) != hipSuccess) return -3;
   return 0;
 }
 
 int DeviceReducer::solve_kii(int nrhs, const double* R_eq, double* X_eq, std::string& err) {
+  StreamScope scope(st_);
   Impl& d = *d_;
   const Symbolic& y = sym_;
   const int neq = y.neq, n1 = y.ndof1;
@@ -1662,6 +1838,7 @@ int DeviceReducer::solve_kii(int nrhs, const double* R_eq, double* X_eq, std::st
 }
 
 int DeviceReducer::gravity_internal(std::vector<double>& gi, std::string& err) {
+  StreamScope scope(st_);
   Impl& d = *d_;
   const Symbolic& y = sym_;
   const int neq = y.neq, n1 = y.ndof1;
@@ -1681,6 +1858,7 @@ int DeviceReducer::gravity_internal(std::vector<double>& gi, std::string& err) {
 }
 
 int DeviceReducer::reduce_loads(int nrhs, const double* F_eq, std::vector<double>& out, std::string& err) {
+  StreamScope scope(st_);
   Impl& d = *d_;
   const Symbolic& y = sym_;
   const int neq = y.neq, n2 = y.ndof2, ngen = res_.ngen, nev = res_.nev, nd = res_.ndim;
@@ -1720,6 +1898,7 @@ int DeviceReducer::reduce_loads(int nrhs, const double* F_eq, std::vector<double
 // static scratch array: no device allocation.  F (neq x nrhs) and Fred
 // (ndim x nrhs) are column-major device arrays.
 int DeviceReducer::reduce_loads_noalloc(int nrhs, const double* F, double* Fred, std::string& err) {
+  StreamScope scope(st_);
   Impl& d = *d_;
   const Symbolic& y = sym_;
   const int neq = y.neq, n2 = y.ndof2, ngen = res_.ngen, nev = res_.nev, nd = res_.ndim;
@@ -1748,6 +1927,7 @@ int DeviceReducer::reduce_loads_noalloc(int nrhs, const double* F, double* Fred,
 }
 
 int DeviceReducer::recover(int nrhs, const double* Ub, const double* Fi, double* Ui, std::string& err) {
+  StreamScope scope(st_);
   Impl& d = *d_;
   const Symbolic& y = sym_;
   const int neq = y.neq, n1 = y.ndof1, n2 = y.ndof2;

@@ -78,9 +78,13 @@ class DeviceReducer {
   ~DeviceReducer();
   // Host analysis (SAM, symbolic, schedules) and upload of the part and all
   // structure arrays.  Called once per part topology.
-  int analyze(const Part& p, const ReduceOptions& o, std::string& err);
+  // upload_now: also upload the part when a device is present (otherwise the
+  // first reduce() does).
+  int analyze(const Part& p, const ReduceOptions& o, std::string& err, bool upload_now = false);
   // Device upload of the part and structure (first reduce() does it too).
   int upload(std::string& err);
+  // The solve plans and work buffers of the reduction (end of upload).
+  int prepare_solves(std::string& err);
   // One full numeric reduction on the device from the resident part data.
   int reduce(std::string& err);
   // Copies of device results in the reference layout (on demand).
@@ -131,6 +135,8 @@ class DeviceReducer {
   Results res_;
   PhaseTimes times_;
   hipStream_t st_ = nullptr;
+  // device-to-host copy ordered after the handle's stream work
+  hipError_t d2h(void* dst, const void* src, size_t bytes) const;
   bool uploaded_ = false;
   double upd_flops_ = 0;
   int64_t upd_launches_ = 0;
