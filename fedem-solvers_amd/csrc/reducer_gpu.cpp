@@ -193,6 +193,9 @@ struct DeviceReducer::Impl {
   // per row its node, per column its front
   DBuf<int> madof, msc, nndof, drank, etype, nfirst, neqd, nbptr, nbl, nbo, rnode, colsn;
   DBuf<double> signs;                    // mode signs and the per-group bests of their search
+  // grow-only work arrays of the load-case calls (freeing device memory
+  // waits for the whole device, so they are kept between calls)
+  DBuf<double> sk_r, sk_t, sk_x, rl_f, rc_s, rc_y, gi_g;
   DBuf<int> ceq_ptr, ceq_node, ceq_dof;  // constraint equations of the dependent DOFs
   DBuf<double> ceq_coef;
   DBuf<int> p2e, pos2;  // perm row -> 0-based equation; dofPosIn2 (for the device-side reduced loads)
@@ -1818,7 +1821,9 @@ int DeviceReducer::solve_kii(int nrhs, const double* R_eq, double* X_eq, std::st
   SolvePlan* p = nullptr;
   if (get_plan(y, d, nrhs, p, err)) return -3;
   if (ensure_work(d, std::max(nrhs, std::max(y.ndof2, opt_.lanczos_block)), err)) return -3;
-  DBuf<double> r, t, x;
+  DBuf<double>& r = d.sk_r;
+  DBuf<double>& t = d.sk_t;
+  DBuf<double>& x = d.sk_x;
   std::vector<double> h((size_t)neq * nrhs, 0.0);
   for (int q = 0; q < n1; q++) {
     int rr = y.eq_to_perm[sam_.meqn1[q] - 1];
@@ -1843,7 +1848,7 @@ int DeviceReducer::gravity_internal(std::vector<double>& gi, std::string& err) {
   const Symbolic& y = sym_;
   const int neq = y.neq, n1 = y.ndof1;
   if (opt_.no_mass) { err = "gravity forces need the mass matrix"; return -1; }
-  DBuf<double> g;
+  DBuf<double>& g = d.gi_g;
   HIPCHK(g.alloc((size_t)neq * 3));
   launch_spmm(st_, neq, 3, d.rowptr.p, d.colind.p, d.Mv.p, d.grav.p, 3, g.p, 3, d.m_lumped);
   std::vector<double> h((size_t)neq * 3);
@@ -1869,7 +1874,7 @@ int DeviceReducer::reduce_loads(int nrhs, const double* F_eq, std::vector<double
     int rr = y.eq_to_perm[e];
     for (int c = 0; c < nrhs; c++) h[(size_t)rr * nrhs + c] = F_eq[e + (size_t)c * neq];
   }
-  DBuf<double> f;
+  DBuf<double>& f = d.rl_f;
   HIPCHK(f.upload(h));
   const int maxab = std::max(std::max(n2, ngen), nrhs);
   HIPCHK(d.part.alloc((size_t)xty_chunks(neq) * maxab * maxab));
@@ -1937,7 +1942,8 @@ int DeviceReducer::recover(int nrhs, const double* Ub, const double* Fi, double*
     // Y = [B; I] U_b in permuted rows, the boundary block of U_b in LDS-sized
     // column chunks
     const int cj = std::max(1, std::min(nrhs, 8192 / std::max(n2, 1)));
-    DBuf<double> S, Yd;
+    DBuf<double>& S = d.rc_s;
+    DBuf<double>& Yd = d.rc_y;
     HIPCHK(Yd.alloc((size_t)neq * cj));
     std::vector<double> hs((size_t)n2 * cj), hy((size_t)neq * cj);
     for (int j0 = 0; j0 < nrhs; j0 += cj) {
