@@ -13,10 +13,12 @@ map, then element matrices, assembly, the LDL^T factorisation with the Schur
 complement, B, the block-Lanczos component modes, the mass / stiffness /
 gravity reduction and the reduced matrices copied back to the host.  The
 part's coordinates, topology and properties are in host memory when a step
-starts (the reducer's input); two reducer handles alternate so that the
-analysis of the next part overlaps the device work of the current one.
-Reported beside it: the same path with nothing overlapped (per-part
-latency) and the numeric re-reduction of an already analysed part.
+starts (the reducer's input).  Parts stream through --pipelines worker
+threads (default 4), each with its own reducer handle and HIP stream, so
+that a part's host analysis and its latency-bound solve phases overlap
+other parts' device work.  Reported beside it: the same path with nothing
+overlapped (per-part latency) and the numeric re-reduction of an already
+analysed part.
 
 Prints one JSON line (see the keys below).  --gpus N > 1 exits: the
 reducer runs on one GPU.
@@ -26,6 +28,7 @@ import json
 import os
 import subprocess
 import sys
+import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 
@@ -241,6 +244,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--pipelines", type=int, default=4,
+                    help="worker threads, each with its own reducer handle, taking the parts of the stream")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision plate measurement")
@@ -281,37 +286,48 @@ def main():
     # ordering, fronts: csAllocPointerArrays/csRenumber/csPreassemble), the
     # upload of the part and its structure, the device build of the CSR
     # pattern, contribution lists and front map, then the numeric reduction
-    # and the reduced matrices back on the host.  Two reducer handles
-    # alternate so that the host analysis of the next part overlaps the
-    # device work of the current one.
-    Rs = [gpu.GpuReducer(), gpu.GpuReducer()]
+    # and the reduced matrices back on the host.  `pipelines` worker threads,
+    # each with its own reducer handle (and HIP stream), take the parts one
+    # after the other, so that one part's host analysis and its latency-bound
+    # solve phases overlap other parts' device work.
+    npipe = max(1, a.pipelines)
+    Rs = [gpu.GpuReducer() for _ in range(npipe)]
     for R in Rs:
         R.set_part(part)
     tuned = None
     if a.autotune:  # one-time tile tuning on this part, outside the timed region
         Rs[0].analyze(ngen=20)
         tuned = Rs[0].autotune()
-        Rs[1].set_option("factorPanel", tuned["factorPanel"])
-        Rs[1].set_option("solveSplitK", tuned["solveSplitK"])
-    pool = ThreadPoolExecutor(1)
+        for R in Rs:
+            R.set_option("factorPanel", tuned["factorPanel"])
+            R.set_option("solveSplitK", tuned["solveSplitK"])
+    pool = ThreadPoolExecutor(npipe)
 
     def stream(n):
-        """n parts, analysis of part k+1 overlapping part k's reduction"""
-        fut = pool.submit(Rs[0].analyze, ngen=20)
-        for k in range(n):
-            R = Rs[k % 2]
-            sizes = fut.result()
-            if k + 1 < n:
-                fut = pool.submit(Rs[(k + 1) % 2].analyze, ngen=20)
-            R.reduce()
-            R.reduced()
-        return sizes
+        """n parts through the pipelines (each part: analyze, reduce, results)"""
+        left = [n]
+        lock = threading.Lock()
+        sizes = {}
 
-    stream(max(2, a.warmup))
+        def worker(i):
+            R = Rs[i]
+            while True:
+                with lock:
+                    if left[0] == 0:
+                        return
+                    left[0] -= 1
+                sizes[i] = R.analyze(ngen=20)
+                R.reduce()
+                R.reduced()
+        for f in [pool.submit(worker, i) for i in range(npipe)]:
+            f.result()
+        return next(iter(sizes.values()))
+
+    stream(max(2 * npipe, a.warmup))
     t0 = time.perf_counter()
     sizes = stream(a.steps)
     el = time.perf_counter() - t0
-    R = Rs[(a.steps - 1) % 2]
+    R = Rs[0]
     phases = R.times()
     info = R.info()
     # the same path with nothing overlapped: per-part latency
@@ -357,7 +373,7 @@ def main():
         "data": "synthetic (seeded part generator)",
         "config": {"workload": "48x24x24 HEX8 block, 91,911 DOF, 72 boundary DOF, 20 component modes, FP64",
                    "neq": sizes["neq"], "ndof2": sizes["ndof2"], "modes": 20, "seed": a.seed,
-                   "parallelism": "1 GPU", "tiles": tuned or "defaults"},
+                   "parallelism": "1 GPU, %d part pipelines" % npipe, "tiles": tuned or "defaults"},
         "roofline": {"bound": "mfma", "kernel": "k_update (trailing LDL^T update, FP64 MFMA)",
                      "achieved": achieved, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_MFMA, "traffic": measured_traffic(),
