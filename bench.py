@@ -14,7 +14,7 @@ complement, B, the block-Lanczos component modes, the mass / stiffness /
 gravity reduction and the reduced matrices copied back to the host.  The
 part's coordinates, topology and properties are in host memory when a step
 starts (the reducer's input).  Parts stream through --pipelines worker
-threads (default 4), each with its own reducer handle and HIP stream, so
+threads (default 6), each with its own reducer handle and HIP stream, so
 that a part's host analysis and its latency-bound solve phases overlap
 other parts' device work.  Reported beside it: the same path with nothing
 overlapped (per-part latency) and the numeric re-reduction of an already
@@ -241,10 +241,10 @@ def served(n_req, slots, seed):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--pipelines", type=int, default=4,
+    ap.add_argument("--pipelines", type=int, default=6,
                     help="worker threads, each with its own reducer handle, taking the parts of the stream")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched and served measurements")
