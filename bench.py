@@ -244,6 +244,7 @@ def main():
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--slots", type=int, default=16, help="server slots of the batched and served measurements")
     ap.add_argument("--pipelines", type=int, default=6,
                     help="worker threads, each with its own reducer handle, taking the parts of the stream")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -267,7 +268,7 @@ def main():
         print(json.dumps({"mixed_precision": mixed_plate(a.seed)}))
         return
     if a.only == "serving":
-        print(json.dumps({"batched": batched(256, 16, a.seed), "served": served(160, 16, a.seed)}))
+        print(json.dumps({"batched": batched(256, a.slots, a.seed), "served": served(160, a.slots, a.seed)}))
         return
     if a.only == "cpu":
         print(json.dumps({"cpu_baselines": cpu_baselines(a.seed)}))
@@ -394,8 +395,8 @@ def main():
     if not a.no_mixed:
         line["mixed_precision"] = mixed_plate(a.seed)
     if not a.no_serving:
-        line["batched"] = batched(256, 16, a.seed)
-        line["served"] = served(160, 16, a.seed)
+        line["batched"] = batched(256, a.slots, a.seed)
+        line["served"] = served(160, a.slots, a.seed)
     if not a.no_cpu_baseline:
         cb = cpu_baseline(a.seed)
         line["cpu_baseline"] = {

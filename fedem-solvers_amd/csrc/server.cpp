@@ -58,10 +58,13 @@ double ms(Clock::time_point a, Clock::time_point b) {
 // `bytes` of device memory whether it is running or idle; a running slot
 // holds at least its request's estimated footprint.  A request is admitted
 // when the memory of all slots, with its own slot at max(held, footprint),
-// fits the budget; if nothing else runs it is admitted after the idle slots
-// have released their buffers (a request larger than the whole budget then
-// runs alone).  A slot that holds more than its share of the budget
-// (budget / slots) after a request releases its buffers.
+// fits the budget; otherwise idle slots release their buffers, the largest
+// holders first, until it fits; if it still does not, it waits for running
+// requests, and when nothing else runs it is admitted anyway (a request
+// larger than the whole budget runs alone).  Slots keep their buffers
+// between requests (releasing device memory waits for the whole device),
+// except that a slot whose request leaves all slots together above the
+// budget releases its own.
 class Server {
  public:
   Server(int slots, size_t budget) : budget_(budget), state_(slots, IDLE), fp_(slots, 0), bytes_(slots) {
@@ -168,7 +171,9 @@ class Server {
       bool shrink;
       {
         std::lock_guard<std::mutex> g(mu_);
-        shrink = bytes_[slot].load() > (int64_t)(budget_ / bytes_.size());
+        int64_t all = 0;
+        for (auto& b : bytes_) all += b.load();
+        shrink = all > (int64_t)budget_;
       }
       if (shrink) trim(slot);  // outside the lock: the slot is still WAITING/RUNNING for the others
       std::lock_guard<std::mutex> g(mu_);
@@ -192,11 +197,17 @@ class Server {
         if (stop_) return true;
         const size_t mine = std::max<size_t>((size_t)std::max<int64_t>(0, bytes_[slot].load()), fp);
         if (others_held(slot) + mine <= budget_) return true;
-        if (others_running(slot)) return false;
-        // nothing else runs: the idle slots give their buffers back
+        // idle slots give their buffers back, the largest holders first
+        std::vector<int> idle;
         for (size_t k = 0; k < state_.size(); k++)
-          if ((int)k != slot && state_[k] == IDLE && bytes_[k].load() > 0) trim((int)k);
-        return true;
+          if ((int)k != slot && state_[k] == IDLE && bytes_[k].load() > 0) idle.push_back((int)k);
+        std::sort(idle.begin(), idle.end(), [&](int a, int b) { return bytes_[a].load() > bytes_[b].load(); });
+        for (int k : idle) {
+          trim(k);
+          if (others_held(slot) + mine <= budget_) return true;
+        }
+        // still too large: wait for the running requests, or run alone
+        return !others_running(slot);
       });
       state_[slot] = RUNNING;
       fp_[slot] = fp;

@@ -149,8 +149,9 @@ def test_batched_256_parts_same_alone_and_oracle_sample():
 
 def test_tight_budget_releases_memory():
     """A 1 GB budget with one large request among small ones: the large one
-    runs alone, slots holding more than their share give their buffers back,
-    results stay bitwise the same, and the slots end within the budget."""
+    runs alone, idle slots give their buffers back when a request needs the
+    room, results stay bitwise the same, and the slots end within the
+    budget."""
     rng = np.random.default_rng(5)
     big = parts.hex_block(30, 16, 16, 0.3, 0.16, 0.16, seed=2, n_ext=8)
     reqs = [(big, 4, parts.random_loads(rng, big, 2), None)]
