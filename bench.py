@@ -253,8 +253,9 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="tune the factor panel width and solve split on the part before timing (default: the "
                          "library defaults, which the committed profiles were measured with)")
-    ap.add_argument("--only", choices=("mixed", "serving", "cpu"), default=None,
-                    help="run one side measurement alone and print it as a JSON line (for profiling)")
+    ap.add_argument("--only", choices=("stream", "mixed", "serving", "cpu"), default=None,
+                    help="run one measurement alone and print it as a JSON line (for profiling); 'stream' is "
+                         "the headline's pipelined part stream without the single-handle phases after it")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one plain device copy (torch.neg over N MiB of FP64) whose HBM "
                          "counters calibrate FETCH_SIZE/WRITE_SIZE of the profiled kernels")
@@ -328,6 +329,10 @@ def main():
     t0 = time.perf_counter()
     sizes = stream(a.steps)
     el = time.perf_counter() - t0
+    if a.only == "stream":
+        print(json.dumps({"stream": {"value": a.steps / el, "unit": "reductions/s", "steps": a.steps,
+                                     "ms_per_step": 1e3 * el / a.steps, "pipelines": npipe}}))
+        return
     R = Rs[0]
     phases = R.times()
     info = R.info()

@@ -1178,9 +1178,10 @@ int DeviceReducer::reduce(std::string& err) {
     int init[6] = {0, 0, 0, INT_MAX, INT_MAX, INT_MAX};
     HIPCHK(hipMemcpyAsync(d.piv.p, init, sizeof init, hipMemcpyHostToDevice, st));
   }
-  if (opt_.time_updates && d.upd_ev.empty()) {
-    d.upd_ev.resize(2 * upd_launches_);
-    for (auto& e : d.upd_ev) HIPCHK(hipEventCreate(&e));
+  if (opt_.time_updates && d.upd_ev.size() < 2 * (size_t)upd_launches_) {  // another schedule may launch more
+    const size_t have = d.upd_ev.size();
+    d.upd_ev.resize(2 * (size_t)upd_launches_);
+    for (size_t q = have; q < d.upd_ev.size(); q++) HIPCHK(hipEventCreate(&d.upd_ev[q]));
   }
   int64_t ui = 0;
   hipError_t fork_err = hipSuccess;  // first failure of a fork or join of the side stream
