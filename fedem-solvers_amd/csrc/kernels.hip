@@ -9,6 +9,7 @@
 // (contribution) matrix the parent adds in.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 #include "kernels.hpp"
 
 namespace fgpu {
@@ -649,7 +650,14 @@ __global__ __launch_bounds__(256) void k_trsm(const FTask* __restrict__ tasks, D
 // k0 = panel, kb = panel width (<= 128).  F[I,J] -= (L_I D) L_J^T over the
 // panel, staged through LDS in UCH-column chunks whose products accumulate
 // in the MFMA registers, so a 128-wide panel reads and writes the trailing
-// tile once for both halves.
+// tile once for both halves.  FP64: the accumulators start from the tile's
+// current values (loaded with the first chunk, so the two latencies
+// overlap) and the staged L_I D is negated, so the MFMA chain forms
+// F - sum (L_I D) L_J^T directly and the tile is only stored at the end.
+// FP32 (mixed precision) sums the products from zero and subtracts them
+// from the tile at the end: accumulating onto the tile through the FP32
+// MFMA left the factor too inexact for the FP64 refinement of a thin
+// 200x200 plate to converge.
 template <class T>
 __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks, DevSym S,
                                                 T* __restrict__ pool) {
@@ -666,8 +674,7 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
-  typename MF<T>::acc acc[2][2];
-  zero_acc(acc);
+  const int lr = lane & 15, lk = lane >> 4;
   T ri[UQ], rj[UQ], rd[UQ];
   // global loads of the chunk at c0 (zero beyond the panel)
   auto load = [&](int c0) {
@@ -682,20 +689,10 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
     }
   };
   load(0);
-  for (int c0 = 0; c0 < kb; c0 += UCH) {
-    const int kc = min(UCH, kb - c0);
-    if (c0 > 0) __syncthreads();  // the previous chunk's products are done with the tiles
-#pragma unroll
-    for (int q = 0; q < UQ; q++) {
-      const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
-      sI[i][m] = ri[q] * rd[q];
-      sJ[i][m] = rj[q];
-    }
-    __syncthreads();
-    if (c0 + UCH < kb) load(c0 + UCH);  // the next chunk's loads are in flight during the products
-    mfma_tile<T>(sI, sJ, (kc + 3) & ~3, acc, lane, wr, wc);
-  }
-  const int lr = lane & 15, lk = lane >> 4;
+  // FP64: the stored elements of the tile (lower triangle inside the
+  // front) seed the accumulators; the others start at zero, never stored
+  constexpr bool seed = std::is_same<T, double>::value;
+  typename MF<T>::acc acc[2][2];
 #pragma unroll
   for (int x = 0; x < 2; x++)
 #pragma unroll
@@ -704,7 +701,33 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
       for (int r = 0; r < 4; r++) {
         const int i = I0 + wr + 16 * x + lr;
         const int j = J0 + wc + 16 * y + MF<T>::col(lk, r);
-        if (i < ld && j < ld && i >= j) F[(int64_t)i + (int64_t)j * ld] -= acc[x][y][r];
+        acc[x][y][r] = (seed && i < ld && j < ld && i >= j) ? F[(int64_t)i + (int64_t)j * ld] : (T)0;
+      }
+  for (int c0 = 0; c0 < kb; c0 += UCH) {
+    const int kc = min(UCH, kb - c0);
+    if (c0 > 0) __syncthreads();  // the previous chunk's products are done with the tiles
+#pragma unroll
+    for (int q = 0; q < UQ; q++) {
+      const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
+      sI[i][m] = seed ? -(ri[q] * rd[q]) : ri[q] * rd[q];
+      sJ[i][m] = rj[q];
+    }
+    __syncthreads();
+    if (c0 + UCH < kb) load(c0 + UCH);  // the next chunk's loads are in flight during the products
+    mfma_tile<T>(sI, sJ, (kc + 3) & ~3, acc, lane, wr, wc);
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int i = I0 + wr + 16 * x + lr;
+        const int j = J0 + wc + 16 * y + MF<T>::col(lk, r);
+        if (i < ld && j < ld && i >= j) {
+          if (seed) F[(int64_t)i + (int64_t)j * ld] = acc[x][y][r];
+          else F[(int64_t)i + (int64_t)j * ld] -= acc[x][y][r];
+        }
       }
 }
 
