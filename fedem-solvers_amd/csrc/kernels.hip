@@ -662,8 +662,10 @@ template <class T>
 __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks, DevSym S,
                                                 T* __restrict__ pool) {
   constexpr int UCH = 16, UQ = 64 * UCH / 256;  // chunk width, loads per thread per operand
-  __shared__ T sI[64][UCH + 1];
-  __shared__ T sJ[64][UCH + 1];
+  // two LDS buffers: chunk c is multiplied from buffer c & 1 while chunk
+  // c + 1 is staged into the other, one barrier per chunk
+  __shared__ T sI[2][64][UCH + 1];
+  __shared__ T sJ[2][64][UCH + 1];
   const FTask t = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int sn = t.sn, k0 = t.k0, ld = S.nrow[sn];
   const int kb = t.kb;
@@ -703,18 +705,27 @@ __global__ __launch_bounds__(256) void k_update(const FTask* __restrict__ tasks,
         const int j = J0 + wc + 16 * y + MF<T>::col(lk, r);
         acc[x][y][r] = (seed && i < ld && j < ld && i >= j) ? F[(int64_t)i + (int64_t)j * ld] : (T)0;
       }
-  for (int c0 = 0; c0 < kb; c0 += UCH) {
-    const int kc = min(UCH, kb - c0);
-    if (c0 > 0) __syncthreads();  // the previous chunk's products are done with the tiles
+  auto stage = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < UQ; q++) {
       const int idx = tid + 256 * q, i = idx & 63, m = idx >> 6;
-      sI[i][m] = seed ? -(ri[q] * rd[q]) : ri[q] * rd[q];
-      sJ[i][m] = rj[q];
+      sI[buf][i][m] = seed ? -(ri[q] * rd[q]) : ri[q] * rd[q];
+      sJ[buf][i][m] = rj[q];
     }
-    __syncthreads();
-    if (c0 + UCH < kb) load(c0 + UCH);  // the next chunk's loads are in flight during the products
-    mfma_tile<T>(sI, sJ, (kc + 3) & ~3, acc, lane, wr, wc);
+  };
+  stage(0);
+  __syncthreads();
+  for (int c0 = 0, buf = 0; c0 < kb; c0 += UCH, buf ^= 1) {
+    const int kc = min(UCH, kb - c0);
+    const bool more = c0 + UCH < kb;
+    if (more) load(c0 + UCH);  // the next chunk's loads are in flight during the products
+    mfma_tile<T>(sI[buf], sJ[buf], (kc + 3) & ~3, acc, lane, wr, wc);
+    // the other buffer was last read by the products of the chunk before
+    // this one, which every wave finished before the previous barrier
+    if (more) {
+      stage(buf ^ 1);
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int x = 0; x < 2; x++)
