@@ -379,6 +379,10 @@ __device__ __forceinline__ void diag_block(const T* __restrict__ F, int ld, int 
                                            const DevSym& S, int colbase) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
+  // the singular-pivot test's |a_jj| of the block's columns, lane j of wave
+  // 0 holding column j: read once here, broadcast in the sweeps below (a
+  // load per pivot would put a memory latency on every step of the chain)
+  const double adv = (wv == 0 && lane < kb) ? S.adiag[colbase + lane] : 0.0;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     const int i = idx & 63, m = idx >> 6;
     sA[i][m] = (i < kb && m <= i) ? (double)F[(int64_t)(k0 + i) + (int64_t)(k0 + m) * ld] : (i == m ? 1.0 : 0.0);
@@ -401,7 +405,7 @@ __device__ __forceinline__ void diag_block(const T* __restrict__ F, int ld, int 
         double dj = lane_bcast(a[j], j);
         if (c0 + j < kb) {  // singular pivot: replaced, column flagged (same decision on every lane)
           const int col = colbase + c0 + j;
-          const double aj = S.adiag[col];
+          const double aj = lane_bcast(adv, c0 + j);
           if (!(fabs(dj) > S.tolf * fabs(aj))) {
             const bool truez = aj == 0.0 || fabs(dj) <= 1.0e3 * 2.220446049250313e-16 * fabs(aj);
             dj = aj != 0.0 ? fabs(aj) : *S.amax;
