@@ -1844,6 +1844,52 @@ __global__ __launch_bounds__(256) void k_sumsq2(int64_t n, const double* __restr
   }
 }
 
+// Column dot products of up to three pairs (A_k, B_k) of row-major n x nr
+// arrays: block b sums rows [b COLDOT_ROWS, (b+1) COLDOT_ROWS) of each
+// column in row order into part[(b * npair + k) * nr + c]; k_coldot_sum
+// then adds the blocks in block order (a fixed order: same bits every run).
+__global__ __launch_bounds__(256) void k_coldot_part(int64_t n, int nr, int npair, const double* __restrict__ A0,
+                                                     const double* __restrict__ B0, const double* __restrict__ A1,
+                                                     const double* __restrict__ B1, const double* __restrict__ A2,
+                                                     const double* __restrict__ B2, double* __restrict__ part) {
+  const int64_t r0 = (int64_t)blockIdx.x * COLDOT_ROWS, r1 = min(n, r0 + COLDOT_ROWS);
+  for (int c = threadIdx.x; c < nr; c += 256) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int64_t r = r0; r < r1; r++) {
+      const int64_t o = r * nr + c;
+      s0 += A0[o] * B0[o];
+      if (npair > 1) s1 += A1[o] * B1[o];
+      if (npair > 2) s2 += A2[o] * B2[o];
+    }
+    double* P = part + (int64_t)blockIdx.x * npair * nr + c;
+    P[0] = s0;
+    if (npair > 1) P[nr] = s1;
+    if (npair > 2) P[2 * nr] = s2;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_coldot_sum(int nb, int nr, int npair, const double* __restrict__ part,
+                                                    double* __restrict__ out) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= npair * nr) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += part[(int64_t)b * npair * nr + idx];
+  out[idx] = s;
+}
+
+// Y[r][c] = a[c] X[r][c] + b[c] Y[r][c] over n rows of row-major n x nr
+// arrays (a or b null: coefficient 1).
+__global__ __launch_bounds__(256) void k_col_axpby(int64_t n, int nr, const double* __restrict__ a,
+                                                   const double* __restrict__ X, const double* __restrict__ b,
+                                                   double* __restrict__ Y) {
+  const int64_t total = n * nr;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (int64_t)gridDim.x * 256) {
+    const int c = (int)(i % nr);
+    const double ya = b ? b[c] * Y[i] : Y[i];
+    Y[i] = (a ? a[c] * X[i] : X[i]) + ya;
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
@@ -2022,6 +2068,19 @@ void launch_recip(hipStream_t st, int n, const double* x, double* y) {
 }
 void launch_axpby(hipStream_t st, int64_t n, double a, const double* X, double b, double* Y) {
   if (n > 0) k_axpby<<<nblk(n, 256), 256, 0, st>>>(n, a, X, b, Y);
+}
+int coldot_blocks(int64_t n) { return (int)((n + COLDOT_ROWS - 1) / COLDOT_ROWS); }
+void launch_coldots(hipStream_t st, int64_t n, int nr, int npair, const double* A0, const double* B0,
+                    const double* A1, const double* B1, const double* A2, const double* B2, double* part,
+                    double* out) {
+  const int nb = coldot_blocks(n);
+  if (nb > 0) k_coldot_part<<<nb, 256, 0, st>>>(n, nr, npair, A0, B0, A1, B1, A2, B2, part);
+  k_coldot_sum<<<nblk((int64_t)npair * nr, 256), 256, 0, st>>>(nb, nr, npair, part, out);
+}
+void launch_col_axpby(hipStream_t st, int64_t n, int nr, const double* a, const double* X, const double* b,
+                      double* Y) {
+  const int64_t total = n * nr;
+  if (total > 0) k_col_axpby<<<(unsigned)std::min<int64_t>(nblk(total, 256), 8192), 256, 0, st>>>(n, nr, a, X, b, Y);
 }
 void launch_sumsq2(hipStream_t st, int64_t n, const double* x, const double* y, double* part) {
   k_sumsq2<<<SUMSQ_BLOCKS, 256, 0, st>>>(n, x, y, part);

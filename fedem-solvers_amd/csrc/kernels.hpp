@@ -128,6 +128,17 @@ void launch_scale_rc(hipStream_t st, int64_t nrows, int nrhs, const double* rs, 
 void launch_recip(hipStream_t st, int n, const double* x, double* y);
 void launch_axpby(hipStream_t st, int64_t n, double a, const double* X, double b, double* Y);
 void launch_sumsq2(hipStream_t st, int64_t n, const double* x, const double* y, double* part);
+// Column dot products out[k * nr + c] = sum_r A_k[r][c] B_k[r][c] of npair
+// (<= 3) pairs of row-major n x nr arrays, summed in a fixed order; part
+// holds coldot_blocks(n) * npair * nr doubles.
+constexpr int COLDOT_ROWS = 256;
+int coldot_blocks(int64_t n);
+void launch_coldots(hipStream_t st, int64_t n, int nr, int npair, const double* A0, const double* B0,
+                    const double* A1, const double* B1, const double* A2, const double* B2, double* part,
+                    double* out);
+// Y = diag-scaled columns: Y[r][c] = a[c] X[r][c] + b[c] Y[r][c] (a, b device arrays; null: 1)
+void launch_col_axpby(hipStream_t st, int64_t n, int nr, const double* a, const double* X, const double* b,
+                      double* Y);
 // element masses (x-translation mass of each element's mass blocks; nen
 // nodes of d DOFs) of one element type list, and their fixed-order total
 void launch_emass(hipStream_t st, int n, const int* elist, int nen, int d, const int64_t* eoff, const double* Mp,

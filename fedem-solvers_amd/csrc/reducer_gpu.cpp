@@ -214,12 +214,13 @@ struct DeviceReducer::Impl {
   DBuf<double> X, Z, Y, part, small, acc, grav, ccoef;
   DBuf<double> emass;  // nel element masses, then their total
   // mixed precision: FP32 fronts and pivot-block inverses, equilibration
-  // scale, refinement work (correction, residual, K X) and norm partials
+  // scale, refinement residual and norm partials
   bool mixed = false;
   bool m_lumped = false;  // only lumped-mass (ANDES4) elements: M is diagonal in a K-shaped pattern
   int solve_split = 128;
   DBuf<float> pool32, ipool32;
-  DBuf<double> dscale, dcs, Dl, Rr, Zr, irpart;
+  DBuf<double> dscale, dcs, Rr, irpart;
+  DBuf<double> cg_p, cg_q, cg_z, cg_t, cg_part, cg_dot, cg_co;  // conjugate-gradient refinement
   std::vector<double> s22;  // mixed: Schur complement from the refined B (n2 x n2, column-major)
   // component modes
   DBuf<double> Q, P, Wl, Zl, Yl, Cc, Phi, Tmp;
@@ -1064,11 +1065,117 @@ static void approx_solve(hipStream_t st, DeviceReducer::Impl& d, const SolvePlan
   launch_scale_rc(st, ndof1, sp.nrhs, d.dscale.p, nullptr, D);
 }
 
+// Mixed precision: conjugate gradients on K_ii X = b, one independent CG
+// per column, preconditioned by the FP32 factor (z = K~_ii^-1 r through
+// approx_solve), residuals and updates in FP64.  The stationary refinement
+// X += K~^-1 r contracts each step by the spread rho of the preconditioned
+// spectrum about 1; CG by about rho / (1 + sqrt(1 - rho^2)) and faster
+// where the spectrum clusters.  On entry X holds the starting solution and
+// R (neq x nr, internal rows) its residual b - K_ii X; X's boundary rows
+// are left alone.  A column whose curvature p^T K_ii p or r^T z is not
+// positive (an indefinite K_ii or factor) turns the remaining steps into
+// stationary refinement for all columns.  Stopping: refine_state on the
+// relative size of the step's correction.
+static int pcg_int(hipStream_t st, DeviceReducer::Impl& d, const Symbolic& y, const ReduceOptions& opt,
+                   const SolvePlan& sp, double* X, double* R, double* Yw, int* steps, const char* what,
+                   std::string& err) {
+  const int neq = y.neq, n1 = y.ndof1, nr = sp.nrhs;
+  const int64_t nall = (int64_t)neq * nr, nint = (int64_t)n1 * nr;
+  HIPCHK(d.cg_p.alloc(nall));
+  HIPCHK(d.cg_q.alloc(nall));
+  HIPCHK(d.cg_z.alloc(nall));
+  HIPCHK(d.cg_t.alloc(nall));
+  HIPCHK(d.cg_part.alloc((size_t)std::max(1, coldot_blocks(n1)) * 3 * nr));
+  HIPCHK(d.cg_dot.alloc((size_t)3 * nr));
+  HIPCHK(d.cg_co.alloc((size_t)3 * nr));
+  std::vector<double> h((size_t)3 * nr), rz(nr), xx(nr), co((size_t)3 * nr);
+  // column dot products of npair pairs over the internal rows, to h
+  auto dots = [&](int npair, const double* A0, const double* B0, const double* A1, const double* B1,
+                  const double* A2, const double* B2) -> int {
+    launch_coldots(st, n1, nr, npair, A0, B0, A1, B1, A2, B2, d.cg_part.p, d.cg_dot.p);
+    HIPCHK(hipMemcpyAsync(h.data(), d.cg_dot.p, sizeof(double) * npair * nr, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  };
+  // z = K~^-1 r (approx_solve scales its input in place: it gets a copy)
+  auto precond = [&]() -> int {
+    HIPCHK(hipMemcpyAsync(d.cg_t.p, R, sizeof(double) * nint, hipMemcpyDeviceToDevice, st));
+    approx_solve(st, d, sp, neq, n1, d.cg_t.p, Yw, d.cg_z.p);
+    return 0;
+  };
+  bool stationary = false;
+  double prev = INFINITY;
+  if (int rc = precond()) return rc;
+  if (int rc = dots(2, R, d.cg_z.p, X, X, nullptr, nullptr)) return rc;
+  for (int c = 0; c < nr; c++) {
+    rz[c] = h[c];
+    xx[c] = h[nr + c];  // |X_c|^2, carried through the updates for the stopping rule
+    if (!(rz[c] >= 0.0)) stationary = true;
+  }
+  // p = z (boundary rows zero, so K p has K_ii p in its internal rows)
+  HIPCHK(hipMemcpyAsync(d.cg_p.p, d.cg_z.p, sizeof(double) * nall, hipMemcpyDeviceToDevice, st));
+  for (int it = 0;; it++) {
+    double rel = 0.0;
+    if (stationary) {
+      // X += z; r = b - K_ii X = r - K_ii z
+      launch_axpby(st, nint, 1.0, d.cg_z.p, 1.0, X);
+      launch_spmm(st, neq, nr, d.rowptr.p, d.colind.p, d.Kv.p, d.cg_z.p, nr, d.cg_q.p, nr);
+      launch_axpby(st, nint, -1.0, d.cg_q.p, 1.0, R);
+      if (rel_correction(st, d, nint, d.cg_z.p, X, rel, err)) return -3;
+    } else {
+      launch_spmm(st, neq, nr, d.rowptr.p, d.colind.p, d.Kv.p, d.cg_p.p, nr, d.cg_q.p, nr);
+      if (int rc = dots(3, d.cg_p.p, d.cg_q.p, d.cg_p.p, d.cg_p.p, d.cg_p.p, X)) return rc;
+      double num = 0.0, den = 0.0;
+      for (int c = 0; c < nr; c++) {
+        const double pq = h[c], pp = h[nr + c], px = h[2 * nr + c];
+        double a = 0.0;
+        if (rz[c] > 0.0) {
+          if (pq > 0.0) a = rz[c] / pq;
+          else stationary = true;
+        }
+        co[c] = a;
+        co[nr + c] = -a;
+        num += a * a * pp;
+        xx[c] += a * (2.0 * px + a * pp);  // |X_c + a p_c|^2
+        den += xx[c];
+      }
+      if (stationary) {  // fall back from the current X and r: redo this step as a stationary one
+        it--;
+        continue;
+      }
+      HIPCHK(hipMemcpyAsync(d.cg_co.p, co.data(), sizeof(double) * 2 * nr, hipMemcpyHostToDevice, st));
+      launch_col_axpby(st, n1, nr, d.cg_co.p, d.cg_p.p, nullptr, X);           // X += a p
+      launch_col_axpby(st, n1, nr, d.cg_co.p + nr, d.cg_q.p, nullptr, R);      // r -= a K p
+      rel = den > 0.0 ? std::sqrt(num / den) : (num > 0.0 ? INFINITY : 0.0);
+    }
+    if (steps) *steps = it + 1;
+    if (std::getenv("FGPU_TRACE_REFINE"))  // FGPU_TRACE_REFINE=1: each step's correction on stderr
+      std::fprintf(stderr, "refine %s step %d: |dX|/|X| %.3e%s\n", what, it + 1, rel,
+                   stationary ? " (stationary)" : "");
+    const int state = refine_state(rel, prev, it + 1, opt, err, what);
+    if (state > 0) return 0;
+    if (state < 0) return -6;
+    prev = rel;
+    if (int rc = precond()) return rc;
+    if (stationary) continue;
+    if (int rc = dots(1, R, d.cg_z.p, nullptr, nullptr, nullptr, nullptr)) return rc;
+    for (int c = 0; c < nr; c++) {
+      const double rzn = h[c];
+      if (!(rzn >= 0.0)) stationary = true;
+      co[2 * nr + c] = rz[c] > 0.0 ? rzn / rz[c] : 0.0;
+      rz[c] = rzn;
+    }
+    if (stationary) continue;
+    HIPCHK(hipMemcpyAsync(d.cg_co.p + 2 * nr, co.data() + 2 * nr, sizeof(double) * nr, hipMemcpyHostToDevice, st));
+    launch_col_axpby(st, n1, nr, nullptr, d.cg_z.p, d.cg_co.p + 2 * nr, d.cg_p.p);  // p = z + beta p
+  }
+}
+
 // X = K_ii^{-1} R on the internal rows (X's boundary rows zero), R and X
 // neq x nrhs row-major in perm order; Yw is neq x nrhs work.  FP64 factor:
-// one forward and one backward pass.  Mixed precision: iterative refinement,
-// X += K_ii^{-1}~ (R - K_ii X) with the residual from the FP64 CSR matrix,
-// until the correction is below ir_tol relative to X.
+// one forward and one backward pass.  Mixed precision: conjugate gradients
+// preconditioned by the FP32 factor with residuals from the FP64 CSR
+// matrix (pcg_int), until the correction is below ir_tol relative to X.
 static int solve_int(hipStream_t st, DeviceReducer::Impl& d, const Symbolic& y, const ReduceOptions& opt,
                      const SolvePlan& sp, const double* R, double* Yw, double* X, int* steps, std::string& err) {
   const int neq = y.neq, n1 = y.ndof1, nr = sp.nrhs;
@@ -1080,27 +1187,11 @@ static int solve_int(hipStream_t st, DeviceReducer::Impl& d, const Symbolic& y, 
     if (steps) *steps = 0;
     return 0;
   }
-  HIPCHK(d.Dl.alloc(nall));
+  // conjugate gradients from X = 0 (residual R)
   HIPCHK(d.Rr.alloc(nall));
-  HIPCHK(d.Zr.alloc(nall));
   HIPCHK(hipMemcpyAsync(d.Rr.p, R, sizeof(double) * nall, hipMemcpyDeviceToDevice, st));
   launch_fill(st, X, nall, 0.0);
-  double prev = INFINITY;
-  for (int it = 0;; it++) {
-    approx_solve(st, d, sp, neq, n1, d.Rr.p, Yw, d.Dl.p);
-    launch_axpby(st, nint, 1.0, d.Dl.p, 1.0, X);
-    double rel = 0.0;
-    if (rel_correction(st, d, nint, d.Dl.p, X, rel, err)) return -3;
-    if (steps) *steps = it + 1;
-    const int state = refine_state(rel, prev, it + 1, opt, err, "a K_ii solve");
-    if (state > 0) return 0;
-    if (state < 0) return -6;
-    prev = rel;
-    // residual R - K_ii X (X's boundary rows are zero, so K X has the K_ii X rows)
-    launch_spmm(st, neq, nr, d.rowptr.p, d.colind.p, d.Kv.p, X, nr, d.Zr.p, nr);
-    HIPCHK(hipMemcpyAsync(d.Rr.p, R, sizeof(double) * nint, hipMemcpyDeviceToDevice, st));
-    launch_axpby(st, nint, -1.0, d.Zr.p, 1.0, d.Rr.p);
-  }
+  return pcg_int(st, d, y, opt, sp, X, d.Rr.p, Yw, steps, "a K_ii solve", err);
 }
 
 static double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -1285,27 +1376,14 @@ int DeviceReducer::reduce(std::string& err) {
       // refinement against the FP64 matrix: Z = K [B; I] has the residual
       // -(K_ii B + K_ib) in its internal rows and K_bb + K_bi B in its
       // boundary rows (the Schur complement once B has converged)
-      HIPCHK(d.Dl.alloc((size_t)neq * n2));
-      bool conv = false;
+      // conjugate gradients on K_ii B = -K_ib from this B: the internal
+      // rows of K [B; I] are -(the residual), its boundary rows K_bb + K_bi B
+      // (the Schur complement once B has converged)
+      launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Kv.p, d.X.p, n2, d.Z.p, n2);
+      launch_axpby(st, (int64_t)n1 * n2, -1.0, d.Z.p, 0.0, d.Z.p);
       int steps = 0;
-      double prev = INFINITY;
-      for (;;) {
-        launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Kv.p, d.X.p, n2, d.Z.p, n2);
-        if (conv) break;
-        double rres = 0.0;
-        if (std::getenv("FGPU_TRACE_REFINE") && rel_correction(st, d, (int64_t)n1 * n2, d.Z.p, d.X.p, rres, err))
-          return -3;
-        approx_solve(st, d, *pb, neq, n1, d.Z.p, d.Y.p, d.Dl.p);
-        launch_axpby(st, (int64_t)n1 * n2, -1.0, d.Dl.p, 1.0, d.X.p);
-        double rel = 0.0;
-        if (rel_correction(st, d, (int64_t)n1 * n2, d.Dl.p, d.X.p, rel, err)) return -3;
-        steps++;
-        if (std::getenv("FGPU_TRACE_REFINE")) std::fprintf(stderr, "refine B step %d: |R|/|X| %.3e  |D|/|X| %.3e\n", steps, rres, rel);
-        const int state = refine_state(rel, prev, steps, opt_, err, "B");
-        if (state < 0) return -6;
-        conv = state > 0;
-        prev = rel;
-      }
+      if (int rc = pcg_int(st, d, y, opt_, *pb, d.X.p, d.Z.p, d.Y.p, &steps, "B", err)) return rc;
+      launch_spmm(st, neq, n2, d.rowptr.p, d.colind.p, d.Kv.p, d.X.p, n2, d.Z.p, n2);
       times_.refine_steps = steps;
       std::vector<double> zb((size_t)n2 * n2);
       HIPCHK(hipMemcpyAsync(zb.data(), d.Z.p + (size_t)n1 * n2, sizeof(double) * zb.size(), hipMemcpyDeviceToHost,
@@ -2043,7 +2121,7 @@ size_t DeviceReducer::footprint_bytes() const {
   b += 20 * sam_.ceq_dof.size() + 4 * sam_.ceq_ptr.size();                          // constraint equations
   b += 8 * (size_t)y.neq * (3 * (size_t)y.ndof2 + 2 * mcap + 8 * (size_t)bs);     // B solves and Lanczos basis
   b += 8 * 2 * wrows * rhs;                                                        // solve work rows
-  if (opt_.mixed) b += 8 * 3 * (size_t)y.neq * rhs;                                // refinement work
+  if (opt_.mixed) b += 8 * 5 * (size_t)y.neq * rhs;                                // refinement work
   return b;
 }
 

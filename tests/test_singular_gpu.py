@@ -84,6 +84,28 @@ def test_indefinite_block_negative_pivots_warn_only():
         assert rel(a, b) <= 1e-10
 
 
+def test_indefinite_block_mixed_precision():
+    """mixed precision on the indefinite block: conjugate gradients need a
+    positive definite K_ii, so the refinement turns stationary at the first
+    non-positive curvature (or r^T z) and still meets the mixed path's
+    1e-8 against the oracle"""
+    p = indefinite_block()
+    print("indefinite block, mixed precision", flush=True)
+    R = gpu.GpuReducer()
+    try:
+        R.set_part(p)
+        R.set_option("mixedPrecision", 1)
+        R.analyze(vgi=True)
+        R.reduce()
+        SK, SM, G = R.reduced()
+        B, vgi = R.B(), R.vgi()
+    finally:
+        R.close()
+    o = orc.reduce(p, want_vgi=True)
+    for a, b in ((SK, o["SK"]), (SM, o["SM"]), (G, o["G"]), (B, o["B"]), (vgi, o["vgi"])):
+        assert rel(a, b) <= 1e-8
+
+
 def test_dropin_reports_singular_nodes(tmp_path):
     p = hinged_blocks()
     nas = tmp_path / "hinge.nas"
