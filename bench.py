@@ -189,9 +189,10 @@ def mixed_plate(seed, reps=3):
     return out
 
 
-def _drain(srv, reqs, gaps=None):
+def _drain(srv, reqs, gaps=None, keep=0):
     """Submit reqs (back to back, or at the given inter-arrival gaps) and
-    wait for all; returns (wall seconds, latencies in ms)."""
+    wait for all; returns (wall seconds, latencies in ms, the results of the
+    first `keep` requests, how many ran in mixed precision)."""
     t0 = time.perf_counter()
     ids, due = [], t0
     for k, r in enumerate(reqs):
@@ -200,11 +201,15 @@ def _drain(srv, reqs, gaps=None):
             while time.perf_counter() < due:
                 time.sleep(min(0.001, max(0.0, due - time.perf_counter())))
         ids.append(srv.submit(*r))
-    lat = []
-    for i in ids:
-        lat.append(srv.result(i)["latency_ms"])
+    lat, kept, nmixed = [], [], 0
+    for k, i in enumerate(ids):
+        res = srv.result(i)
+        lat.append(res["latency_ms"])
+        nmixed += int(res["mixed"])
+        if k < keep:
+            kept.append(res)
         srv.release(i)
-    return time.perf_counter() - t0, np.array(lat)
+    return time.perf_counter() - t0, np.array(lat), kept, nmixed
 
 
 def served(n_req, slots, seed):
@@ -217,25 +222,67 @@ def served(n_req, slots, seed):
     from fedempy_amd.server import Server
     rng = np.random.default_rng(seed)
     reqs = [parts.served_request(rng) for _ in range(n_req)]
-    cap = {}
-    for k in (1, slots):
+    nkeep = 24
+    cap, kept = {}, {}
+
+    def capacity(k, mixed=False):
         srv = Server(slots=k)
+        if mixed:
+            srv.set_option("mixedPrecision", 1)
         for r in reqs[:4]:  # warm the slots
             srv.result(srv.submit(*r))
-        el, _ = _drain(srv, reqs)
-        cap[k] = n_req / el
+        el, _, res, nmixed = _drain(srv, reqs, keep=nkeep)
         srv.close()
+        return n_req / el, res, nmixed
+
+    def poisson(rate, mixed=False):
+        srv = Server(slots=slots)
+        if mixed:
+            srv.set_option("mixedPrecision", 1)
+        for r in reqs[:4]:
+            srv.result(srv.submit(*r))
+        el, lat, _, _ = _drain(srv, reqs, rng.exponential(1.0 / rate, n_req))
+        srv.close()
+        return n_req / el, lat
+
+    for k in (1, slots):
+        cap[k], kept[k], _ = capacity(k)
     rate = 0.7 * cap[slots]
-    srv = Server(slots=slots)
-    for r in reqs[:4]:
-        srv.result(srv.submit(*r))
-    el, lat = _drain(srv, reqs, rng.exponential(1.0 / rate, n_req))
-    srv.close()
+    served_rate, lat = poisson(rate)
+    # the same mix with mixedPrecision set on the server: capacity, latency
+    # at 70 % of that capacity, and the error of the first requests'
+    # mode-independent outputs against their FP64 results
+    cap_m, kept_m, nmixed = capacity(slots, mixed=True)
+    rate_m = 0.7 * cap_m
+    served_m, lat_m = poisson(rate_m, mixed=True)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    err = {"SK": 0.0, "SM": 0.0, "G": 0.0, "Fred": 0.0, "Ui": 0.0}
+    for a, m in zip(kept[slots], kept_m):
+        if not m["mixed"]:
+            continue  # redone in FP64
+        n2 = a["ndof2"]
+        err["SK"] = max(err["SK"], rel(m["SK"][:n2, :n2], a["SK"][:n2, :n2]))
+        err["SM"] = max(err["SM"], rel(m["SM"][:n2, :n2], a["SM"][:n2, :n2]))
+        err["G"] = max(err["G"], rel(m["G"][:n2], a["G"][:n2]))
+        if a["Fred"].size:
+            err["Fred"] = max(err["Fred"], rel(m["Fred"][:n2], a["Fred"][:n2]))
+            err["Ui"] = max(err["Ui"], rel(m["Ui"], a["Ui"]))
     return {"workload": "%d requests, DOF lognormal median 20k cap 300k (mean %.0f), 1-8 load cases + "
                         "recovery each, FP64" % (n_req, np.mean([6 * len(r[0]["xyz"]) for r in reqs])),
             "slots": slots, "requests_per_s": cap[slots], "plain_one_stream_requests_per_s": cap[1],
-            "speedup": cap[slots] / cap[1], "poisson_rate_per_s": rate, "poisson_requests_per_s": n_req / el,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+            "speedup": cap[slots] / cap[1], "poisson_rate_per_s": rate, "poisson_requests_per_s": served_rate,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mixed_precision": {
+                "requests_per_s": cap_m, "vs_plain_fp64_one_stream": cap_m / cap[1],
+                "vs_fp64_same_slots": cap_m / cap[slots], "poisson_rate_per_s": rate_m,
+                "poisson_requests_per_s": served_m, "p50_ms": float(np.percentile(lat_m, 50)),
+                "p99_ms": float(np.percentile(lat_m, 99)),
+                "requests_in_mixed_precision": nmixed,
+                "requests_redone_in_fp64": n_req - nmixed,
+                "max_rel_error_vs_fp64": err,
+                "error_sample": "those of the first %d requests that ran in mixed precision; SK, SM, G: boundary "
+                                "(Guyan) blocks, Fred: boundary rows, Ui: all (the mode blocks depend on the "
+                                "modes' basis)" % nkeep}}
 
 
 def main():

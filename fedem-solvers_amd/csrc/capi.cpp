@@ -26,6 +26,29 @@ struct fgpu_reducer {
     return -99;                          \
   }
 
+namespace fgpu {
+// The numeric options of fgpu_set_option (and of the server's requests).
+int set_reduce_option(ReduceOptions& o, const std::string& n, double value) {
+  if (n == "lanczosBlock") o.lanczos_block = (int)value;
+  else if (n == "lanczosMaxVectors") o.lanczos_maxvec = (int)value;
+  else if (n == "lanczosTol") o.tol_eig = value;
+  else if (n == "eigenshift") o.eig_shift = value;
+  else if (n == "seed") o.seed = (uint64_t)value;
+  else if (n == "leafNodes") o.sym.leaf_nodes = (int)value;
+  else if (n == "timeUpdates") o.time_updates = value != 0.0;
+  else if (n == "mixedPrecision") o.mixed = value != 0.0;
+  else if (n == "solveSplitK") o.solve_split = (int)value;
+  else if (n == "factorLookahead") o.factor_lookahead = value != 0.0;
+  else if (n == "factorPanel") o.factor_panel = (int)value;
+  else if (n == "refineTol") o.ir_tol = value;
+  else if (n == "refineMaxSteps") o.ir_maxit = (int)value;
+  else if (n == "tolFactorize") o.tol_factorize = value;
+  else if (n == "singularityHandler") o.sing_handler = (int)value;
+  else return -1;
+  return 0;
+}
+}  // namespace fgpu
+
 extern "C" {
 
 int fgpu_create(fgpu_reducer** h) {
@@ -160,24 +183,10 @@ int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags) {
 
 int fgpu_set_option(fgpu_reducer* h, const char* name, double value) {
   if (!h || !name) return -1;
-  std::string n(name);
-  fgpu::ReduceOptions& o = h->opts;
-  if (n == "lanczosBlock") o.lanczos_block = (int)value;
-  else if (n == "lanczosMaxVectors") o.lanczos_maxvec = (int)value;
-  else if (n == "lanczosTol") o.tol_eig = value;
-  else if (n == "eigenshift") o.eig_shift = value;
-  else if (n == "seed") o.seed = (uint64_t)value;
-  else if (n == "leafNodes") o.sym.leaf_nodes = (int)value;
-  else if (n == "timeUpdates") o.time_updates = value != 0.0;
-  else if (n == "mixedPrecision") o.mixed = value != 0.0;
-  else if (n == "solveSplitK") o.solve_split = (int)value;
-  else if (n == "factorLookahead") o.factor_lookahead = value != 0.0;
-  else if (n == "factorPanel") o.factor_panel = (int)value;
-  else if (n == "refineTol") o.ir_tol = value;
-  else if (n == "refineMaxSteps") o.ir_maxit = (int)value;
-  else if (n == "tolFactorize") o.tol_factorize = value;
-  else if (n == "singularityHandler") o.sing_handler = (int)value;
-  else { h->err = "fgpu_set_option: unknown option " + n; return -1; }
+  if (fgpu::set_reduce_option(h->opts, name, value)) {
+    h->err = std::string("fgpu_set_option: unknown option ") + name;
+    return -1;
+  }
   return 0;
 }
 

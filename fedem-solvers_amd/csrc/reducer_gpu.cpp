@@ -1037,12 +1037,14 @@ static int rel_correction(hipStream_t st, DeviceReducer::Impl& d, int64_t n, con
 // the latest correction: done when it is below tol, or when it no longer
 // halves (the residual has reached FP64 rounding level, where the
 // correction is noise of the size cond(K) * eps64); failure when it stays
-// large or grows past the step limit.
+// large past the step limit, or above 0.1 after 12 steps.
 static int refine_state(double rel, double prev, int steps, const ReduceOptions& opt, std::string& err,
                         const char* what) {
   if (rel <= opt.ir_tol) return 1;
   if (steps >= 2 && rel > 0.5 * prev && rel < 1e-6) return 1;
-  if (steps >= opt.ir_maxit || !std::isfinite(rel)) {
+  // a correction still above 0.1 after 12 steps has contracted by less than
+  // 0.83 a step: it cannot reach ir_tol within the step limit, so stop now
+  if (steps >= opt.ir_maxit || !std::isfinite(rel) || (steps >= 12 && rel > 0.1)) {
     char buf[160];
     std::snprintf(buf, sizeof buf, "mixed-precision iterative refinement of %s did not converge "
                   "(correction %.3e after %d steps)", what, rel, steps);

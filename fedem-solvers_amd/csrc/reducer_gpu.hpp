@@ -47,6 +47,9 @@ struct ReduceOptions {
   SymbolicOptions sym;
 };
 
+// Sets a numeric option by its fgpu_set_option name; -1 for an unknown name.
+int set_reduce_option(ReduceOptions& o, const std::string& name, double value);
+
 struct PhaseTimes {  // milliseconds, measured with HIP events on the work stream
   double elements = 0, assembly = 0, factor = 0, bsolve = 0, modes = 0, mass = 0, gravity = 0, total = 0;
   double update_kernel_ms = 0;   // sum of trailing-update (MFMA) launches, when timed

@@ -161,6 +161,13 @@ class Server {
       int rc;
       try {
         rc = run(slot, *r, err);
+        if (rc == -6 && r->opt.mixed) {
+          // the mixed-precision refinement did not converge (an FP32 factor
+          // too inexact for this part): the request is redone in FP64
+          r->opt.mixed = false;
+          err.clear();
+          rc = run(slot, *r, err);
+        }
       } catch (const std::exception& e) {
         err = std::string("request failed: ") + e.what();
         rc = -99;
@@ -275,6 +282,7 @@ class Server {
 
 struct fgpu_server {
   std::unique_ptr<fgpu::Server> s;
+  fgpu::ReduceOptions base;  // options of the requests submitted from now on
 };
 
 extern "C" {
@@ -312,6 +320,7 @@ int fgpu_server_submit(fgpu_server* s, int nnod, const double* xyz, const int* s
   for (int e = 0; e < nel; e++)
     if (p.eptr[e + 1] - p.eptr[e] != fgpu::elem_nodes(p.etype[e])) return -2;
   p.eprop.assign(eprop, eprop + (size_t)fgpu::NPROP * nel);
+  r->opt = s->base;
   r->opt.ngen = ngen > 0 ? ngen : 0;
   // no stream capture in the server: another slot's allocations could
   // synchronise the device in the middle of a capture
@@ -327,6 +336,17 @@ int fgpu_server_submit(fgpu_server* s, int nnod, const double* xyz, const int* s
     if (ub && nub > 0) r->ub.assign(ub, ub + (size_t)nub * nlc);
   }
   return s->s->submit(std::move(r));
+}
+
+int fgpu_server_set_option(fgpu_server* s, const char* name, double value) {
+  if (!s || !name) return -1;
+  return fgpu::set_reduce_option(s->base, name, value);
+}
+
+int fgpu_server_precision(fgpu_server* s, int id) {
+  fgpu::Request* r = s ? s->s->find_done(id) : nullptr;
+  if (!r) return -1;
+  return r->opt.mixed ? 1 : 0;
 }
 
 int fgpu_server_wait(fgpu_server* s, int id) {

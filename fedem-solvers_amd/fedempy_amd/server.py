@@ -21,7 +21,9 @@ def _load():
         lib.fgpu_server_destroy.argtypes = [P]
         lib.fgpu_server_destroy.restype = None
         lib.fgpu_server_submit.argtypes = [P, I, P, P, I, P, P, P, P, I, I, P, P, P, P, I, P]
+        lib.fgpu_server_set_option.argtypes = [P, ctypes.c_char_p, D]
         lib.fgpu_server_wait.argtypes = [P, I]
+        lib.fgpu_server_precision.argtypes = [P, I]
         lib.fgpu_server_sizes.argtypes = [P, I, P]
         lib.fgpu_server_result.argtypes = [P, I, P, P, P, P, P, P]
         lib.fgpu_server_error.argtypes = [P, I, ctypes.c_char_p, I]
@@ -70,6 +72,11 @@ class Server:
         self._keep[rid] = (arrs, lptr, lnode, ldof, lval, ubf)
         return rid
 
+    def set_option(self, name, value):
+        """numeric option (fgpu_set_option's names) of the requests submitted from now on"""
+        if self.lib.fgpu_server_set_option(self.h, name.encode(), float(value)) != 0:
+            raise ValueError("unknown option " + name)
+
     def wait(self, rid):
         return self.lib.fgpu_server_wait(self.h, rid)
 
@@ -87,7 +94,8 @@ class Server:
         t = np.zeros(4)
         self.lib.fgpu_server_result(self.h, rid, _p(SK), _p(SM), _p(G), _p(Fr), _p(Ui), _p(t))
         return dict(SK=SK, SM=SM, G=G, Fred=Fr[:, :nlc], Ui=Ui[:, :nlc], ndof2=n2,
-                    queue_ms=t[0], service_ms=t[1], latency_ms=t[2], footprint=t[3])
+                    queue_ms=t[0], service_ms=t[1], latency_ms=t[2], footprint=t[3],
+                    mixed=self.lib.fgpu_server_precision(self.h, rid) == 1)
 
     def device_bytes(self):
         """device memory the slots' buffers hold now"""

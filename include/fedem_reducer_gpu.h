@@ -206,10 +206,12 @@ int fgpu_recover(fgpu_reducer* h, int nrhs, const double* Ub, const double* Fi, 
  * the batched/served form of the same reduction.  `slots` worker threads,
  * each with its own reducer and HIP stream, take requests in arrival order;
  * a request starts when its estimated device footprint, with the buffers
- * every slot holds (slots keep them between requests), fits the budget; a
- * slot holding more than budget/slots after a request releases them, and a
- * request that does not fit while nothing else runs starts once the idle
- * slots have released theirs (a request above the whole budget runs alone).  After the reduction each load case gives the
+ * every slot holds (slots keep them between requests), fits the budget;
+ * otherwise idle slots release their buffers, the largest holders first,
+ * and a request that still does not fit waits for running ones (a request
+ * above the whole budget runs alone).  A slot releases its own buffers
+ * after a request only when all slots together hold more than the budget.
+ * After the reduction each load case gives the
  * reduced load vector (as fgpu_reduce_loads) and the recovered internal
  * displacements (as fgpu_recover with Fi = the case's internal loads).
  * Results of a request are bitwise the same alone or among others. */
@@ -224,8 +226,16 @@ void fgpu_server_destroy(fgpu_server* s);
 int fgpu_server_submit(fgpu_server* s, int nnod, const double* xyz, const int* status, int nel, const int* melcon,
                        const int* mpmnpc, const int* mmnpc, const double* eprop, int ngen, int nlc, const int* lptr,
                        const int* lnode, const int* ldof, const double* lval, int nub, const double* ub);
+/* Numeric option (the names of fgpu_set_option, e.g. "mixedPrecision") for
+ * the requests submitted after the call; not to be called while another
+ * thread submits.  0, or -1 for an unknown name. */
+int fgpu_server_set_option(fgpu_server* s, const char* name, double value);
 /* Blocks until request id is done; its status (0 or negative). */
 int fgpu_server_wait(fgpu_server* s, int id);
+/* Arithmetic a finished request ran in: 1 mixed precision, 0 FP64 (a
+ * mixed-precision request whose refinement did not converge is redone in
+ * FP64 on its slot), -1 no such finished request. */
+int fgpu_server_precision(fgpu_server* s, int id);
 /* sizes[0..4] = ndim, ndof1, ndof2, nlc, nev of a finished request. */
 int fgpu_server_sizes(fgpu_server* s, int id, int* sizes);
 /* SK, SM (ndim^2), G (ndim x 3), Fred (ndim x nlc), Ui (ndof1 x nlc);

@@ -76,6 +76,34 @@ def test_server_matches_oracle():
     assert rel(r["Ui"], Ui) <= 1e-10
 
 
+def test_server_mixed_precision_requests():
+    """requests served with mixedPrecision set on the server: the Guyan
+    (boundary) blocks of SK, SM and G, the boundary rows of the reduced loads
+    and the recovered internal displacements within the mixed path's 1e-8
+    of the same requests in FP64 (the mode blocks depend on the modes' basis,
+    which test_mixed_gpu.py checks against the oracle)"""
+    reqs = requests(11, 6)
+    out = {}
+    for mode in ("fp64", "mixed"):
+        print("server: %s" % mode, flush=True)
+        srv = Server(slots=4)
+        if mode == "mixed":
+            srv.set_option("mixedPrecision", 1)
+        ids = [srv.submit(part, ngen, loads, ub) for part, ngen, loads, ub in reqs]
+        out[mode] = [srv.result(i) for i in ids]
+        srv.close()
+    rel = lambda a, b: np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+    for a, m in zip(out["fp64"], out["mixed"]):
+        n2 = a["ndof2"]
+        for k in ("SK", "SM"):
+            assert rel(m[k][:n2, :n2], a[k][:n2, :n2]) <= 1e-8, k
+        assert rel(m["G"][:n2], a["G"][:n2]) <= 1e-8
+        assert rel(m["Fred"][:n2], a["Fred"][:n2]) <= 1e-8
+        assert rel(m["Ui"], a["Ui"]) <= 1e-8
+    with pytest.raises(ValueError):
+        Server(slots=1).set_option("noSuchOption", 1)
+
+
 def test_server_heavy_mix_same_as_alone():
     """Sixteen slots busy at once (workgroups of one launch start late under
     the load of the other streams): every request still gives the bits of
