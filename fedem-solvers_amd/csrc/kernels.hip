@@ -1474,15 +1474,20 @@ __global__ __launch_bounds__(256) void k_xty_mfma(int n, int crows, int na, int 
 
 // One wave per output element: lane l sums chunks l, l+64, ... then a fixed
 // butterfly combines the lanes (same order on every run).
+// C = the sum of the chunk partials; acc (optional) = C, or acc + C with
+// acc_add.
 __global__ __launch_bounds__(256) void k_xty_sum(int nchunk, int nab, const double* __restrict__ part,
-                                                 double* __restrict__ C) {
+                                                 double* __restrict__ C, double* __restrict__ acc, int acc_add) {
   const int e = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (e >= nab) return;
   double s = 0.0;
   for (int c = lane; c < nchunk; c += 64) s += part[(int64_t)c * nab + e];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) C[e] = s;
+  if (lane == 0) {
+    C[e] = s;
+    if (acc) acc[e] = acc_add ? acc[e] + s : s;
+  }
 }
 
 // Sign convention of the component modes: for column i of Phi (rows in
@@ -2002,18 +2007,22 @@ void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* 
 }
 int xty_chunks(int n) { return (n + XTY_ROWS - 1) / XTY_ROWS; }
 void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Y, int ldy,
-                double* part, double* C) {
+                double* part, double* C, double* acc, bool acc_add) {
   if (na <= 0 || nb <= 0) return;
   // wide products (many 64x64 output tiles) take longer row chunks: fewer
   // partial matrices to write and add, still thousands of workgroups
   const int tiles = ((na + 63) / 64) * ((nb + 63) / 64);
   const int crows = tiles >= 16 ? 8 * XTY_ROWS : XTY_ROWS;
   const int nch = (n + crows - 1) / crows;
-  if (nch == 0) { launch_fill(st, C, (int64_t)na * nb, 0.0); return; }
+  if (nch == 0) {
+    launch_fill(st, C, (int64_t)na * nb, 0.0);
+    if (acc && !acc_add) launch_fill(st, acc, (int64_t)na * nb, 0.0);
+    return;
+  }
   dim3 g(nch, (na + 63) / 64, (nb + 63) / 64);
   if (tiles >= 16) k_xty_mfma<<<g, 256, 0, st>>>(n, crows, na, nb, X, ldx, Y, ldy, part);
   else k_xty_part<<<g, 256, 0, st>>>(n, crows, na, nb, X, ldx, Y, ldy, part);
-  k_xty_sum<<<nblk((int64_t)na * nb, 4), 256, 0, st>>>(nch, na * nb, part, C);
+  k_xty_sum<<<nblk((int64_t)na * nb, 4), 256, 0, st>>>(nch, na * nb, part, C, acc, acc_add ? 1 : 0);
 }
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta) {

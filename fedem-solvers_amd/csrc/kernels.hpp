@@ -100,8 +100,10 @@ void launch_spmm(hipStream_t st, int n, int nrhs, const int* rowptr, const int* 
                  const double* X, int ldx, double* Z, int ldz,
                  bool skip_zeros = false);
 int xty_chunks(int n);
+// C = X^T Y (X n x na, Y n x nb, row-major; C na x nb row-major); acc
+// (optional) = C, or acc + C with acc_add
 void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Y, int ldy,
-                double* part, double* C);
+                double* part, double* C, double* acc = nullptr, bool acc_add = false);
 // Y = alpha Y + beta X S (X n x na, S na x nb, row-major).  For nb <= 16, Y
 // may be X itself (same leading dimension): each row tile reads its rows
 // before it writes them.

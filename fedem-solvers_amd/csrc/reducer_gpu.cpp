@@ -1599,11 +1599,10 @@ int DeviceReducer::reduce(std::string& err) {
       else HIPCHK(run_graph(st, d.g_lanczos,
                        {pl, pl->tasks.p, pl->part.p, d.Tmp.p, d.Yl.p, d.Wl.p, d.W.p, d.V.p, d.pool.p, d.ipool.p,
                         d.S.foff, d.S.woff, d.S.dpiv}, solve_step));
-      launch_fill(st, d.Cacc.p, (int64_t)mj * bs, 0.0);
       for (int pass = 0; pass < 2; pass++) {
-        launch_xty(st, neq, mj, bs, d.P.p, mcap, d.Wl.p, bs, d.part.p, d.Cc.p);
+        // coefficients of the pass into Cc, their running sum into Cacc
+        launch_xty(st, neq, mj, bs, d.P.p, mcap, d.Wl.p, bs, d.part.p, d.Cc.p, d.Cacc.p, pass > 0);
         launch_gemm_small(st, neq, mj, bs, d.Q.p, mcap, d.Cc.p, d.Wl.p, bs, 1.0, -1.0);
-        launch_axpby(st, (int64_t)mj * bs, 1.0, d.Cc.p, 1.0, d.Cacc.p);
       }
       HIPCHK(hipMemcpyAsync(d.hC.p + (size_t)par * mcap * bs, d.Cacc.p, sizeof(double) * mj * bs,
                             hipMemcpyDeviceToHost, st));
