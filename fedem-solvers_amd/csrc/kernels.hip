@@ -1575,9 +1575,15 @@ __global__ __launch_bounds__(256) void k_gemm_small(int n, int na, int nb, const
 // workgroup takes 64 rows of X, staged through LDS in 32-column chunks read
 // in row order (coalesced), the next chunk's loads in flight during the
 // current chunk's products; wave w accumulates rows 16 w .. 16 w + 15.
-__global__ __launch_bounds__(256) void k_gemm_rows16(int n, int na, int nb, const double* __restrict__ X, int ldx,
-                                                     const double* __restrict__ Sm, double* __restrict__ Y,
-                                                     int ldy, double alpha, double beta) {
+// blockIdx.y = 1 takes the second pair (X2, Y2) with the same S.
+__global__ __launch_bounds__(256) void k_gemm_rows16(int n, int na, int nb, const double* __restrict__ X1, int ldx1,
+                                                     const double* __restrict__ Sm, double* __restrict__ Y1,
+                                                     int ldy1, double alpha, double beta,
+                                                     const double* __restrict__ X2, int ldx2,
+                                                     double* __restrict__ Y2, int ldy2) {
+  const double* __restrict__ X = blockIdx.y ? X2 : X1;
+  double* __restrict__ Y = blockIdx.y ? Y2 : Y1;
+  const int ldx = blockIdx.y ? ldx2 : ldx1, ldy = blockIdx.y ? ldy2 : ldy1;
   __shared__ double sX[64][33];
   __shared__ double sS[16][33];  // S chunk transposed: sS[j][k]
   const int r0 = blockIdx.x * 64;
@@ -2024,11 +2030,24 @@ void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx,
   else k_xty_part<<<g, 256, 0, st>>>(n, crows, na, nb, X, ldx, Y, ldy, part);
   k_xty_sum<<<nblk((int64_t)na * nb, 4), 256, 0, st>>>(nch, na * nb, part, C, acc, acc_add ? 1 : 0);
 }
+void launch_gemm_small2(hipStream_t st, int n, int na, int nb, const double* X1, int ldx1, double* Y1, int ldy1,
+                        const double* X2, int ldx2, double* Y2, int ldy2, const double* Sm, double alpha,
+                        double beta) {
+  if (n <= 0 || nb <= 0) return;
+  if (nb > 16) {
+    launch_gemm_small(st, n, na, nb, X1, ldx1, Sm, Y1, ldy1, alpha, beta);
+    launch_gemm_small(st, n, na, nb, X2, ldx2, Sm, Y2, ldy2, alpha, beta);
+    return;
+  }
+  k_gemm_rows16<<<dim3((unsigned)((n + 63) / 64), 2), 256, 0, st>>>(n, na, nb, X1, ldx1, Sm, Y1, ldy1, alpha, beta,
+                                                                     X2, ldx2, Y2, ldy2);
+}
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta) {
   if (n <= 0 || nb <= 0) return;
   if (nb <= 16) {  // MFMA over 64-row tiles
-    k_gemm_rows16<<<(unsigned)((n + 63) / 64), 256, 0, st>>>(n, na, nb, X, ldx, Sm, Y, ldy, alpha, beta);
+    k_gemm_rows16<<<(unsigned)((n + 63) / 64), 256, 0, st>>>(n, na, nb, X, ldx, Sm, Y, ldy, alpha, beta, nullptr,
+                                                             0, nullptr, 0);
     return;
   }
   // S is staged in LDS in row chunks of at most 64 KiB; chunk k adds

@@ -109,6 +109,10 @@ void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx,
 // before it writes them.
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta);
+// the same for two pairs (X1 -> Y1, X2 -> Y2) with one S, in one launch
+void launch_gemm_small2(hipStream_t st, int n, int na, int nb, const double* X1, int ldx1, double* Y1, int ldy1,
+                        const double* X2, int ldx2, double* Y2, int ldy2, const double* Sm, double alpha,
+                        double beta);
 // sign convention of the modes Phi (nrows x nev row-major): largest-magnitude
 // internal component (rows p1[0..n1-1], equation order) made positive;
 // sign: nev * (1 + 3 * MODE_SIGN_GROUPS) doubles of scratch

@@ -1494,8 +1494,7 @@ int DeviceReducer::reduce(std::string& err) {
         double* qo = pass ? d.Q.p + (size_t)j * bs : (inplace ? d.Wl.p : d.Tmp.p);
         double* po = pass ? d.P.p + (size_t)j * bs : (inplace ? d.Zl.p : d.Yl.p);
         const int ldo = pass ? mcap : bs;
-        launch_gemm_small(st, neq, bs, bs, d.Wl.p, bs, d.small.p, qo, ldo, 0.0, 1.0);
-        launch_gemm_small(st, neq, bs, bs, d.Zl.p, bs, d.small.p, po, ldo, 0.0, 1.0);
+        launch_gemm_small2(st, neq, bs, bs, d.Wl.p, bs, qo, ldo, d.Zl.p, bs, po, ldo, d.small.p, 0.0, 1.0);
         if (pass == 0 && !inplace) {
           HIPCHK(hipMemcpyAsync(d.Wl.p, d.Tmp.p, sizeof(double) * neq * bs, hipMemcpyDeviceToDevice, st));
           HIPCHK(hipMemcpyAsync(d.Zl.p, d.Yl.p, sizeof(double) * neq * bs, hipMemcpyDeviceToDevice, st));
