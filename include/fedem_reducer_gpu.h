@@ -118,9 +118,12 @@ int fgpu_analyze(fgpu_reducer* h, int ngen, int neval, int flags);
  * panel is factored; default 0),
  * "mixedPrecision" (1: FP32 factor of the equilibrated K with FP32 MFMA
  * trailing and Schur updates and FP64 diagonal blocks; B and every K_ii
- * solve refined against the FP64 matrix), "refineTol" (relative correction
- * that ends the refinement, default 1e-14), "refineMaxSteps" (default 40;
- * reduce returns -6 when the refinement does not converge). */
+ * solve refined against the FP64 matrix by conjugate gradients
+ * preconditioned by the FP32 factor, stationary refinement where K_ii is
+ * indefinite), "refineTol" (relative correction that ends the refinement,
+ * default 1e-14), "refineMaxSteps" (default 40, and a correction still above
+ * 0.1 after 12 steps ends it; reduce returns -6 when the refinement does not
+ * converge). */
 int fgpu_set_option(fgpu_reducer* h, const char* name, double value);
 
 /* Constraint equations of the dependent DOFs in SAM's layout (samModule.f90:
