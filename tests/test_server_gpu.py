@@ -104,6 +104,27 @@ def test_server_mixed_precision_requests():
         Server(slots=1).set_option("noSuchOption", 1)
 
 
+def test_server_redoes_unconverged_mixed_requests_in_fp64():
+    """a mixed-precision request whose refinement does not converge (forced
+    here by a step limit of 1) is redone in FP64 on its slot: flagged as
+    FP64 and bitwise the same as the request submitted in FP64"""
+    reqs = requests(13, 3)
+    out = {}
+    for mode in ("fp64", "capped"):
+        print("server: %s" % mode, flush=True)
+        srv = Server(slots=2)
+        if mode == "capped":
+            srv.set_option("mixedPrecision", 1)
+            srv.set_option("refineMaxSteps", 1)
+        ids = [srv.submit(part, ngen, loads, ub) for part, ngen, loads, ub in reqs]
+        out[mode] = [srv.result(i) for i in ids]
+        srv.close()
+    for a, m in zip(out["fp64"], out["capped"]):
+        assert not a["mixed"] and not m["mixed"]
+        for k in ("SK", "SM", "G", "Fred", "Ui"):
+            assert np.array_equal(a[k], m[k]), k
+
+
 def test_server_heavy_mix_same_as_alone():
     """Sixteen slots busy at once (workgroups of one launch start late under
     the load of the other streams): every request still gives the bits of
