@@ -1575,9 +1575,10 @@ __global__ __launch_bounds__(256) void k_gemm_small(int n, int na, int nb, const
 // workgroup takes 64 rows of X, staged through LDS in 32-column chunks read
 // in row order (coalesced), the next chunk's loads in flight during the
 // current chunk's products; wave w accumulates rows 16 w .. 16 w + 15.
-// blockIdx.y = 1 takes the second pair (X2, Y2) with the same S.
+// blockIdx.y = 1 takes the second pair (X2, Y2) with the same S.  S rows
+// are lds apart (the nb columns may be a slice of a wider S).
 __global__ __launch_bounds__(256) void k_gemm_rows16(int n, int na, int nb, const double* __restrict__ X1, int ldx1,
-                                                     const double* __restrict__ Sm, double* __restrict__ Y1,
+                                                     const double* __restrict__ Sm, int lds, double* __restrict__ Y1,
                                                      int ldy1, double alpha, double beta,
                                                      const double* __restrict__ X2, int ldx2,
                                                      double* __restrict__ Y2, int ldy2) {
@@ -1599,7 +1600,7 @@ __global__ __launch_bounds__(256) void k_gemm_rows16(int n, int na, int nb, cons
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       const int idx = tid + 256 * q, k = idx >> 4, j = idx & 15;
-      rs[q] = (kc + k < na && j < nb) ? Sm[(int64_t)(kc + k) * nb + j] : 0.0;
+      rs[q] = (kc + k < na && j < nb) ? Sm[(int64_t)(kc + k) * lds + j] : 0.0;
     }
   };
   if (na > 0) load(0);
@@ -2039,15 +2040,16 @@ void launch_gemm_small2(hipStream_t st, int n, int na, int nb, const double* X1,
     launch_gemm_small(st, n, na, nb, X2, ldx2, Sm, Y2, ldy2, alpha, beta);
     return;
   }
-  k_gemm_rows16<<<dim3((unsigned)((n + 63) / 64), 2), 256, 0, st>>>(n, na, nb, X1, ldx1, Sm, Y1, ldy1, alpha, beta,
-                                                                     X2, ldx2, Y2, ldy2);
+  k_gemm_rows16<<<dim3((unsigned)((n + 63) / 64), 2), 256, 0, st>>>(n, na, nb, X1, ldx1, Sm, nb, Y1, ldy1, alpha,
+                                                                     beta, X2, ldx2, Y2, ldy2);
 }
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta) {
   if (n <= 0 || nb <= 0) return;
-  if (nb <= 16) {  // MFMA over 64-row tiles
-    k_gemm_rows16<<<(unsigned)((n + 63) / 64), 256, 0, st>>>(n, na, nb, X, ldx, Sm, Y, ldy, alpha, beta, nullptr,
-                                                             0, nullptr, 0);
+  if (nb <= 64) {  // MFMA over 64-row tiles, 16 columns of S per launch
+    for (int c0 = 0; c0 < nb; c0 += 16)
+      k_gemm_rows16<<<(unsigned)((n + 63) / 64), 256, 0, st>>>(n, na, std::min(16, nb - c0), X, ldx, Sm + c0, nb,
+                                                               Y + c0, ldy, alpha, beta, nullptr, 0, nullptr, 0);
     return;
   }
   // S is staged in LDS in row chunks of at most 64 KiB; chunk k adds

@@ -104,9 +104,10 @@ int xty_chunks(int n);
 // (optional) = C, or acc + C with acc_add
 void launch_xty(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Y, int ldy,
                 double* part, double* C, double* acc = nullptr, bool acc_add = false);
-// Y = alpha Y + beta X S (X n x na, S na x nb, row-major).  For nb <= 16, Y
-// may be X itself (same leading dimension): each row tile reads its rows
-// before it writes them.
+// Y = alpha Y + beta X S (X n x na, S na x nb, row-major); up to 64 columns
+// on FP64 MFMA, 16 columns of S per launch.  For nb <= 16, Y may be X itself
+// (same leading dimension): each row tile reads its rows before it writes
+// them.
 void launch_gemm_small(hipStream_t st, int n, int na, int nb, const double* X, int ldx, const double* Sm,
                        double* Y, int ldy, double alpha, double beta);
 // the same for two pairs (X1 -> Y1, X2 -> Y2) with one S, in one launch
