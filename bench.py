@@ -136,22 +136,28 @@ def batched(n_parts, slots, seed):
     from fedempy_amd.server import Server
     rng = np.random.default_rng(seed)
     reqs = [parts.random_part(rng) for _ in range(n_parts)]
-    out = {}
-    for k in (1, slots):
+    out, nmixed = {}, 0
+    for k, mixed in ((1, False), (slots, False), (slots, True)):
         srv = Server(slots=k)
+        if mixed:
+            srv.set_option("mixedPrecision", 1)
         for p, g in reqs[:4]:  # warm the slots (first-touch allocations)
             srv.result(srv.submit(p, g))
         t0 = time.perf_counter()
         ids = [srv.submit(p, g) for p, g in reqs]
         for i in ids:
-            srv.result(i)
+            nmixed += int(srv.result(i)["mixed"])
             srv.release(i)
-        out[k] = n_parts / (time.perf_counter() - t0)
+        out[(k, mixed)] = n_parts / (time.perf_counter() - t0)
         srv.close()
     return {"workload": "%d parts: shell plates 10-80 squared, HEX8 blocks 6-24 cubed, beam frames 50-500 "
                         "elements, 2-16 external nodes, 0-10 modes, FP64" % n_parts,
-            "slots": slots, "parts_per_s": out[slots], "plain_one_stream_parts_per_s": out[1],
-            "speedup": out[slots] / out[1]}
+            "slots": slots, "parts_per_s": out[(slots, False)], "plain_one_stream_parts_per_s": out[(1, False)],
+            "speedup": out[(slots, False)] / out[(1, False)],
+            "mixed_precision": {"parts_per_s": out[(slots, True)],
+                                "vs_plain_fp64_one_stream": out[(slots, True)] / out[(1, False)],
+                                "parts_in_mixed_precision": nmixed,
+                                "parts_redone_in_fp64": n_parts - nmixed}}
 
 
 def mixed_plate(seed, reps=3):
