@@ -1823,9 +1823,10 @@ __global__ void k_recip(int n, const double* __restrict__ x, double* __restrict_
 }
 
 // Y[g] = a * X[g] + b * Y[g] over the first n entries
-__global__ void k_axpby(int64_t n, double a, const double* __restrict__ X, double b, double* __restrict__ Y) {
+// Y = a X + b Y (Y not read when b = 0; X and Y may be the same array)
+__global__ void k_axpby(int64_t n, double a, const double* X, double b, double* Y) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < n) Y[g] = a * X[g] + b * Y[g];
+  if (g < n) Y[g] = b == 0.0 ? a * X[g] : a * X[g] + b * Y[g];
 }
 
 // part[blk] = (sum x^2, sum y^2) over the block's slice of the first n
