@@ -1576,14 +1576,14 @@ __global__ __launch_bounds__(256) void k_gemm_small(int n, int na, int nb, const
 // in row order (coalesced), the next chunk's loads in flight during the
 // current chunk's products; wave w accumulates rows 16 w .. 16 w + 15.
 // blockIdx.y = 1 takes the second pair (X2, Y2) with the same S.  S rows
-// are lds apart (the nb columns may be a slice of a wider S).
-__global__ __launch_bounds__(256) void k_gemm_rows16(int n, int na, int nb, const double* __restrict__ X1, int ldx1,
-                                                     const double* __restrict__ Sm, int lds, double* __restrict__ Y1,
-                                                     int ldy1, double alpha, double beta,
-                                                     const double* __restrict__ X2, int ldx2,
-                                                     double* __restrict__ Y2, int ldy2) {
-  const double* __restrict__ X = blockIdx.y ? X2 : X1;
-  double* __restrict__ Y = blockIdx.y ? Y2 : Y1;
+// are lds apart (the nb columns may be a slice of a wider S).  Y may be X
+// (in place: a workgroup has read its rows of X before it writes them).
+__global__ __launch_bounds__(256) void k_gemm_rows16(int n, int na, int nb, const double* X1, int ldx1,
+                                                     const double* __restrict__ Sm, int lds, double* Y1, int ldy1,
+                                                     double alpha, double beta, const double* X2, int ldx2,
+                                                     double* Y2, int ldy2) {
+  const double* X = blockIdx.y ? X2 : X1;
+  double* Y = blockIdx.y ? Y2 : Y1;
   const int ldx = blockIdx.y ? ldx2 : ldx1, ldy = blockIdx.y ? ldy2 : ldy1;
   __shared__ double sX[64][33];
   __shared__ double sS[16][33];  // S chunk transposed: sS[j][k]
